@@ -1,0 +1,204 @@
+/*
+ * physim_gpu: MI355X (gfx950) gravity and integration engine behind the
+ * physim element API. Plain C types only; every call returns a pg_status
+ * and pg_last_error() holds the message of the last failure on the calling
+ * thread.
+ *
+ * Reference interfaces replaced (the reference's hot path is single-threaded
+ * FP64 Rust on the CPU):
+ *   - pg_transform_aos            AstroOctreeElement::transform /
+ *                                 AstroElement::transform /
+ *                                 SimpleAstroElement::transform
+ *                                 (astro/src/transformers.rs:123-160,32-69,219-244),
+ *                                 called through the C-ABI transform vtable
+ *                                 (physim-core/src/plugin/transform.rs:85-104).
+ *   - pg_sim_step                 one pipeline iteration of
+ *                                 synths(none) -> Verlet::integrate(acc_fn =
+ *                                 that transform) -> state = new_state
+ *                                 (physim-core/src/pipeline.rs:143-182,
+ *                                 integrators/src/verlet.rs:85-108), with the
+ *                                 state resident on the device.
+ *   - pg_sim_upload / _download   the Vec<Entity> hand-over of
+ *                                 pipeline.rs:100-107,173,179.
+ *   - pg_gen_cube / _star /       RandomCube / SingleStar / Plummer
+ *     _plummer_ref                (astro/src/initialisers.rs:81-107,145-191,293-330).
+ * Everything else (batching, serving, virial Plummer, energy) has no
+ * reference counterpart and says so where declared.
+ */
+#ifndef PHYSIM_GPU_H
+#define PHYSIM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "physim_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pg_status {
+    PG_OK = 0,
+    PG_ERR_HIP = 1,        /* a HIP runtime call failed */
+    PG_ERR_ARG = 2,        /* invalid argument */
+    PG_ERR_NOMEM = 3,      /* device allocation failed */
+    PG_ERR_BUDGET = 4,     /* request exceeds the engine's HBM budget */
+    PG_ERR_DEPTH = 5,      /* two bodies share every tree level the key holds */
+    PG_ERR_COINCIDENT = 6, /* bodies within 1e-9 on every axis (reference merge rule) */
+    PG_ERR_NONFINITE = 7,  /* NaN or infinite coordinate */
+    PG_ERR_STATE = 8,      /* call out of order */
+    PG_ERR_INTERNAL = 9,   /* a device-side bounded wait expired or a capacity was hit */
+} pg_status;
+
+typedef enum pg_kind {
+    PG_KIND_DIRECT = 0,   /* simple_astro: direct O(N^2) sum */
+    PG_KIND_QUADTREE = 2, /* astro: Barnes-Hut, 4-way (x, y) tree */
+    PG_KIND_OCTREE = 3,   /* astro2: Barnes-Hut, 8-way tree */
+} pg_kind;
+
+typedef enum pg_precision {
+    PG_PREC_MIXED = 0, /* FP32 pair forces, compensated FP32 per-body sums,
+                          FP64 positions/differences/tree/criterion/integrator */
+    PG_PREC_FP64 = 1,  /* the reference's FP64 arithmetic throughout */
+} pg_precision;
+
+typedef struct pg_params {
+    int32_t kind;      /* pg_kind */
+    int32_t precision; /* pg_precision */
+    double theta;      /* opening parameter (reference default 1.0) */
+    double e;          /* softening added to r^2 (reference default 1.0; abs taken) */
+    double dt;         /* Stormer time step */
+} pg_params;
+
+typedef struct pg_engine pg_engine;
+typedef struct pg_sim pg_sim;
+
+const char* pg_last_error(void);
+const char* pg_version(void);
+
+/* ---- engine ----------------------------------------------------------- */
+/* One engine per device; owns one HIP stream and the shared scratch of the
+ * batched step. hbm_budget_bytes = 0 means 90% of the device's memory. */
+int pg_engine_create(int device, uint64_t hbm_budget_bytes, pg_engine** out);
+void pg_engine_destroy(pg_engine* eng);
+/* Bytes of device memory the engine holds / its budget. */
+int pg_engine_memory(pg_engine* eng, uint64_t* used, uint64_t* budget);
+/* Device memory a simulation of n bodies needs (state + its share of
+ * scratch), used for admission against the budget. */
+uint64_t pg_sim_bytes(uint64_t n, int32_t kind);
+/* Capture repeated steps in hipGraphs (default on). */
+int pg_engine_set_graphs(pg_engine* eng, int enable);
+/* Per-kernel HIP-event timing of every launch (eager mode, no graphs). */
+int pg_engine_set_profiling(pg_engine* eng, int enable);
+/* Accumulated per-kernel times. names: '\n'-separated kernel names
+ * written into `names` (cap bytes); ms[i], launches[i] per kernel. Returns
+ * the kernel count in *count. */
+int pg_engine_profile_read(pg_engine* eng, char* names, size_t cap, double* ms,
+                           uint64_t* launches, uint32_t max_kernels, uint32_t* count);
+int pg_engine_profile_reset(pg_engine* eng);
+int pg_engine_synchronize(pg_engine* eng);
+
+/* ---- simulations ------------------------------------------------------ */
+int pg_sim_create(pg_engine* eng, uint64_t n, const pg_params* params, pg_sim** out);
+void pg_sim_destroy(pg_sim* sim);
+int pg_sim_set_params(pg_sim* sim, const pg_params* params);
+/* Host AoS -> device SoA. Clears the Stormer history, so the next step is
+ * the Taylor first step (verlet.rs:102-103). */
+int pg_sim_upload(pg_sim* sim, const Entity* state, uint64_t n);
+/* Device -> host AoS in upload order; radius, id and fixed are those
+ * uploaded, as verlet copies them (verlet.rs:41,72). */
+int pg_sim_download(pg_sim* sim, Entity* out, uint64_t n);
+/* Advance `steps` pipeline iterations on the device. */
+int pg_sim_step(pg_sim* sim, uint32_t steps);
+/* Accelerations of the current device state (no integration), FP64, n*3
+ * values in upload order. */
+int pg_sim_accel(pg_sim* sim, double* acc_xyz);
+/* FP64 kinetic, potential (force-consistent, O(N^2)), total energy and
+ * momentum of the current device state: out = {K, V, E, Px, Py, Pz}. No
+ * reference counterpart (the reference's energysink is not force-consistent). */
+int pg_sim_energy(pg_sim* sim, double out[6]);
+
+typedef struct pg_stats {
+    uint64_t steps_done;
+    uint64_t nodes;        /* tree nodes of the last build */
+    uint64_t interactions; /* interaction-list entries of the last force pass */
+    uint32_t max_level;    /* deepest leaf level of the last build */
+    uint32_t key_levels;   /* levels the Morton key holds */
+    uint32_t error;        /* sticky device-side pg_status of this sim */
+    uint32_t reserved;
+} pg_stats;
+int pg_sim_stats(pg_sim* sim, pg_stats* out);
+/* Count interactions per body in the force pass (debug; off by default). */
+int pg_sim_set_counting(pg_sim* sim, int enable);
+/* Per-body interaction-list length of the last force pass (upload order). */
+int pg_sim_interaction_counts(pg_sim* sim, int64_t* counts);
+/* Tree of the last build in preorder, children in octant order:
+ * rec[4k..] = {level, octant in parent (-1 root), leaf body (upload index) or
+ * -1, first sorted position}; fl[8k..] = {mass, com x, y, z, centre x, y, z,
+ * half-width}. *count receives the node count (arrays may be NULL). */
+int pg_sim_tree(pg_sim* sim, int64_t* rec, double* fl, uint64_t cap, uint64_t* count);
+/* Morton keys of the last build in sorted order and the body permutation
+ * (upload index of each sorted position). */
+int pg_sim_morton(pg_sim* sim, uint64_t* keys, uint32_t* perm, uint64_t n);
+
+/* ---- batching (no reference counterpart) ------------------------------- */
+/* Advance several simulations together: one launch sequence per step covers
+ * all of them. All must share kind and precision; theta, e, dt and n may
+ * differ. Each result is bit-identical to stepping that simulation alone. */
+int pg_batch_step(pg_engine* eng, pg_sim* const* sims, uint32_t count, uint32_t steps);
+
+/* ---- transform (plugin path) ------------------------------------------- */
+/* acc[i] += acceleration of body i under params (kind, theta, e; dt unused).
+ * Host AoS in, host AoS out; the device holds nothing between calls except
+ * reusable scratch. */
+int pg_transform_aos(pg_engine* eng, const pg_params* params, const Entity* state, uint64_t n,
+                     Acceleration* acc);
+
+/* ---- serving (no reference counterpart) -------------------------------- */
+typedef struct pg_server pg_server;
+typedef struct pg_request_info {
+    uint64_t id;
+    uint64_t n;
+    uint32_t steps;
+    uint32_t status;       /* pg_status of the request */
+    double submit_s;       /* engine clock, seconds */
+    double admit_s;
+    double complete_s;
+} pg_request_info;
+/* max_active: simulations stepped together; 0 = unlimited (budget only). */
+int pg_server_create(pg_engine* eng, uint32_t max_active, pg_server** out);
+void pg_server_destroy(pg_server* srv);
+/* Queue a whole simulation; copies `state`. */
+int pg_server_submit(pg_server* srv, const Entity* state, uint64_t n, const pg_params* params,
+                     uint32_t steps, uint64_t* request_id);
+/* Admit queued requests that fit, advance every active simulation by up to
+ * `max_steps` co-batched steps, retire finished ones. *active / *queued
+ * receive the counts after the tick. */
+int pg_server_tick(pg_server* srv, uint32_t max_steps, uint32_t* active, uint32_t* queued);
+/* Result of a completed request (state after its steps); frees it. */
+int pg_server_take(pg_server* srv, uint64_t request_id, Entity* out, uint64_t n,
+                   pg_request_info* info);
+int pg_server_info(pg_server* srv, uint64_t request_id, pg_request_info* info);
+double pg_server_clock(pg_server* srv);
+
+/* ---- initial conditions ------------------------------------------------ */
+/* RandomCube::create_entities (astro/src/initialisers.rs:81-107). */
+int pg_gen_cube(uint64_t n, uint64_t seed, double spin, double size, const double centre[3],
+                double mass, uint64_t id, Entity* out);
+/* SingleStar (astro/src/initialisers.rs:145-191). */
+int pg_gen_star(double x, double y, double z, double vx, double vy, double vz, double radius,
+                double mass, uint64_t id, int fixed, Entity* out);
+/* Plummer::create_entities (astro/src/initialisers.rs:293-330). */
+int pg_gen_plummer_ref(uint64_t n, uint64_t seed, double mass, double a, const double centre[3],
+                       const double v[3], double spin, uint64_t id, Entity* out);
+/* Isotropic Plummer sphere in virial equilibrium (Plummer 1911 radii,
+ * Aarseth-Henon-Wielen 1974 speeds), centre-of-mass frame shifted to
+ * centre / v. No reference counterpart. */
+int pg_gen_plummer_virial(uint64_t n, uint64_t seed, double mass, double a,
+                          const double centre[3], const double v[3], uint64_t id, Entity* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYSIM_GPU_H */

@@ -1,0 +1,1003 @@
+// physim_gpu engine: device memory, batched step sequencing, hipGraph replay,
+// per-kernel event timing and host-boundary conversions. Kernels live in
+// pg_kernels.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pg_common.h"
+#include "pg_engine.h"
+#include "pg_kernels.h"
+
+using namespace pg;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+int pg_set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+extern "C" const char* pg_last_error(void) { return g_err.c_str(); }
+extern "C" const char* pg_version(void) { return "physim_gpu 0.1.0 (gfx950)"; }
+
+#define HIP_TRY(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return pg_set_error(PG_ERR_HIP, "%s failed: %s (%s:%d)", #call,                 \
+                                hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+#define PG_TRY(call)              \
+    do {                          \
+        int rc_ = (call);         \
+        if (rc_ != PG_OK) return rc_; \
+    } while (0)
+
+// ------------------------------------------------------------ engine types
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+struct pg_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t sims_bytes = 0;
+    uint64_t scratch_bytes = 0;
+    std::recursive_mutex mu;
+    bool graphs = true;
+    bool profiling = false;
+
+    // batch scratch
+    uint64_t cap_bodies = 0, cap_nodes = 0;
+    uint32_t cap_segs = 0, cap_blocks = 0;
+    void* scratch = nullptr;
+    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* first_node = nullptr;
+    uint8_t* lo_level = nullptr;
+    uint64_t* sort_status = nullptr;
+    uint64_t* scan_status = nullptr;
+    NodeHot* hot = nullptr;
+    NodeCold* cold = nullptr;
+    double* acc_out = nullptr;
+    SegDesc* desc[2] = {nullptr, nullptr};
+    uint64_t* seg_off = nullptr;
+    uint32_t* blk_seg = nullptr;
+    uint32_t* blk_tile = nullptr;
+    StepCounters* sc = nullptr;
+    uint32_t* err_global = nullptr;
+
+    // host mirrors
+    std::vector<SegDesc> h_desc[2];
+    std::vector<uint64_t> h_seg_off;
+    std::vector<uint32_t> h_blk_seg, h_blk_tile;
+
+    // graph cache (two entries: the two parities of an odd step count)
+    struct GraphEntry {
+        std::vector<uint64_t> sig;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<GraphEntry> graph_cache;
+
+    // per-kernel timing
+    std::vector<EventPair> ev_pool;
+    std::vector<std::pair<const char*, EventPair>> ev_pending;
+    std::map<std::string, std::pair<double, uint64_t>> prof;
+
+    // last build (for tree / Morton export)
+    pg_sim* last_sim = nullptr;
+    uint64_t last_total = 0;
+    int last_passes = 0;
+    int last_dim = 3;
+    int last_lmax = 21;
+
+    // transient simulation of the transform path
+    pg_sim* xform = nullptr;
+};
+
+struct pg_sim {
+    pg_engine* eng = nullptr;
+    uint64_t n = 0;
+    pg_params p{};
+    void* block = nullptr;
+    uint64_t bytes = 0;
+    double *x[2], *y[2], *z[2], *px[2], *py[2], *pz[2], *vx[2], *vy[2], *vz[2], *m[2];
+    uint8_t* fix[2];
+    uint32_t* id[2];
+    Entity* aos = nullptr;
+    SegState* st = nullptr;
+    int64_t* counts = nullptr;
+    bool counting = false;
+    int cur = 0;
+    bool uploaded = false;
+};
+
+// ---------------------------------------------------------------- sizing
+static inline int key_dim(int kind) { return kind == PG_KIND_QUADTREE ? 2 : 3; }
+
+static inline int ceil_log2(uint32_t v) {
+    int b = 0;
+    while ((1u << b) < v) ++b;
+    return b;
+}
+
+static inline int key_lmax(int dim, int segbits) {
+    int l = (64 - segbits) / dim;
+    int cap = dim == 3 ? 21 : 32;
+    return l < cap ? l : cap;
+}
+
+static inline uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
+
+static uint64_t state_bytes(uint64_t n) {
+    // two SoA copies (9 doubles + mass + fixed + id), the uploaded AoS, counts, SegState
+    return 2 * (align256(n * 8) * 10 + align256(n) + align256(n * 4)) + align256(n * 80) +
+           align256(n * 8) + 256;
+}
+
+static uint64_t nodes_bound(uint64_t n, int dim, int lmax) {
+    (void)dim;
+    return n + (n * (uint64_t)lmax + 1) / 2 + 1;
+}
+
+extern "C" uint64_t pg_sim_bytes(uint64_t n, int32_t kind) {
+    const int dim = key_dim(kind);
+    const uint64_t scratch =
+        kind == PG_KIND_DIRECT
+            ? n * 24
+            : n * (8 * 2 + 4 * 2 + 4 + 1 + 24) + nodes_bound(n, dim, key_lmax(dim, 0)) *
+                                                      (sizeof(NodeHot) + sizeof(NodeCold));
+    return state_bytes(n) + scratch;
+}
+
+// --------------------------------------------------------------- engine
+extern "C" int pg_engine_create(int device, uint64_t budget, pg_engine** out) {
+    if (!out) return pg_set_error(PG_ERR_ARG, "out is NULL");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return pg_set_error(PG_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    pg_engine* e = new pg_engine();
+    e->device = device;
+    hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete e;
+        return pg_set_error(PG_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he));
+    }
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    e->budget = budget ? budget : (uint64_t)(0.9 * (double)totalb);
+    *out = e;
+    return PG_OK;
+}
+
+static void free_scratch(pg_engine* e) {
+    for (auto& g : e->graph_cache)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    e->graph_cache.clear();
+    if (e->scratch) (void)hipFree(e->scratch);
+    e->scratch = nullptr;
+    e->scratch_bytes = 0;
+    e->cap_bodies = e->cap_nodes = 0;
+    e->cap_segs = e->cap_blocks = 0;
+}
+
+extern "C" void pg_engine_destroy(pg_engine* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->xform) pg_sim_destroy(e->xform);
+    (void)hipStreamSynchronize(e->stream);
+    free_scratch(e);
+    for (auto& ev : e->ev_pool) {
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    for (auto& ev : e->ev_pending) {
+        (void)hipEventDestroy(ev.second.a);
+        (void)hipEventDestroy(ev.second.b);
+    }
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+extern "C" int pg_engine_memory(pg_engine* e, uint64_t* used, uint64_t* budget) {
+    if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    if (used) *used = e->sims_bytes + e->scratch_bytes;
+    if (budget) *budget = e->budget;
+    return PG_OK;
+}
+
+extern "C" int pg_engine_set_graphs(pg_engine* e, int enable) {
+    if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    e->graphs = enable != 0;
+    return PG_OK;
+}
+
+extern "C" int pg_engine_set_profiling(pg_engine* e, int enable) {
+    if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    e->profiling = enable != 0;
+    return PG_OK;
+}
+
+extern "C" int pg_engine_profile_reset(pg_engine* e) {
+    if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    e->prof.clear();
+    return PG_OK;
+}
+
+extern "C" int pg_engine_profile_read(pg_engine* e, char* names, size_t cap, double* ms,
+                                      uint64_t* launches, uint32_t max_kernels, uint32_t* count) {
+    if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    std::string all;
+    uint32_t k = 0;
+    for (auto& kv : e->prof) {
+        if (k < max_kernels) {
+            if (ms) ms[k] = kv.second.first;
+            if (launches) launches[k] = kv.second.second;
+            all += kv.first;
+            all += '\n';
+        }
+        ++k;
+    }
+    if (names && cap) {
+        size_t len = std::min(cap - 1, all.size());
+        memcpy(names, all.data(), len);
+        names[len] = 0;
+    }
+    if (count) *count = k;
+    return PG_OK;
+}
+
+extern "C" int pg_engine_synchronize(pg_engine* e) {
+    if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return PG_OK;
+}
+
+// Grows the batch scratch to hold `bodies` bodies, `nodes` tree nodes, `segs`
+// segments and `blocks` direct-sum blocks. One allocation, carved in place.
+static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_t segs,
+                          uint32_t blocks) {
+    if (bodies <= e->cap_bodies && nodes <= e->cap_nodes && segs <= e->cap_segs &&
+        blocks <= e->cap_blocks && e->scratch)
+        return PG_OK;
+    bodies = std::max<uint64_t>(bodies, e->cap_bodies);
+    nodes = std::max<uint64_t>(nodes, e->cap_nodes);
+    segs = std::max<uint32_t>(segs, e->cap_segs);
+    blocks = std::max<uint32_t>(blocks, e->cap_blocks);
+    bodies = std::max<uint64_t>(bodies, 1024);
+    segs = std::max<uint32_t>(segs, 4);
+    blocks = std::max<uint32_t>(blocks, 16);
+    nodes = std::max<uint64_t>(nodes, 1024);
+    const uint64_t sort_tiles = (bodies + kSortTile - 1) / kSortTile;
+    const uint64_t scan_tiles = (bodies + kScanTile - 1) / kScanTile;
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        uint64_t o = off;
+        off += align256(bytes);
+        return o;
+    };
+    const uint64_t o_k0 = take(bodies * 8), o_k1 = take(bodies * 8);
+    const uint64_t o_v0 = take(bodies * 4), o_v1 = take(bodies * 4);
+    const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies);
+    const uint64_t o_ss = take(sort_tiles * 256 * 8), o_cs = take(scan_tiles * 8);
+    const uint64_t o_hot = take(nodes * sizeof(NodeHot)), o_cold = take(nodes * sizeof(NodeCold));
+    const uint64_t o_acc = take(bodies * 24);
+    const uint64_t o_d0 = take(segs * sizeof(SegDesc)), o_d1 = take(segs * sizeof(SegDesc));
+    const uint64_t o_so = take((segs + 1) * 8);
+    const uint64_t o_bs = take(blocks * 4), o_bt = take(blocks * 4);
+    const uint64_t o_sc = take(sizeof(StepCounters)), o_eg = take(256);
+    const uint64_t total = off;
+    if (e->sims_bytes + total > e->budget)
+        return pg_set_error(PG_ERR_BUDGET, "scratch of %llu bytes exceeds the HBM budget",
+                            (unsigned long long)total);
+    (void)hipStreamSynchronize(e->stream);
+    free_scratch(e);
+    void* p = nullptr;
+    hipError_t he = hipMalloc(&p, total);
+    if (he != hipSuccess)
+        return pg_set_error(PG_ERR_NOMEM, "hipMalloc(%llu) for scratch: %s",
+                            (unsigned long long)total, hipGetErrorString(he));
+    char* b = (char*)p;
+    e->scratch = p;
+    e->scratch_bytes = total;
+    e->keys[0] = (uint64_t*)(b + o_k0);
+    e->keys[1] = (uint64_t*)(b + o_k1);
+    e->vals[0] = (uint32_t*)(b + o_v0);
+    e->vals[1] = (uint32_t*)(b + o_v1);
+    e->first_node = (uint32_t*)(b + o_fn);
+    e->lo_level = (uint8_t*)(b + o_lo);
+    e->sort_status = (uint64_t*)(b + o_ss);
+    e->scan_status = (uint64_t*)(b + o_cs);
+    e->hot = (NodeHot*)(b + o_hot);
+    e->cold = (NodeCold*)(b + o_cold);
+    e->acc_out = (double*)(b + o_acc);
+    e->desc[0] = (SegDesc*)(b + o_d0);
+    e->desc[1] = (SegDesc*)(b + o_d1);
+    e->seg_off = (uint64_t*)(b + o_so);
+    e->blk_seg = (uint32_t*)(b + o_bs);
+    e->blk_tile = (uint32_t*)(b + o_bt);
+    e->sc = (StepCounters*)(b + o_sc);
+    e->err_global = (uint32_t*)(b + o_eg);
+    // Status words are versioned; zero them once so no stale tag can match.
+    HIP_TRY(hipMemsetAsync(e->sort_status, 0, sort_tiles * 256 * 8, e->stream));
+    HIP_TRY(hipMemsetAsync(e->scan_status, 0, scan_tiles * 8, e->stream));
+    HIP_TRY(hipMemsetAsync(e->sc, 0, sizeof(StepCounters), e->stream));
+    HIP_TRY(hipMemsetAsync(e->err_global, 0, 256, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->cap_bodies = bodies;
+    e->cap_nodes = nodes;
+    e->cap_segs = segs;
+    e->cap_blocks = blocks;
+    e->last_sim = nullptr;
+    return PG_OK;
+}
+
+// ----------------------------------------------------------- kernel timing
+static EventPair ev_get(pg_engine* e) {
+    if (!e->ev_pool.empty()) {
+        EventPair p = e->ev_pool.back();
+        e->ev_pool.pop_back();
+        return p;
+    }
+    EventPair p;
+    (void)hipEventCreate(&p.a);
+    (void)hipEventCreate(&p.b);
+    return p;
+}
+
+template <class F>
+static void launch(pg_engine* e, bool capturing, const char* name, F&& f) {
+    if (e->profiling && !capturing) {
+        EventPair p = ev_get(e);
+        (void)hipEventRecord(p.a, e->stream);
+        f();
+        (void)hipEventRecord(p.b, e->stream);
+        e->ev_pending.push_back({name, p});
+    } else {
+        f();
+    }
+}
+
+static void ev_collect(pg_engine* e) {
+    for (auto& pe : e->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.second.a, pe.second.b) == hipSuccess) {
+            auto& slot = e->prof[pe.first];
+            slot.first += ms;
+            slot.second += 1;
+        }
+        e->ev_pool.push_back(pe.second);
+    }
+    e->ev_pending.clear();
+}
+
+// ---------------------------------------------------------- step sequence
+struct BatchPlan {
+    int kind = PG_KIND_OCTREE;
+    int precision = PG_PREC_MIXED;
+    int dim = 3, segbits = 0, lmax = 21, passes = 8;
+    uint32_t nseg = 0;
+    uint64_t total = 0;
+    uint32_t nblocks = 0;
+    bool integrate = true;
+};
+
+static inline unsigned grid_for(uint64_t n, unsigned cap) {
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    return (unsigned)std::min<uint64_t>(g, cap);
+}
+
+template <int DIM, bool FP64, bool INTEG>
+static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap) {
+    hipStream_t st = e->stream;
+    SegDesc* desc = e->desc[q];
+    const uint64_t total = bp.total;
+    launch(e, cap, "prep", [&] {
+        k_prep<<<1, kBlock, 0, st>>>(desc, bp.nseg, e->sc, INTEG ? 1 : 0);
+    });
+    launch(e, cap, "extent", [&] {
+        k_extent<<<grid_for(total, 2048), kBlock, 0, st>>>(desc, e->seg_off, bp.nseg, total);
+    });
+    launch(e, cap, "keys", [&] {
+        k_keys<DIM><<<grid_for(total, 512), kBlock, 0, st>>>(desc, e->seg_off, bp.nseg, total,
+                                                            bp.segbits, bp.lmax, bp.passes,
+                                                            e->keys[0], e->vals[0], e->sc);
+    });
+    const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
+    for (int p = 0; p < bp.passes; ++p) {
+        const int a = p & 1, b = a ^ 1;
+        launch(e, cap, "sort_pass", [&] {
+            k_sort_pass<<<sort_tiles, kBlock, 0, st>>>(e->keys[a], e->vals[a], e->keys[b],
+                                                       e->vals[b], total, p, e->sc,
+                                                       e->sort_status, e->err_global);
+        });
+    }
+    const uint64_t* skeys = e->keys[bp.passes & 1];
+    const uint32_t* svals = e->vals[bp.passes & 1];
+    const unsigned scan_tiles = (unsigned)((total + kScanTile - 1) / kScanTile);
+    launch(e, cap, "scan_gather", [&] {
+        k_scan_gather<DIM><<<scan_tiles, kBlock, 0, st>>>(
+            desc, total, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->first_node, e->lo_level,
+            e->sc, e->scan_status, e->err_global);
+    });
+    launch(e, cap, "nodes", [&] {
+        k_nodes<DIM><<<grid_for(total, 1u << 20), kBlock, 0, st>>>(
+            desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->first_node, e->lo_level, e->hot,
+            e->cold, e->cap_nodes);
+    });
+    launch(e, cap, "com", [&] {
+        k_com<<<grid_for(total, 1u << 20), kBlock, 0, st>>>(total, e->first_node, e->hot,
+                                                           e->cold, e->cap_nodes);
+    });
+    launch(e, cap, "walk", [&] {
+        k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
+            desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
+            e->acc_out);
+    });
+}
+
+template <bool FP64, bool INTEG>
+static void enqueue_direct_step(pg_engine* e, const BatchPlan& bp, int q, bool cap) {
+    hipStream_t st = e->stream;
+    SegDesc* desc = e->desc[q];
+    launch(e, cap, "prep", [&] {
+        k_prep<<<1, kBlock, 0, st>>>(desc, bp.nseg, e->sc, INTEG ? 1 : 0);
+    });
+    launch(e, cap, "direct", [&] {
+        k_direct<FP64, INTEG><<<bp.nblocks, kBlock, 0, st>>>(desc, e->blk_seg, e->blk_tile,
+                                                            e->acc_out);
+    });
+}
+
+static void enqueue_step(pg_engine* e, const BatchPlan& bp, int q, bool cap) {
+    const bool f64 = bp.precision == PG_PREC_FP64;
+    if (bp.kind == PG_KIND_DIRECT) {
+        if (bp.integrate) {
+            if (f64) enqueue_direct_step<true, true>(e, bp, q, cap);
+            else enqueue_direct_step<false, true>(e, bp, q, cap);
+        } else {
+            if (f64) enqueue_direct_step<true, false>(e, bp, q, cap);
+            else enqueue_direct_step<false, false>(e, bp, q, cap);
+        }
+        return;
+    }
+    if (bp.dim == 3) {
+        if (bp.integrate) {
+            if (f64) enqueue_tree_step<3, true, true>(e, bp, q, cap);
+            else enqueue_tree_step<3, false, true>(e, bp, q, cap);
+        } else {
+            if (f64) enqueue_tree_step<3, true, false>(e, bp, q, cap);
+            else enqueue_tree_step<3, false, false>(e, bp, q, cap);
+        }
+    } else {
+        if (bp.integrate) {
+            if (f64) enqueue_tree_step<2, true, true>(e, bp, q, cap);
+            else enqueue_tree_step<2, false, true>(e, bp, q, cap);
+        } else {
+            if (f64) enqueue_tree_step<2, true, false>(e, bp, q, cap);
+            else enqueue_tree_step<2, false, false>(e, bp, q, cap);
+        }
+    }
+}
+
+static uint32_t map_err(uint32_t bits) {
+    if (bits & kErrNonFinite) return PG_ERR_NONFINITE;
+    if (bits & kErrCoincident) return PG_ERR_COINCIDENT;
+    if (bits & kErrDepth) return PG_ERR_DEPTH;
+    if (bits & (kErrCapacity | kErrSpin)) return PG_ERR_INTERNAL;
+    return PG_OK;
+}
+
+static int err_text(uint32_t bits) {
+    const uint32_t code = map_err(bits);
+    switch (code) {
+        case PG_ERR_NONFINITE:
+            return pg_set_error(PG_ERR_NONFINITE, "non-finite body coordinate");
+        case PG_ERR_COINCIDENT:
+            return pg_set_error(PG_ERR_COINCIDENT,
+                                "bodies within 1e-9 on every axis (the reference merges them; "
+                                "not supported on the device)");
+        case PG_ERR_DEPTH:
+            return pg_set_error(PG_ERR_DEPTH,
+                                "two bodies share every level the Morton key holds");
+        case PG_ERR_INTERNAL:
+            return pg_set_error(PG_ERR_INTERNAL,
+                                "device-side capacity or bounded wait exceeded (bits 0x%x)", bits);
+        default:
+            return PG_OK;
+    }
+}
+
+// Core of every device computation: `steps` steps of all sims together
+// (integrate) or one force evaluation into e->acc_out (no integrate).
+static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint32_t steps,
+                     bool integrate) {
+    std::vector<pg_sim*> sims;
+    for (uint32_t i = 0; i < count; ++i) {
+        pg_sim* s = sims_in[i];
+        if (!s) return pg_set_error(PG_ERR_ARG, "sim %u is NULL", i);
+        if (s->eng != e) return pg_set_error(PG_ERR_ARG, "sim %u belongs to another engine", i);
+        if (!s->uploaded) return pg_set_error(PG_ERR_STATE, "sim %u has no uploaded state", i);
+        if (s->n) sims.push_back(s);
+    }
+    if (sims.empty() || steps == 0) return PG_OK;
+    BatchPlan bp;
+    bp.kind = sims[0]->p.kind;
+    bp.precision = sims[0]->p.precision;
+    for (pg_sim* s : sims)
+        if (s->p.kind != bp.kind || s->p.precision != bp.precision)
+            return pg_set_error(PG_ERR_ARG, "a batch must share kind and precision");
+    bp.integrate = integrate;
+    bp.nseg = (uint32_t)sims.size();
+    bp.dim = key_dim(bp.kind);
+    bp.segbits = bp.nseg > 1 ? ceil_log2(bp.nseg) : 0;
+    bp.lmax = key_lmax(bp.dim, bp.segbits);
+    bp.passes = (bp.segbits + bp.dim * bp.lmax + 7) / 8;
+    for (pg_sim* s : sims) bp.total += s->n;
+    if (bp.total >= (1ull << 32))
+        return pg_set_error(PG_ERR_ARG, "a batch holds at most 2^32-1 bodies");
+    uint64_t nodes = bp.kind == PG_KIND_DIRECT ? 0 : nodes_bound(bp.total, bp.dim, bp.lmax) + bp.nseg;
+    if (nodes >= (1ull << 32)) return pg_set_error(PG_ERR_ARG, "tree too large for 32-bit nodes");
+    e->h_blk_seg.clear();
+    e->h_blk_tile.clear();
+    if (bp.kind == PG_KIND_DIRECT) {
+        for (uint32_t s = 0; s < bp.nseg; ++s) {
+            const uint32_t nb = (uint32_t)((sims[s]->n + kBlock - 1) / kBlock);
+            for (uint32_t t = 0; t < nb; ++t) {
+                e->h_blk_seg.push_back(s);
+                e->h_blk_tile.push_back(t);
+            }
+        }
+        bp.nblocks = (uint32_t)e->h_blk_seg.size();
+    }
+    PG_TRY(ensure_scratch(e, bp.total, nodes, bp.nseg, bp.nblocks));
+
+    // descriptors for both parities
+    e->h_seg_off.assign(bp.nseg + 1, 0);
+    for (int q = 0; q < 2; ++q) e->h_desc[q].assign(bp.nseg, SegDesc{});
+    uint64_t off = 0;
+    for (uint32_t s = 0; s < bp.nseg; ++s) {
+        pg_sim* sm = sims[s];
+        e->h_seg_off[s] = off;
+        for (int q = 0; q < 2; ++q) {
+            const int src = sm->cur ^ q;
+            const int dst = bp.kind == PG_KIND_DIRECT && !integrate ? src : src ^ 1;
+            SegDesc& d = e->h_desc[q][s];
+            d.sx = sm->x[src]; d.sy = sm->y[src]; d.sz = sm->z[src];
+            d.spx = sm->px[src]; d.spy = sm->py[src]; d.spz = sm->pz[src];
+            d.svx = sm->vx[src]; d.svy = sm->vy[src]; d.svz = sm->vz[src];
+            d.sm = sm->m[src]; d.sfix = sm->fix[src]; d.sid = sm->id[src];
+            d.dx = sm->x[dst]; d.dy = sm->y[dst]; d.dz = sm->z[dst];
+            d.dpx = sm->px[dst]; d.dpy = sm->py[dst]; d.dpz = sm->pz[dst];
+            d.dvx = sm->vx[dst]; d.dvy = sm->vy[dst]; d.dvz = sm->vz[dst];
+            d.dm = sm->m[dst]; d.dfix = sm->fix[dst]; d.did = sm->id[dst];
+            d.off = off;
+            d.n = sm->n;
+            d.theta = sm->p.theta;
+            d.e = std::fabs(sm->p.e);
+            d.dt = sm->p.dt;
+            d.seg = s;
+            d.count = sm->counting ? 1u : 0u;
+            d.counts = sm->counts;
+            d.st = sm->st;
+        }
+        off += sm->n;
+    }
+    e->h_seg_off[bp.nseg] = off;
+    HIP_TRY(hipMemcpyAsync(e->desc[0], e->h_desc[0].data(), bp.nseg * sizeof(SegDesc),
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->desc[1], e->h_desc[1].data(), bp.nseg * sizeof(SegDesc),
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->seg_off, e->h_seg_off.data(), (bp.nseg + 1) * 8,
+                           hipMemcpyHostToDevice, e->stream));
+    if (bp.nblocks) {
+        HIP_TRY(hipMemcpyAsync(e->blk_seg, e->h_blk_seg.data(), bp.nblocks * 4,
+                               hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->blk_tile, e->h_blk_tile.data(), bp.nblocks * 4,
+                               hipMemcpyHostToDevice, e->stream));
+    }
+
+    const bool flips = bp.kind != PG_KIND_DIRECT || integrate;
+    uint32_t done = 0;
+    if (integrate && e->graphs && !e->profiling && steps >= 2) {
+        // signature: sims, parities, sizes, flags and the scratch it was captured on
+        std::vector<uint64_t> sig;
+        sig.push_back((uint64_t)(uintptr_t)e->scratch);
+        sig.push_back((uint64_t)bp.kind * 16 + bp.precision);
+        for (pg_sim* s : sims) {
+            sig.push_back((uint64_t)(uintptr_t)s);
+            sig.push_back(s->n * 4 + (uint64_t)s->cur * 2 + (s->counting ? 1 : 0));
+        }
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : e->graph_cache)
+            if (g.sig == sig) exec = g.exec;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(e, bp, 0, true);
+            enqueue_step(e, bp, 1, true);
+            HIP_TRY(hipStreamEndCapture(e->stream, &graph));
+            HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            if (e->graph_cache.size() >= 4) {
+                (void)hipGraphExecDestroy(e->graph_cache.front().exec);
+                e->graph_cache.erase(e->graph_cache.begin());
+            }
+            e->graph_cache.push_back({sig, exec});
+        }
+        for (; done + 2 <= steps; done += 2) HIP_TRY(hipGraphLaunch(exec, e->stream));
+    }
+    for (; done < steps; ++done) {
+        enqueue_step(e, bp, (int)(done & 1), false);
+        if (!integrate) break;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->profiling) ev_collect(e);
+
+    if (flips && (steps & 1))
+        for (pg_sim* s : sims) s->cur ^= 1;
+
+    e->last_sim = bp.nseg == 1 && bp.kind != PG_KIND_DIRECT ? sims[0] : nullptr;
+    e->last_total = bp.total;
+    e->last_passes = bp.passes;
+    e->last_dim = bp.dim;
+    e->last_lmax = bp.lmax;
+
+    uint32_t gerr = 0;
+    HIP_TRY(hipMemcpy(&gerr, e->err_global, 4, hipMemcpyDeviceToHost));
+    if (gerr) {
+        uint32_t z = 0;
+        HIP_TRY(hipMemcpy(e->err_global, &z, 4, hipMemcpyHostToDevice));
+        return err_text(gerr);
+    }
+    int rc = PG_OK;
+    for (pg_sim* s : sims) {
+        SegState st;
+        HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
+        if (st.err && rc == PG_OK) rc = err_text(st.err);
+    }
+    return rc;
+}
+
+// ------------------------------------------------------------- simulations
+static int validate_params(const pg_params* p) {
+    if (!p) return pg_set_error(PG_ERR_ARG, "params is NULL");
+    if (p->kind != PG_KIND_DIRECT && p->kind != PG_KIND_QUADTREE && p->kind != PG_KIND_OCTREE)
+        return pg_set_error(PG_ERR_ARG, "unknown kind %d", p->kind);
+    if (p->precision != PG_PREC_MIXED && p->precision != PG_PREC_FP64)
+        return pg_set_error(PG_ERR_ARG, "unknown precision %d", p->precision);
+    return PG_OK;
+}
+
+extern "C" int pg_sim_create(pg_engine* e, uint64_t n, const pg_params* p, pg_sim** out) {
+    if (!e || !out) return pg_set_error(PG_ERR_ARG, "engine or out is NULL");
+    PG_TRY(validate_params(p));
+    if (n >= (1ull << 31)) return pg_set_error(PG_ERR_ARG, "n too large");
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    const uint64_t bytes = state_bytes(n);
+    if (e->sims_bytes + e->scratch_bytes + bytes > e->budget)
+        return pg_set_error(PG_ERR_BUDGET, "simulation of %llu bodies exceeds the HBM budget",
+                            (unsigned long long)n);
+    pg_sim* s = new pg_sim();
+    s->eng = e;
+    s->n = n;
+    s->p = *p;
+    hipError_t he = hipMalloc(&s->block, bytes);
+    if (he != hipSuccess) {
+        delete s;
+        return pg_set_error(PG_ERR_NOMEM, "hipMalloc(%llu): %s", (unsigned long long)bytes,
+                            hipGetErrorString(he));
+    }
+    s->bytes = bytes;
+    char* b = (char*)s->block;
+    uint64_t off = 0;
+    auto take = [&](uint64_t sz) {
+        char* q = b + off;
+        off += align256(sz);
+        return q;
+    };
+    for (int k = 0; k < 2; ++k) {
+        s->x[k] = (double*)take(n * 8); s->y[k] = (double*)take(n * 8); s->z[k] = (double*)take(n * 8);
+        s->px[k] = (double*)take(n * 8); s->py[k] = (double*)take(n * 8); s->pz[k] = (double*)take(n * 8);
+        s->vx[k] = (double*)take(n * 8); s->vy[k] = (double*)take(n * 8); s->vz[k] = (double*)take(n * 8);
+        s->m[k] = (double*)take(n * 8);
+        s->fix[k] = (uint8_t*)take(n);
+        s->id[k] = (uint32_t*)take(n * 4);
+    }
+    s->aos = (Entity*)take(n * 80);
+    s->counts = (int64_t*)take(n * 8);
+    s->st = (SegState*)take(sizeof(SegState));
+    HIP_TRY(hipMemset(s->st, 0, sizeof(SegState)));
+    e->sims_bytes += bytes;
+    *out = s;
+    return PG_OK;
+}
+
+extern "C" void pg_sim_destroy(pg_sim* s) {
+    if (!s) return;
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    // graphs captured on this sim's buffers must not be replayed
+    for (auto it = e->graph_cache.begin(); it != e->graph_cache.end();) {
+        bool uses = false;
+        for (uint64_t w : it->sig)
+            if (w == (uint64_t)(uintptr_t)s) uses = true;
+        if (uses) {
+            (void)hipGraphExecDestroy(it->exec);
+            it = e->graph_cache.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    if (e->last_sim == s) e->last_sim = nullptr;
+    (void)hipFree(s->block);
+    e->sims_bytes -= s->bytes;
+    delete s;
+}
+
+extern "C" int pg_sim_set_params(pg_sim* s, const pg_params* p) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    PG_TRY(validate_params(p));
+    s->p = *p;
+    return PG_OK;
+}
+
+extern "C" int pg_sim_set_counting(pg_sim* s, int enable) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    s->counting = enable != 0;
+    return PG_OK;
+}
+
+extern "C" int pg_sim_upload(pg_sim* s, const Entity* state, uint64_t n) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    if (n != s->n) return pg_set_error(PG_ERR_ARG, "n %llu != sim n %llu", (unsigned long long)n,
+                                       (unsigned long long)s->n);
+    if (n && !state) return pg_set_error(PG_ERR_ARG, "state is NULL");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    s->cur = 0;
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(s->aos, state, n * sizeof(Entity), hipMemcpyHostToDevice, e->stream));
+        k_unpack<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(
+            s->aos, n, s->x[0], s->y[0], s->z[0], s->px[0], s->py[0], s->pz[0], s->vx[0],
+            s->vy[0], s->vz[0], s->m[0], s->fix[0], s->id[0]);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemsetAsync(s->st, 0, sizeof(SegState), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    s->uploaded = true;
+    if (e->last_sim == s) e->last_sim = nullptr;
+    return PG_OK;
+}
+
+extern "C" int pg_sim_download(pg_sim* s, Entity* out, uint64_t n) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    if (n != s->n) return pg_set_error(PG_ERR_ARG, "n mismatch");
+    if (!s->uploaded) return pg_set_error(PG_ERR_STATE, "no state uploaded");
+    if (!n) return PG_OK;
+    if (!out) return pg_set_error(PG_ERR_ARG, "out is NULL");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    const int c = s->cur;
+    k_pack<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(s->aos, n, s->x[c], s->y[c], s->z[c],
+                                                           s->vx[c], s->vy[c], s->vz[c], s->id[c]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, s->aos, n * sizeof(Entity), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return PG_OK;
+}
+
+extern "C" int pg_sim_step(pg_sim* s, uint32_t steps) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    return run_batch(e, &s, 1, steps, true);
+}
+
+extern "C" int pg_batch_step(pg_engine* e, pg_sim* const* sims, uint32_t count, uint32_t steps) {
+    if (!e || (count && !sims)) return pg_set_error(PG_ERR_ARG, "engine or sims is NULL");
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    return run_batch(e, sims, count, steps, true);
+}
+
+extern "C" int pg_sim_accel(pg_sim* s, double* acc) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    if (s->n && !acc) return pg_set_error(PG_ERR_ARG, "acc is NULL");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    PG_TRY(run_batch(e, &s, 1, 1, false));
+    if (s->n) HIP_TRY(hipMemcpy(acc, e->acc_out, s->n * 24, hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+extern "C" int pg_transform_aos(pg_engine* e, const pg_params* p, const Entity* state, uint64_t n,
+                                Acceleration* acc) {
+    if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    PG_TRY(validate_params(p));
+    if (n == 0) return PG_OK;
+    if (!state || !acc) return pg_set_error(PG_ERR_ARG, "state or acc is NULL");
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->xform && e->xform->n != n) {
+        pg_sim_destroy(e->xform);
+        e->xform = nullptr;
+    }
+    if (!e->xform) PG_TRY(pg_sim_create(e, n, p, &e->xform));
+    PG_TRY(pg_sim_set_params(e->xform, p));
+    PG_TRY(pg_sim_upload(e->xform, state, n));
+    std::vector<double> a(n * 3);
+    PG_TRY(pg_sim_accel(e->xform, a.data()));
+    for (uint64_t i = 0; i < n; ++i) {
+        acc[i].x += a[3 * i + 0];
+        acc[i].y += a[3 * i + 1];
+        acc[i].z += a[3 * i + 2];
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_sim_energy(pg_sim* s, double out[6]) {
+    if (!s || !out) return pg_set_error(PG_ERR_ARG, "sim or out is NULL");
+    if (!s->uploaded) return pg_set_error(PG_ERR_STATE, "no state uploaded");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    const uint64_t n = s->n;
+    for (int i = 0; i < 6; ++i) out[i] = 0.0;
+    if (!n) return PG_OK;
+    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    double* part = nullptr;
+    HIP_TRY(hipMalloc(&part, (size_t)nb * 5 * 8));
+    const int c = s->cur;
+    k_energy<<<nb, kBlock, 0, e->stream>>>(s->x[c], s->y[c], s->z[c], s->vx[c], s->vy[c],
+                                          s->vz[c], s->m[c], n, std::fabs(s->p.e), part);
+    hipError_t le = hipGetLastError();
+    std::vector<double> h((size_t)nb * 5);
+    hipError_t ce = hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, e->stream);
+    hipError_t se = hipStreamSynchronize(e->stream);
+    (void)hipFree(part);
+    if (le != hipSuccess || ce != hipSuccess || se != hipSuccess)
+        return pg_set_error(PG_ERR_HIP, "energy kernel failed");
+    long double V = 0, K = 0, P[3] = {0, 0, 0};
+    for (unsigned b = 0; b < nb; ++b) {
+        V += h[5 * b + 0];
+        K += h[5 * b + 1];
+        P[0] += h[5 * b + 2];
+        P[1] += h[5 * b + 3];
+        P[2] += h[5 * b + 4];
+    }
+    out[0] = (double)K;
+    out[1] = (double)V;
+    out[2] = (double)(K + V);
+    out[3] = (double)P[0];
+    out[4] = (double)P[1];
+    out[5] = (double)P[2];
+    return PG_OK;
+}
+
+extern "C" int pg_sim_stats(pg_sim* s, pg_stats* out) {
+    if (!s || !out) return pg_set_error(PG_ERR_ARG, "sim or out is NULL");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    SegState st;
+    HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof *out);
+    out->steps_done = st.steps_done;
+    out->interactions = st.interactions;
+    out->max_level = st.max_level;
+    out->error = map_err(st.err);
+    out->key_levels = (uint32_t)key_lmax(key_dim(s->p.kind), 0);
+    if (e->last_sim == s) {
+        uint32_t nn = 0;
+        HIP_TRY(hipMemcpy(&nn, e->first_node + e->last_total, 4, hipMemcpyDeviceToHost));
+        out->nodes = nn;
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_sim_interaction_counts(pg_sim* s, int64_t* counts) {
+    if (!s || (s->n && !counts)) return pg_set_error(PG_ERR_ARG, "sim or counts is NULL");
+    if (!s->counting) return pg_set_error(PG_ERR_STATE, "counting is off");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    if (s->n) HIP_TRY(hipMemcpy(counts, s->counts, s->n * 8, hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+extern "C" int pg_sim_morton(pg_sim* s, uint64_t* keys, uint32_t* perm, uint64_t n) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (e->last_sim != s) return pg_set_error(PG_ERR_STATE, "the last build was not of this sim");
+    if (n != s->n) return pg_set_error(PG_ERR_ARG, "n mismatch");
+    HIP_TRY(hipSetDevice(e->device));
+    if (keys) HIP_TRY(hipMemcpy(keys, e->keys[e->last_passes & 1], n * 8, hipMemcpyDeviceToHost));
+    if (perm) HIP_TRY(hipMemcpy(perm, s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, uint64_t* count) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (e->last_sim != s) return pg_set_error(PG_ERR_STATE, "the last build was not of this sim");
+    HIP_TRY(hipSetDevice(e->device));
+    const uint64_t n = s->n;
+    std::vector<uint32_t> fn(n + 1);
+    HIP_TRY(hipMemcpy(fn.data(), e->first_node, (n + 1) * 4, hipMemcpyDeviceToHost));
+    const uint64_t nn = fn[n];
+    if (count) *count = nn;
+    if (!rec && !fl) return PG_OK;
+    std::vector<NodeHot> hot(nn);
+    std::vector<NodeCold> cold(nn);
+    std::vector<uint64_t> keys(n);
+    std::vector<uint32_t> id(n);
+    HIP_TRY(hipMemcpy(hot.data(), e->hot, nn * sizeof(NodeHot), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cold.data(), e->cold, nn * sizeof(NodeCold), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(keys.data(), e->keys[e->last_passes & 1], n * 8, hipMemcpyDeviceToHost));
+    // The permuted ids live in the buffer the build wrote: the current one.
+    HIP_TRY(hipMemcpy(id.data(), s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
+    const int dim = e->last_dim, lmax = e->last_lmax;
+    for (uint64_t k = 0; k < nn && k < cap; ++k) {
+        const NodeHot& h = hot[k];
+        const NodeCold& c = cold[k];
+        const int L = (int)h.level;
+        int oct = -1;
+        if (L > 0) oct = (int)((keys[c.first] >> (dim * (lmax - L))) & ((1u << dim) - 1u));
+        const bool leaf = h.skip == k + 1;
+        if (rec) {
+            rec[4 * k + 0] = L;
+            rec[4 * k + 1] = oct;
+            rec[4 * k + 2] = leaf ? (int64_t)id[c.first] : -1;
+            rec[4 * k + 3] = c.first;
+        }
+        if (fl) {
+            fl[8 * k + 0] = h.mass;
+            fl[8 * k + 1] = h.mx;
+            fl[8 * k + 2] = h.my;
+            fl[8 * k + 3] = h.mz;
+            fl[8 * k + 4] = h.cx;
+            fl[8 * k + 5] = h.cy;
+            fl[8 * k + 6] = h.cz;
+        }
+    }
+    if (fl) {
+        SegState st;
+        HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
+        double E;
+        memcpy(&E, &st.extent_bits, 8);
+        for (uint64_t k = 0; k < nn && k < cap; ++k) fl[8 * k + 7] = std::ldexp(E, -(int)hot[k].level);
+    }
+    return PG_OK;
+}

@@ -1,0 +1,84 @@
+// Shared host/device definitions of the physim_gpu engine.
+#pragma once
+
+#include <stdint.h>
+
+#include "physim_gpu.h"
+
+namespace pg {
+
+constexpr int kBlock = 256;        // threads per block (4 wave64)
+constexpr int kSortItems = 8;      // keys per thread in a radix-sort tile
+constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kScanItems = 8;      // counts per thread in the node-offset scan
+constexpr int kScanTile = kBlock * kScanItems;
+constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
+constexpr uint32_t kSpinLimit = 1u << 22;  // bound of every device-side wait
+
+// Device-side error bits (sticky per segment), mapped to pg_status on the host.
+enum : uint32_t {
+    kErrDepth = 1u << 0,
+    kErrCoincident = 1u << 1,
+    kErrNonFinite = 1u << 2,
+    kErrCapacity = 1u << 3,
+    kErrSpin = 1u << 4,
+};
+
+struct SegState;
+
+// One simulation's view inside a batched launch: the current state (src) and
+// the buffer the step writes (dst), both SoA in the order of the previous
+// build's Morton sort.
+struct SegDesc {
+    const double* sx; const double* sy; const double* sz;
+    const double* spx; const double* spy; const double* spz;
+    const double* svx; const double* svy; const double* svz;
+    const double* sm; const uint8_t* sfix; const uint32_t* sid;
+    double* dx; double* dy; double* dz;
+    double* dpx; double* dpy; double* dpz;
+    double* dvx; double* dvy; double* dvz;
+    double* dm; uint8_t* dfix; uint32_t* did;
+    uint64_t off;   // first global (batch) index of this segment
+    uint64_t n;
+    double theta, e, dt;
+    uint32_t seg;
+    uint32_t count; // per-body interaction counting into counts[] (debug)
+    int64_t* counts;
+    struct SegState* st;
+};
+
+// Per-segment device scalars.
+struct SegState {
+    uint64_t extent_bits;   // max |coord| as the bit pattern of a non-negative double
+    uint64_t interactions;
+    uint64_t steps_done;
+    uint32_t history;       // 1 when the Stormer history is valid for this step
+    uint32_t err;
+    uint32_t max_level;
+    uint32_t pad;
+};
+
+// Tree node, split into the part the walk reads (64 bytes, one line) and the
+// part only the build reads.
+struct NodeHot {
+    double cx, cy, cz;      // geometric centre (reference rounding)
+    double mx, my, mz;      // centre of mass (leaf: the body's exact position)
+    double mass;
+    uint32_t skip;          // index of the next node after this subtree
+    uint32_t level;
+};
+struct NodeCold {
+    uint32_t first, last;   // sorted-body range
+    int32_t parent;
+    uint32_t arrivals;      // bottom-up reduction counter
+};
+
+// Batch-wide counters reset at the start of every step.
+struct StepCounters {
+    uint32_t gen;           // bumped per step: versions look-back status words
+    uint32_t sort_ticket[kMaxPasses];
+    uint32_t scan_ticket;
+    uint32_t hist[kMaxPasses * 256];
+};
+
+}  // namespace pg

@@ -1,0 +1,925 @@
+// Device kernels of the physim_gpu engine (gfx950, wave64). Included once,
+// by engine.hip. Compiled with -ffp-contract=off: every FP64 expression that
+// must round as the reference does is written in the reference's operation
+// order, and FP32 force code asks for fused multiply-adds explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pg_common.h"
+
+namespace pg {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ double bits_f64(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ uint64_t f64_bits(double d) { return (uint64_t)__double_as_longlong(d); }
+
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Look-back status word: [63:34] version tag, [33:32] flag, [31:0] value.
+constexpr uint64_t kFlagAgg = 1, kFlagPrefix = 2;
+__device__ __forceinline__ uint64_t status_pack(uint32_t tag, uint64_t flag, uint32_t v) {
+    return ((uint64_t)(tag & 0x3FFFFFFFu) << 34) | (flag << 32) | v;
+}
+__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Segment of global index g: largest s with seg_off[s] <= g (empty segments
+// share an offset with their successor and are skipped).
+__device__ __forceinline__ uint32_t seg_of(const uint64_t* __restrict__ seg_off, uint32_t nseg,
+                                           uint64_t g) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (seg_off[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t seg_of_key(uint64_t key, int segbits) {
+    return segbits ? (uint32_t)(key >> (64 - segbits)) : 0u;
+}
+
+// Key prefix holding the segment bits and the top `level` tree digits.
+template <int DIM>
+__device__ __forceinline__ uint64_t key_prefix(uint64_t key, int lmax, int level) {
+    int sh = DIM * (lmax - level);
+    return sh >= 64 ? 0ull : key >> sh;
+}
+
+// Common prefix length, in tree levels, of two keys of one segment.
+template <int DIM>
+__device__ __forceinline__ int key_lcp(uint64_t a, uint64_t b, int lmax) {
+    uint64_t x = a ^ b;
+    int mbits = DIM * lmax;
+    if (mbits < 64) x &= (1ull << mbits) - 1ull;
+    if (x == 0) return lmax;
+    int lz = __clzll((long long)x) - (64 - mbits);
+    return lz / DIM;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave,
+                                                         uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(inc, o);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        uint32_t c = s_wave[w];
+        if (w < wave) wbase += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return wbase + inc - v;
+}
+
+// Decoupled look-back: exclusive prefix of tile `tile` for one value lane.
+__device__ __forceinline__ uint32_t lookback(const uint64_t* status, uint32_t tile,
+                                             uint32_t stride, uint32_t col, uint32_t tag,
+                                             uint32_t* err) {
+    uint32_t excl = 0;
+    int64_t t = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (t >= 0) {
+        uint64_t w = status_load(status + (uint64_t)t * stride + col);
+        uint32_t wtag = (uint32_t)(w >> 34);
+        uint64_t flag = (w >> 32) & 3ull;
+        if (wtag != (tag & 0x3FFFFFFFu) || flag == 0) {
+            if (++spins > kSpinLimit) {
+                atomicOr(err, kErrSpin);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += (uint32_t)w;
+        if (flag == kFlagPrefix) break;
+        --t;
+    }
+    return excl;
+}
+
+// ------------------------------------------------------------ step prologue
+__global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCounters* sc,
+                       int integrate) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if (t == 0) {
+        sc->gen += 1;
+        for (int p = 0; p < kMaxPasses; ++p) sc->sort_ticket[p] = 0;
+        sc->scan_ticket = 0;
+    }
+    for (uint32_t i = t; i < kMaxPasses * 256; i += stride) sc->hist[i] = 0;
+    for (uint32_t s = t; s < nseg; s += stride) {
+        SegState* st = desc[s].st;
+        st->extent_bits = 0;
+        st->interactions = 0;
+        st->max_level = 0;
+        st->history = st->steps_done > 0 ? 1u : 0u;
+        if (integrate) st->steps_done += 1;
+    }
+}
+
+// transformers.rs:126-131: max |coord| over the segment (exact), NaN/Inf flagged.
+__global__ void k_extent(const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off,
+                         uint32_t nseg, uint64_t total) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
+        const uint64_t g = base + threadIdx.x;
+        const bool valid = g < total;
+        uint32_t s = 0;
+        uint64_t bits = 0;
+        if (valid) {
+            s = seg_of(seg_off, nseg, g);
+            const SegDesc& d = desc[s];
+            const uint64_t j = g - d.off;
+            const double x = d.sx[j], y = d.sy[j], z = d.sz[j];
+            if (!isfinite(x) || !isfinite(y) || !isfinite(z)) atomicOr(&d.st->err, kErrNonFinite);
+            bits = f64_bits(fmax(fmax(fabs(x), fabs(y)), fabs(z)));
+        }
+        const uint64_t vmask = __ballot(valid);
+        const int lead = __ffsll((long long)vmask) - 1;
+        const uint32_t s0 = __shfl(s, lead < 0 ? 0 : lead);
+        const bool uniform = __ballot(valid && s != s0) == 0;
+        if (uniform) {
+            uint64_t m = bits;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                uint64_t t2 = __shfl_xor(m, o);
+                m = t2 > m ? t2 : m;
+            }
+            if ((int)(threadIdx.x & 63) == lead) atomicMax((unsigned long long*)&desc[s0].st->extent_bits,
+                                                      (unsigned long long)m);
+        } else if (valid) {
+            atomicMax((unsigned long long*)&desc[s].st->extent_bits, (unsigned long long)bits);
+        }
+    }
+}
+
+// Morton key by exact FP64 descent (octree.rs:160-215 / quadtree.rs:161-194):
+// at each level the octant is (x > cx) | (y > cy) << 1 [| (z > cz) << 2] and
+// the child centre is c -/+ extent/2, rounded as the reference rounds it.
+// Also accumulates the digit histograms of every radix pass.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_keys(const SegDesc* __restrict__ desc,
+                                                 const uint64_t* __restrict__ seg_off,
+                                                 uint32_t nseg, uint64_t total, int segbits,
+                                                 int lmax, int passes, uint64_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals, StepCounters* sc) {
+    __shared__ uint32_t h[kMaxPasses * 256];
+    for (int i = threadIdx.x; i < passes * 256; i += kBlock) h[i] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        const uint32_t s = seg_of(seg_off, nseg, g);
+        const SegDesc& d = desc[s];
+        const uint64_t j = g - d.off;
+        const double px = d.sx[j], py = d.sy[j], pz = d.sz[j];
+        double ext = bits_f64(d.st->extent_bits);
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        uint64_t key = 0;
+        for (int l = 0; l < lmax; ++l) {
+            uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
+            if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
+            const double h2 = ext / 2.0;
+            cx = (o & 1u) ? cx + h2 : cx - h2;
+            cy = (o & 2u) ? cy + h2 : cy - h2;
+            if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+            ext = h2;
+            key = (key << DIM) | o;
+        }
+        if (segbits) key |= (uint64_t)s << (64 - segbits);
+        keys[g] = key;
+        vals[g] = (uint32_t)g;
+        for (int p = 0; p < passes; ++p) atomicAdd(&h[p * 256 + ((key >> (8 * p)) & 255u)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * 256; i += kBlock)
+        if (h[i]) atomicAdd(&sc->hist[i], h[i]);
+}
+
+// One stable LSD radix pass (8-bit digit) over (key, value) pairs: the
+// single-pass "onesweep" scheme. Ranks inside a tile come from wave64
+// match-any ballots and per-wave LDS counters, the tile's global offsets from
+// the precomputed digit histogram plus a decoupled look-back over tiles;
+// the tile is reordered in LDS and written out in runs per digit.
+__global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict__ kin,
+                                                      const uint32_t* __restrict__ vin,
+                                                      uint64_t* __restrict__ kout,
+                                                      uint32_t* __restrict__ vout, uint64_t n,
+                                                      int pass, StepCounters* sc,
+                                                      uint64_t* status, uint32_t* err) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wcnt[kBlock / 64][256];
+    __shared__ uint32_t s_lstart[256];
+    __shared__ uint32_t s_gbase[256];
+    __shared__ uint32_t s_scan[kBlock / 64];
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(&sc->sort_ticket[pass], 1u);
+    for (int i = tid; i < (kBlock / 64) * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t tag = (sc->gen << 4) | (uint32_t)(pass + 1);
+    const uint64_t base = (uint64_t)tile * kSortTile;
+    const int shift = 8 * pass;
+
+    uint64_t key[kSortItems];
+    uint32_t val[kSortItems], dig[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t idx = base + (uint64_t)wave * 64 * kSortItems + k * 64 + lane;
+        const bool valid = idx < n;
+        key[k] = valid ? kin[idx] : 0ull;
+        val[k] = valid ? vin[idx] : 0u;
+        dig[k] = valid ? (uint32_t)((key[k] >> shift) & 255u) : 256u;
+    }
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t d = dig[k];
+        const bool valid = d < 256u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        if (!valid) peers = 1ull << lane;
+        const int leader = __ffsll((long long)peers) - 1;
+        uint32_t old = 0;
+        if (valid && lane == leader) {
+            old = s_wcnt[wave][d];
+            s_wcnt[wave][d] = old + (uint32_t)__popcll(peers);
+        }
+        old = __shfl(old, leader);
+        rank[k] = old + (uint32_t)__popcll(peers & lt);
+    }
+    __syncthreads();
+    // Per digit (one thread each): exclusive offsets over waves, tile count.
+    const uint32_t d = tid;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t c = s_wcnt[w][d];
+        s_wcnt[w][d] = cnt;
+        cnt += c;
+    }
+    status_store(status + (uint64_t)tile * 256 + d,
+                 status_pack(tag, tile == 0 ? kFlagPrefix : kFlagAgg, cnt));
+    uint32_t dummy;
+    const uint32_t lstart = block_exclusive_scan(cnt, s_scan, &dummy);
+    const uint32_t ghist = sc->hist[pass * 256 + d];
+    const uint32_t gbase = block_exclusive_scan(ghist, s_scan, &dummy);
+    uint32_t excl = 0;
+    if (tile > 0) {
+        excl = lookback(status, tile, 256, d, tag, err);
+        status_store(status + (uint64_t)tile * 256 + d, status_pack(tag, kFlagPrefix, excl + cnt));
+    }
+    s_lstart[d] = lstart;
+    s_gbase[d] = gbase + excl - lstart;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        if (dig[k] < 256u) {
+            const uint32_t p = s_lstart[dig[k]] + s_wcnt[wave][dig[k]] + rank[k];
+            s_keys[p] = key[k];
+            s_vals[p] = val[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t valid_n = (uint32_t)min((uint64_t)kSortTile, n - base);
+    for (uint32_t p = tid; p < valid_n; p += kBlock) {
+        const uint64_t k = s_keys[p];
+        const uint32_t o = s_gbase[(k >> shift) & 255u] + p;
+        if (o < n) {
+            kout[o] = k;
+            vout[o] = s_vals[p];
+        } else {
+            atomicOr(err, kErrCapacity);
+        }
+    }
+}
+
+// Per sorted body: its common-prefix lengths with its neighbours give the
+// tree levels it owns (lo..hi, hi = its leaf); the exclusive scan of the
+// per-body node counts lays the tree out in preorder. Also permutes the
+// state into the new Morton order and flags equal keys (depth) and the
+// reference's 1e-9 merge rule (octree.rs:69-80).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_scan_gather(
+    const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys,
+    const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
+    uint32_t* __restrict__ first_node, uint8_t* __restrict__ lo_level, StepCounters* sc,
+    uint64_t* status, uint32_t* err) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_scan[kBlock / 64];
+    __shared__ uint32_t s_excl;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(&sc->scan_ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t tag = (sc->gen << 4) | 15u;
+    const uint64_t base = (uint64_t)tile * kScanTile + (uint64_t)tid * kScanItems;
+
+    uint32_t cnt[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k;
+        cnt[k] = 0;
+        if (i < total) {
+            const uint64_t key = skeys[i];
+            const uint32_t s = seg_of_key(key, segbits);
+            const uint64_t src = s < nseg ? (uint64_t)svals[i] - desc[s].off : ~0ull;
+            if (s >= nseg || src >= desc[s].n || i < desc[s].off || i >= desc[s].off + desc[s].n) {
+                atomicOr(err, kErrCapacity);
+                lo_level[i] = 0;
+                cnt[k] = 1;
+                sum += 1;
+                continue;
+            }
+            const SegDesc& d = desc[s];
+            const uint64_t end = d.off + d.n;
+            int lcpL = i > d.off ? key_lcp<DIM>(skeys[i - 1], key, lmax) : -1;
+            int lcpR = i + 1 < end ? key_lcp<DIM>(key, skeys[i + 1], lmax) : -1;
+            if (lcpR >= lmax) {
+                atomicOr(&d.st->err, kErrDepth);
+                lcpR = lmax - 1;
+            }
+            if (lcpL >= lmax) lcpL = lmax - 1;
+            cnt[k] = 1u + (uint32_t)max(0, lcpR - lcpL);
+            lo_level[i] = (uint8_t)(lcpL + 1);
+            const uint64_t j = i - d.off;
+            const double x = d.sx[src], y = d.sy[src], z = d.sz[src];
+            d.dx[j] = x; d.dy[j] = y; d.dz[j] = z;
+            d.dpx[j] = d.spx[src]; d.dpy[j] = d.spy[src]; d.dpz[j] = d.spz[src];
+            d.dvx[j] = d.svx[src]; d.dvy[j] = d.svy[src]; d.dvz[j] = d.svz[src];
+            d.dm[j] = d.sm[src];
+            d.dfix[j] = d.sfix[src];
+            d.did[j] = d.sid[src];
+            if (i + 1 < end) {
+                const uint64_t src2 = min((uint64_t)svals[i + 1] - d.off, d.n - 1);
+                if (fabs(x - d.sx[src2]) < 1e-9 && fabs(y - d.sy[src2]) < 1e-9 &&
+                    fabs(z - d.sz[src2]) < 1e-9)
+                    atomicOr(&d.st->err, kErrCoincident);
+            }
+        }
+        sum += cnt[k];
+    }
+    uint32_t tile_total;
+    const uint32_t texcl = block_exclusive_scan(sum, s_scan, &tile_total);
+    if (tid == 0) {
+        uint64_t* my = status + tile;
+        if (tile == 0) {
+            status_store(my, status_pack(tag, kFlagPrefix, tile_total));
+            s_excl = 0;
+        } else {
+            status_store(my, status_pack(tag, kFlagAgg, tile_total));
+            const uint32_t ex = lookback(status, tile, 1, 0, tag, err);
+            status_store(my, status_pack(tag, kFlagPrefix, ex + tile_total));
+            s_excl = ex;
+        }
+    }
+    __syncthreads();
+    uint32_t run = s_excl + texcl;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k;
+        if (i < total) first_node[i] = run;
+        run += cnt[k];
+        if (i + 1 == total) first_node[total] = run;
+    }
+}
+
+// Last sorted index j >= i whose key shares i's prefix at `level`.
+template <int DIM>
+__device__ __forceinline__ uint64_t range_end(const uint64_t* __restrict__ skeys, uint64_t i,
+                                              uint64_t end, int lmax, int level) {
+    const uint64_t P = key_prefix<DIM>(skeys[i], lmax, level);
+    uint64_t last = i, step = 1;
+    for (;;) {
+        const uint64_t probe = last + step;
+        if (probe >= end || key_prefix<DIM>(skeys[probe], lmax, level) != P) break;
+        last = probe;
+        step <<= 1;
+    }
+    uint64_t hi = min(last + step, end);
+    while (hi - last > 1) {
+        const uint64_t mid = (last + hi) >> 1;
+        if (key_prefix<DIM>(skeys[mid], lmax, level) == P) last = mid; else hi = mid;
+    }
+    return last;
+}
+
+// First sorted index j <= i (j >= begin) whose key shares i's prefix at `level`.
+template <int DIM>
+__device__ __forceinline__ uint64_t range_begin(const uint64_t* __restrict__ skeys, uint64_t i,
+                                                uint64_t begin, int lmax, int level) {
+    const uint64_t P = key_prefix<DIM>(skeys[i], lmax, level);
+    uint64_t first = i, step = 1;
+    for (;;) {
+        if (first < begin + step) break;
+        const uint64_t probe = first - step;
+        if (key_prefix<DIM>(skeys[probe], lmax, level) != P) break;
+        first = probe;
+        step <<= 1;
+    }
+    // keys[first] matches; keys[lo] (if lo >= begin and lo < first) may not.
+    uint64_t lo = first >= begin + step ? first - step : begin;
+    if (key_prefix<DIM>(skeys[lo], lmax, level) == P) return lo;
+    while (first - lo > 1) {
+        const uint64_t mid = (lo + first) >> 1;
+        if (key_prefix<DIM>(skeys[mid], lmax, level) == P) first = mid; else lo = mid;
+    }
+    return first;
+}
+
+// Writes every node a sorted body owns: geometric centre by the reference's
+// descent, half-width level, sorted range, skip (= first node of the body
+// after the range), parent; a leaf takes its body's exact position and mass.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_nodes(
+    const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys,
+    uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ first_node,
+    const uint8_t* __restrict__ lo_level, NodeHot* __restrict__ hot, NodeCold* __restrict__ cold,
+    uint64_t cap) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint64_t key = skeys[i];
+        const uint32_t s = seg_of_key(key, segbits);
+        if (s >= nseg) continue;
+        const SegDesc& d = desc[s];
+        const uint64_t end = d.off + d.n;
+        const int lo = lo_level[i];
+        const uint32_t nbase = first_node[i];
+        const int hi = lo + (int)(first_node[i + 1] - nbase) - 1;
+        double ext = bits_f64(d.st->extent_bits);
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int L = 0; L <= hi; ++L) {
+            if (L >= lo) {
+                const uint64_t nd = (uint64_t)nbase + (uint64_t)(L - lo);
+                if (nd >= cap) {
+                    atomicOr(&d.st->err, kErrCapacity);
+                    break;
+                }
+                const bool leaf = L == hi;
+                const uint64_t last = leaf ? i : range_end<DIM>(skeys, i, end, lmax, L);
+                int32_t parent;
+                if (L > lo) parent = (int32_t)(nd - 1);
+                else if (L == 0) parent = -1;
+                else {
+                    const uint64_t a = range_begin<DIM>(skeys, i, d.off, lmax, L - 1);
+                    parent = (int32_t)(first_node[a] + (uint32_t)(L - 1 - lo_level[a]));
+                }
+                NodeHot h;
+                h.cx = cx; h.cy = cy; h.cz = cz;
+                if (leaf) {
+                    const uint64_t j = i - d.off;
+                    h.mx = d.dx[j]; h.my = d.dy[j]; h.mz = d.dz[j];
+                    h.mass = d.dm[j];
+                } else {
+                    h.mx = h.my = h.mz = 0.0;
+                    h.mass = 0.0;
+                }
+                h.skip = first_node[last + 1];
+                h.level = (uint32_t)L;
+                hot[nd] = h;
+                NodeCold c;
+                c.first = (uint32_t)i;
+                c.last = (uint32_t)last;
+                c.parent = parent;
+                c.arrivals = 0;
+                cold[nd] = c;
+            }
+            if (L < hi) {
+                const uint32_t o = (uint32_t)(key >> (DIM * (lmax - 1 - L))) & ((1u << DIM) - 1u);
+                const double h2 = ext / 2.0;
+                cx = (o & 1u) ? cx + h2 : cx - h2;
+                cy = (o & 2u) ? cy + h2 : cy - h2;
+                if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+                ext = h2;
+            }
+        }
+        atomicMax(&d.st->max_level, (uint32_t)hi);
+    }
+}
+
+// Bottom-up mass and centre of mass: each leaf climbs; at every parent the
+// thread that completes the parent's body count sums the children (octant
+// order, so the result does not depend on which thread arrives last).
+__global__ __launch_bounds__(kBlock) void k_com(uint64_t total,
+                                                const uint32_t* __restrict__ first_node,
+                                                NodeHot* hot, NodeCold* cold, uint64_t cap) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        uint32_t nd = first_node[i + 1] - 1u;
+        if (nd >= cap) continue;
+        for (;;) {
+            const int32_t p = cold[nd].parent;
+            if (p < 0 || (uint32_t)p >= nd) break;  // parents precede children in preorder
+            const uint32_t nb = cold[nd].last - cold[nd].first + 1u;
+            const uint32_t need = cold[p].last - cold[p].first + 1u;
+            __threadfence();
+            const uint32_t old = atomicAdd(&cold[p].arrivals, nb);
+            if (old + nb != need) break;
+            __threadfence();
+            double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+            const uint32_t send = __hip_atomic_load(&hot[p].skip, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+            for (uint32_t c = (uint32_t)p + 1u; c < send;) {
+                const double m = __hip_atomic_load(&hot[c].mass, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                const double x = __hip_atomic_load(&hot[c].mx, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                const double y = __hip_atomic_load(&hot[c].my, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                const double z = __hip_atomic_load(&hot[c].mz, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                M += m;
+                X += m * x;
+                Y += m * y;
+                Z += m * z;
+                const uint32_t nxt = hot[c].skip;
+                c = nxt > c ? nxt : c + 1u;
+            }
+            const double inv = 1.0 / M;
+            __hip_atomic_store(&hot[p].mx, X * inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&hot[p].my, Y * inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&hot[p].mz, Z * inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&hot[p].mass, M, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            nd = (uint32_t)p;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- forces
+struct Kahan3 {
+    float sx, sy, sz, cx, cy, cz;
+    __device__ __forceinline__ void zero() { sx = sy = sz = cx = cy = cz = 0.0f; }
+    __device__ __forceinline__ static void add1(float& s, float& c, float v) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ void add(float x, float y, float z) {
+        add1(sx, cx, x);
+        add1(sy, cy, y);
+        add1(sz, cz, z);
+    }
+};
+
+// Pair acceleration on body a from (b, mb), mixed precision: the separation
+// b - a is taken in FP64 and rounded once; |F|/m_a = mb / (s + e) along the
+// unsoftened direction (astro/src/lib.rs:84-113, with m_a cancelled).
+__device__ __forceinline__ void pair_mixed(double ax, double ay, double az, double bx, double by,
+                                           double bz, double mb, float ef, Kahan3& acc) {
+    const float fx = (float)(bx - ax), fy = (float)(by - ay), fz = (float)(bz - az);
+    const float s = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+    if (s > 0.0f) {
+        const float w = (float)mb * __builtin_amdgcn_rsqf(s) * __builtin_amdgcn_rcpf(s + ef);
+        acc.add(fx * w, fy * w, fz * w);
+    }
+}
+
+// Pair force on a from b exactly as astro/src/lib.rs:84-113 computes it (FP64,
+// same operation order; powi(2) is x*x and powf(0.5) lowers to sqrt).
+__device__ __forceinline__ void pair_fp64(double ax, double ay, double az, double am, double bx,
+                                          double by, double bz, double bm, double e, double f[3]) {
+    const double d0 = ax - bx, d1 = ay - by, d2 = az - bz;
+    const double rn = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double re = d0 * d0 + d1 * d1 + d2 * d2 + e;
+    const double r0 = (bx - ax) / rn, r1 = (by - ay) / rn, r2 = (bz - az) / rn;
+    f[0] += r0 * 1.0 * am * bm / re;
+    f[1] += r1 * 1.0 * am * bm / re;
+    f[2] += r2 * 1.0 * am * bm / re;
+}
+
+// Barnes-Hut walk of one sorted body per lane, wave-coherent and stackless:
+// the wave steps through the preorder node array together (one node address
+// per wave, so node loads are scalar); each lane applies the reference's own
+// opening test (octree.rs:137-157: accept iff half_width / |x - centre| <
+// theta, at every node, own cell included) and skips an accepted subtree
+// through its skip index; the wave descends into a node iff some lane opened
+// it. Then the Stormer update (verlet.rs:24-82) in place on the permuted
+// state, or the acceleration into acc_out (upload order) without it.
+template <int DIM, bool FP64, bool INTEGRATE>
+__global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ desc, uint64_t total,
+                                                 const uint64_t* __restrict__ skeys, int segbits,
+                                                 const uint32_t* __restrict__ first_node,
+                                                 const NodeHot* __restrict__ hot, uint64_t cap,
+                                                 uint32_t nseg, double* __restrict__ acc_out) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = g < total;
+    uint32_t s = 0;
+    double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
+    bool fixed = true;
+    uint64_t j = 0;
+    if (valid) {
+        s = seg_of_key(skeys[g], segbits);
+        valid = s < nseg;
+    }
+    if (valid) {
+        const SegDesc& d = desc[s];
+        j = g - d.off;
+        x = d.dx[j]; y = d.dy[j]; z = d.dz[j];
+        m = d.dm[j];
+        fixed = d.dfix[j] != 0;
+    }
+    Kahan3 ka;
+    ka.zero();
+    double f64[3] = {0.0, 0.0, 0.0};
+    uint32_t count = 0;
+    bool pending = valid;
+    while (__ballot(pending)) {
+        const int lead = __ffsll((long long)__ballot(pending)) - 1;
+        const uint32_t s0 = uniform_u32(__shfl(s, lead));
+        const bool mine = pending && s == s0;
+        const SegDesc& d0 = desc[s0];
+        const uint32_t root = uniform_u32(first_node[d0.off]);
+        const uint32_t end = uniform_u32(min((uint64_t)first_node[d0.off + d0.n], cap));
+        const double E = bits_f64(d0.st->extent_bits);
+        const double theta = d0.theta;
+        const double e = d0.e;
+        const float ef = (float)e;
+        const bool walker = mine && !fixed;
+        uint32_t skip_until = 0;
+        uint32_t nd = root;
+        while (nd < end) {
+            const NodeHot h = hot[nd];
+            const bool leaf = h.skip == nd + 1u;
+            bool open = false;
+            if (walker && nd >= skip_until) {
+                bool take = leaf;
+                if (!leaf) {
+                    const double d0x = x - h.cx, d0y = y - h.cy, d0z = z - h.cz;
+                    const double r = sqrt(d0x * d0x + d0y * d0y + d0z * d0z);
+                    take = ldexp(E, -(int)h.level) / r < theta;
+                }
+                if (take) {
+                    ++count;
+                    skip_until = h.skip;
+                    if (!(h.mx == x && h.my == y && h.mz == z)) {
+                        if (FP64)
+                            pair_fp64(x, y, z, m, h.mx, h.my, h.mz, h.mass, e, f64);
+                        else
+                            pair_mixed(x, y, z, h.mx, h.my, h.mz, h.mass, ef, ka);
+                    }
+                } else {
+                    open = true;
+                }
+            }
+            nd = uniform_u32(__ballot(open) || h.skip <= nd ? nd + 1u : h.skip);
+        }
+        pending = pending && !mine;
+    }
+    if (!valid) return;
+    const SegDesc& d = desc[s];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (!fixed) {
+        if (FP64) {
+            ax = f64[0] / m; ay = f64[1] / m; az = f64[2] / m;
+        } else if (m == 0.0) {
+            ax = ay = az = __builtin_nan("");
+        } else {
+            ax = (double)ka.sx - (double)ka.cx;
+            ay = (double)ka.sy - (double)ka.cy;
+            az = (double)ka.sz - (double)ka.cz;
+        }
+    }
+    if (d.count) d.counts[d.did[j]] = count;
+    atomicAdd((unsigned long long*)&d.st->interactions, (unsigned long long)count);
+    if (INTEGRATE) {
+        const double dt = d.dt;
+        const double dt2 = dt * dt;
+        double nx, ny, nz, vx, vy, vz;
+        if (d.st->history == 0) {
+            const double ux = d.dvx[j], uy = d.dvy[j], uz = d.dvz[j];
+            nx = x + ux * dt + 0.5 * ax * dt2;
+            ny = y + uy * dt + 0.5 * ay * dt2;
+            nz = z + uz * dt + 0.5 * az * dt2;
+            vx = ux + ax * dt;
+            vy = uy + ay * dt;
+            vz = uz + az * dt;
+        } else {
+            nx = 2.0 * x - d.dpx[j] + ax * dt2;
+            ny = 2.0 * y - d.dpy[j] + ay * dt2;
+            nz = 2.0 * z - d.dpz[j] + az * dt2;
+            vx = (nx - x) / dt;
+            vy = (ny - y) / dt;
+            vz = (nz - z) / dt;
+        }
+        d.dpx[j] = x; d.dpy[j] = y; d.dpz[j] = z;
+        d.dx[j] = nx; d.dy[j] = ny; d.dz[j] = nz;
+        d.dvx[j] = vx; d.dvy[j] = vy; d.dvz[j] = vz;
+    } else {
+        double* o = acc_out + 3 * (d.off + d.did[j]);
+        o[0] = ax; o[1] = ay; o[2] = az;
+    }
+}
+
+// simple_astro (transformers.rs:219-244): LDS-tiled all-pairs sum. Each block
+// holds 256 targets of one segment and streams the segment's sources through
+// LDS. Mixed precision sums each 256-source tile in FP32 and adds the tile
+// partial into a compensated FP32 sum. Reads src, writes dst (same order).
+template <bool FP64, bool INTEGRATE>
+__global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ desc,
+                                                   const uint32_t* __restrict__ blk_seg,
+                                                   const uint32_t* __restrict__ blk_tile,
+                                                   double* __restrict__ acc_out) {
+    __shared__ double4 sj[kBlock];
+    const uint32_t s = blk_seg[blockIdx.x];
+    const SegDesc& d = desc[s];
+    const uint64_t i = (uint64_t)blk_tile[blockIdx.x] * kBlock + threadIdx.x;
+    const bool valid = i < d.n;
+    double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
+    bool fixed = true;
+    if (valid) {
+        x = d.sx[i]; y = d.sy[i]; z = d.sz[i];
+        m = d.sm[i];
+        fixed = d.sfix[i] != 0;
+    }
+    const double e = d.e;
+    const float ef = (float)e;
+    Kahan3 ka;
+    ka.zero();
+    double f64[3] = {0.0, 0.0, 0.0};
+    for (uint64_t t0 = 0; t0 < d.n; t0 += kBlock) {
+        const uint64_t jj = t0 + threadIdx.x;
+        if (jj < d.n) sj[threadIdx.x] = make_double4(d.sx[jj], d.sy[jj], d.sz[jj], d.sm[jj]);
+        __syncthreads();
+        const int cnt = (int)min((uint64_t)kBlock, d.n - t0);
+        if (FP64) {
+            for (int k = 0; k < cnt; ++k) {
+                const double4 b = sj[k];
+                if (!(b.x == x && b.y == y && b.z == z))
+                    pair_fp64(x, y, z, m, b.x, b.y, b.z, b.w, e, f64);
+            }
+        } else {
+            float px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll 4
+            for (int k = 0; k < cnt; ++k) {
+                const double4 b = sj[k];
+                const float fx = (float)(b.x - x), fy = (float)(b.y - y), fz = (float)(b.z - z);
+                const float s2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                const float t = s2 + ef;
+                float w = (float)b.w * __builtin_amdgcn_rsqf(s2 * t * t);
+                w = s2 > 0.0f ? w : 0.0f;
+                px = fmaf(fx, w, px);
+                py = fmaf(fy, w, py);
+                pz = fmaf(fz, w, pz);
+            }
+            ka.add(px, py, pz);
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (!fixed) {
+        if (FP64) {
+            ax = f64[0] / m; ay = f64[1] / m; az = f64[2] / m;
+        } else if (m == 0.0) {
+            ax = ay = az = __builtin_nan("");
+        } else {
+            ax = (double)ka.sx - (double)ka.cx;
+            ay = (double)ka.sy - (double)ka.cy;
+            az = (double)ka.sz - (double)ka.cz;
+        }
+    }
+    if (INTEGRATE) {
+        const double dt = d.dt;
+        const double dt2 = dt * dt;
+        double nx, ny, nz, vx, vy, vz;
+        if (d.st->history == 0) {
+            const double ux = d.svx[i], uy = d.svy[i], uz = d.svz[i];
+            nx = x + ux * dt + 0.5 * ax * dt2;
+            ny = y + uy * dt + 0.5 * ay * dt2;
+            nz = z + uz * dt + 0.5 * az * dt2;
+            vx = ux + ax * dt;
+            vy = uy + ay * dt;
+            vz = uz + az * dt;
+        } else {
+            nx = 2.0 * x - d.spx[i] + ax * dt2;
+            ny = 2.0 * y - d.spy[i] + ay * dt2;
+            nz = 2.0 * z - d.spz[i] + az * dt2;
+            vx = (nx - x) / dt;
+            vy = (ny - y) / dt;
+            vz = (nz - z) / dt;
+        }
+        d.dpx[i] = x; d.dpy[i] = y; d.dpz[i] = z;
+        d.dx[i] = nx; d.dy[i] = ny; d.dz[i] = nz;
+        d.dvx[i] = vx; d.dvy[i] = vy; d.dvz[i] = vz;
+        d.dm[i] = m;
+        d.dfix[i] = d.sfix[i];
+        d.did[i] = d.sid[i];
+    } else {
+        double* o = acc_out + 3 * (d.off + d.sid[i]);
+        o[0] = ax; o[1] = ay; o[2] = az;
+    }
+}
+
+// FP64 energy terms of one state: per block partial sums of
+// {V (half of the all-pairs sum), K, Px, Py, Pz}. U(r) = -(ma mb / sqrt(e))
+// atan(sqrt(e) / r) is the potential of |F| = ma mb / (r^2 + e).
+__global__ __launch_bounds__(kBlock) void k_energy(const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ z,
+                                                   const double* __restrict__ vx,
+                                                   const double* __restrict__ vy,
+                                                   const double* __restrict__ vz,
+                                                   const double* __restrict__ m, uint64_t n,
+                                                   double e, double* __restrict__ partial) {
+    __shared__ double4 sj[kBlock];
+    __shared__ double red[5][kBlock];
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    double xi = 0, yi = 0, zi = 0, mi = 0;
+    if (valid) { xi = x[i]; yi = y[i]; zi = z[i]; mi = m[i]; }
+    const double se = sqrt(e);
+    double V = 0.0;
+    for (uint64_t t0 = 0; t0 < n; t0 += kBlock) {
+        const uint64_t jj = t0 + threadIdx.x;
+        if (jj < n) sj[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
+        __syncthreads();
+        const int cnt = (int)min((uint64_t)kBlock, n - t0);
+        if (valid) {
+            for (int k = 0; k < cnt; ++k) {
+                const double4 b = sj[k];
+                if (b.x == xi && b.y == yi && b.z == zi) continue;
+                const double d0 = xi - b.x, d1 = yi - b.y, d2 = zi - b.z;
+                const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                const double mm = mi * b.w;
+                V += e > 0.0 ? -(mm / se) * atan(se / r) : -mm / r;
+            }
+        }
+        __syncthreads();
+    }
+    double K = 0, Px = 0, Py = 0, Pz = 0;
+    if (valid) {
+        K = 0.5 * mi * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+        Px = mi * vx[i]; Py = mi * vy[i]; Pz = mi * vz[i];
+    }
+    red[0][threadIdx.x] = 0.5 * V;
+    red[1][threadIdx.x] = K;
+    red[2][threadIdx.x] = Px;
+    red[3][threadIdx.x] = Py;
+    red[4][threadIdx.x] = Pz;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int c = 0; c < 5; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) partial[5 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// AoS <-> SoA at the host boundary.
+__global__ void k_unpack(const Entity* __restrict__ aos, uint64_t n, double* x, double* y,
+                         double* z, double* px, double* py, double* pz, double* vx, double* vy,
+                         double* vz, double* m, uint8_t* fix, uint32_t* id) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Entity e = aos[i];
+    x[i] = e.x; y[i] = e.y; z[i] = e.z;
+    px[i] = e.x; py[i] = e.y; pz[i] = e.z;
+    vx[i] = e.vx; vy[i] = e.vy; vz[i] = e.vz;
+    m[i] = e.mass;
+    fix[i] = e.fixed ? 1 : 0;
+    id[i] = (uint32_t)i;
+}
+
+__global__ void k_pack(Entity* __restrict__ aos, uint64_t n, const double* x, const double* y,
+                       const double* z, const double* vx, const double* vy, const double* vz,
+                       const uint32_t* id) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    Entity& e = aos[id[j]];
+    e.x = x[j]; e.y = y[j]; e.z = z[j];
+    e.vx = vx[j]; e.vy = vy[j]; e.vz = vz[j];
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace pg

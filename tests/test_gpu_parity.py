@@ -1,0 +1,299 @@
+"""GPU parity of the astro2 / astro / simple_astro transforms and the Stormer
+step against the oracle, through libphysim_gpu.so's C-ABI.
+
+Bars (stated per test):
+  * tree topology, node geometric centres and half-widths, interaction-list
+    lengths: bit-exact;
+  * FP64 mode accelerations: p99.9 <= 1e-12 relative, max <= 1e-8. The only
+    differences are summation order and the rounding order of each cell's
+    centre of mass (the reference folds bodies in insertion order, the device
+    sums children bottom-up). A heavy body whose own ancestor cell is accepted
+    (the reference's opening rule allows it) sees that cell's centre of mass
+    ~1e-10 away, so the pair direction is a cancellation and inherits the COM
+    rounding difference; with theta = 0 (no pseudo-particles) the bar is 1e-12
+    for every body;
+  * mixed mode (FP32 pair forces, compensated sums): p99.9 relative error
+    <= 1e-4, max <= 1e-3;
+  * Stormer update: bit-exact given identical accelerations.
+"""
+import numpy as np
+import pytest
+
+import physim_amd as P
+from physim_amd.plugin import PluginLibrary, zero_acc
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CFG1 = dict(theta=1.5, e=0.5)
+
+
+def rel_err(a, ref):
+    na = np.linalg.norm(ref, axis=1)
+    return np.linalg.norm(a - ref, axis=1) / np.where(na > 0, na, 1.0)
+
+
+def fp64_ok(r):
+    return np.quantile(r, 0.999) <= 1e-12 and r.max() <= 1e-8
+
+
+def small_cube(n, seed=1):
+    return P.cube_and_stars(n, seed=seed)
+
+
+# ------------------------------------------------------------- transforms
+@pytest.mark.parametrize("kind,dim", [("astro2", 3), ("astro", 2)])
+def test_bh_fp64_matches_oracle(engine, kind, dim):
+    s = small_cube(10000)
+    ref, counts, fault = O.bh_transform(s, CFG1["theta"], CFG1["e"], dim=dim, counts=True)
+    assert fault == 0
+    sim = P.Sim(engine, s, kind=kind, precision="fp64", **CFG1)
+    sim.set_counting(True)
+    a = sim.accel()
+    np.testing.assert_array_equal(sim.interaction_counts(), counts)
+    r = rel_err(a, ref)
+    print(f"{kind} fp64 rel err p99.9 {np.quantile(r, 0.999):.3e} max {r.max():.3e}")
+    assert fp64_ok(r)
+
+
+@pytest.mark.parametrize("kind,dim", [("astro2", 3), ("astro", 2)])
+@pytest.mark.parametrize("theta", [0.3, 0.7, 1.0, 1.5])
+def test_bh_mixed_matches_oracle(engine, kind, dim, theta):
+    s = small_cube(6000, seed=2)
+    ref, counts, fault = O.bh_transform(s, theta, 0.5, dim=dim, counts=True)
+    sim = P.Sim(engine, s, kind=kind, precision="mixed", theta=theta, e=0.5)
+    sim.set_counting(True)
+    a = sim.accel()
+    np.testing.assert_array_equal(sim.interaction_counts(), counts)
+    r = rel_err(a, ref)
+    p999 = np.quantile(r, 0.999)
+    print(f"{kind} theta {theta} mixed rel err p99.9 {p999:.3e} max {r.max():.3e}")
+    assert p999 <= 1e-4 and r.max() <= 1e-3
+
+
+@pytest.mark.parametrize("kind,dim", [("astro2", 3), ("astro", 2)])
+def test_tree_topology_bit_exact(engine, kind, dim):
+    for seed, n in ((1, 10000), (5, 3000), (9, 257)):
+        s = small_cube(n, seed=seed)
+        sim = P.Sim(engine, s, kind=kind, precision="fp64", **CFG1)
+        sim.accel()
+        rec, fl = sim.tree()
+        t = O.Tree.from_state(s, dim)
+        orec, ofl = t.serialize()
+        assert t.merges == 0 and t.fault == 0
+        assert rec.shape == orec.shape
+        np.testing.assert_array_equal(rec[:, :3], orec[:, :3])      # level, octant, leaf body
+        np.testing.assert_array_equal(fl[:, 4:8], ofl[:, 4:8])      # centres, half-widths: exact
+        np.testing.assert_allclose(fl[:, 0], ofl[:, 0], rtol=1e-13)  # mass
+        np.testing.assert_allclose(fl[:, 1:4], ofl[:, 1:4], rtol=1e-13, atol=1e-15)
+        leaf = rec[:, 2] >= 0
+        np.testing.assert_array_equal(fl[leaf, 1:4], ofl[leaf, 1:4])  # leaves hold exact bodies
+
+
+def octant_path_key(p, extent, dim, levels):
+    c = [0.0, 0.0, 0.0]
+    h = extent
+    key = 0
+    for _ in range(levels):
+        o = int(p[0] > c[0]) | (int(p[1] > c[1]) << 1)
+        if dim == 3:
+            o |= int(p[2] > c[2]) << 2
+        h2 = h / 2.0
+        c[0] = c[0] + h2 if o & 1 else c[0] - h2
+        c[1] = c[1] + h2 if o & 2 else c[1] - h2
+        if dim == 3:
+            c[2] = c[2] + h2 if o & 4 else c[2] - h2
+        h = h2
+        key = (key << dim) | o
+    return key
+
+
+@pytest.mark.parametrize("kind,dim,levels", [("astro2", 3, 21), ("astro", 2, 32)])
+def test_morton_order_and_permutation(engine, kind, dim, levels):
+    s = small_cube(3000, seed=3)
+    sim = P.Sim(engine, s, kind=kind, **CFG1)
+    sim.accel()
+    keys, perm = sim.morton()
+    ext = O.extent(s)
+    ref = np.array([octant_path_key((b["x"], b["y"], b["z"]), ext, dim, levels) for b in s],
+                   dtype=np.uint64)
+    order = np.lexsort((np.arange(len(s)), ref))  # stable: ties by upload index
+    np.testing.assert_array_equal(keys, ref[order])
+    np.testing.assert_array_equal(perm, order.astype(np.uint32))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
+def test_simple_astro_matches_oracle(engine, precision, tol):
+    s = small_cube(3000, seed=4)
+    ref = O.simple_transform(s, 0.5)
+    sim = P.Sim(engine, s, kind="simple_astro", precision=precision, e=0.5)
+    a = sim.accel()
+    r = rel_err(a, ref)
+    print(f"simple_astro {precision} p99.9 {np.quantile(r, 0.999):.3e} max {r.max():.3e}")
+    assert np.quantile(r, 0.999) <= tol
+
+
+def test_theta_zero_is_direct_sum(engine):
+    s = small_cube(2000, seed=6)
+    ref = O.simple_transform(s, 0.5)
+    for kind in ("astro2", "astro"):
+        a = P.Sim(engine, s, kind=kind, precision="fp64", theta=0.0, e=0.5).accel()
+        assert rel_err(a, ref).max() <= 1e-12
+
+
+def test_cube_bodies_fp64_exact_to_rounding(engine):
+    """Without the two heavy stars no body's own ancestor pseudo-particle is
+    ill-conditioned: every body agrees to 1e-12 in FP64 mode."""
+    s = P.gen_cube(8000, seed=11, spin=1000.0)
+    ref, _, _ = O.bh_transform(s, 1.5, 0.5)
+    a = P.Sim(engine, s, kind="astro2", precision="fp64", theta=1.5, e=0.5).accel()
+    assert rel_err(a, ref).max() <= 1e-12
+
+
+# ------------------------------------------------------------ integration
+def test_stormer_steps_fp64_match_oracle(engine):
+    """cfg1 pipeline (cube 10000 + 2 stars, astro2 theta 1.5 e 0.5, verlet
+    dt 1e-5) for 100 steps. FP64 device steps track the oracle to summation
+    order; positions agree to 1e-9 relative after 100 steps."""
+    s = small_cube(10000)
+    ref = s.copy()
+    O.pipeline_steps(ref, "astro2", 1.5, 0.5, 1e-5, 100)
+    sim = P.Sim(engine, s, kind="astro2", precision="fp64", theta=1.5, e=0.5, dt=1e-5)
+    sim.step(100)
+    out = sim.download()
+    for f in ("radius", "mass", "id", "fixed"):
+        np.testing.assert_array_equal(out[f], s[f])
+    pos = np.stack([out["x"], out["y"], out["z"]], 1)
+    rpos = np.stack([ref["x"], ref["y"], ref["z"]], 1)
+    err = np.linalg.norm(pos - rpos, axis=1) / np.linalg.norm(rpos, axis=1)
+    print(f"fp64 100-step position rel err max {err.max():.3e}")
+    assert err.max() <= 1e-9
+    assert sim.stats()["steps_done"] == 100
+
+
+def test_stormer_steps_mixed_within_tolerance(engine):
+    """Same pipeline in mixed precision: positions after 100 steps within
+    1e-6 relative (p99.9) of the FP64 oracle."""
+    s = small_cube(10000)
+    ref = s.copy()
+    O.pipeline_steps(ref, "astro2", 1.5, 0.5, 1e-5, 100)
+    sim = P.Sim(engine, s, kind="astro2", precision="mixed", theta=1.5, e=0.5, dt=1e-5)
+    sim.step(100)
+    out = sim.download()
+    pos = np.stack([out["x"], out["y"], out["z"]], 1)
+    rpos = np.stack([ref["x"], ref["y"], ref["z"]], 1)
+    err = np.linalg.norm(pos - rpos, axis=1) / np.linalg.norm(rpos, axis=1)
+    print(f"mixed 100-step position rel err p99.9 {np.quantile(err, 0.999):.3e} max {err.max():.3e}")
+    assert np.quantile(err, 0.999) <= 1e-6
+
+
+def test_stormer_update_bit_exact(engine):
+    """With identical accelerations the device update equals the oracle's
+    bit for bit: two bodies far apart with the other fixed see a = 0 exactly
+    (fixed bodies get no acceleration), so both sides integrate a = 0."""
+    s = O.concat([O.gen_star(x=1.0, y=2.0, z=3.0, vx=0.3, vy=-1.0, vz=2.5, mass=1.0, fixed=True),
+                  O.gen_star(x=-1.5, y=0.25, z=0.5, vx=-0.7, vy=0.1, vz=0.0, mass=2.0, fixed=True)])
+    ref = s.copy()
+    O.pipeline_steps(ref, "astro2", 1.0, 0.1, 1e-3, 7)
+    sim = P.Sim(engine, s, kind="astro2", theta=1.0, e=0.1, dt=1e-3)
+    sim.step(7)
+    out = sim.download()
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(out[f], ref[f])
+
+
+@pytest.mark.parametrize("kind", ["astro2", "astro", "simple_astro"])
+def test_graph_and_eager_steps_identical(engine, kind):
+    s = small_cube(4000, seed=8)
+    a = P.Sim(engine, s, kind=kind, theta=1.0, e=0.5, dt=1e-5)
+    a.step(6)
+    engine.set_graphs(False)
+    try:
+        b = P.Sim(engine, s, kind=kind, theta=1.0, e=0.5, dt=1e-5)
+        for _ in range(6):
+            b.step(1)
+    finally:
+        engine.set_graphs(True)
+    assert a.download().tobytes() == b.download().tobytes()
+
+
+# ---------------------------------------------------------------- batching
+def test_batch_equals_alone(engine):
+    states = [small_cube(n, seed=k) for k, n in enumerate((5000, 1, 2, 777, 12000))]
+    thetas = (1.5, 0.7, 1.0, 0.5, 1.2)
+    alone = []
+    for st, th in zip(states, thetas):
+        sim = P.Sim(engine, st, theta=th, e=0.5, dt=1e-5)
+        sim.step(5)
+        alone.append(sim.download())
+    sims = [P.Sim(engine, st, theta=th, e=0.5, dt=1e-5) for st, th in zip(states, thetas)]
+    engine.batch_step(sims, 5)
+    for sim, ref in zip(sims, alone):
+        assert sim.download().tobytes() == ref.tobytes()
+
+
+# ------------------------------------------------------------ plugin path
+def test_plugin_transform_accumulates(engine):
+    pl = PluginLibrary()
+    s = small_cube(5000, seed=7)
+    el = pl.load("astro2", {"theta": 1.5, "e": 0.5, "precision": "fp64"})
+    acc = zero_acc(len(s))
+    acc["x"] = 1.0
+    el.transform(s, acc)
+    ref, _, _ = O.bh_transform(s, 1.5, 0.5)
+    got = acc.view(np.float64).reshape(-1, 3).copy()
+    got[:, 0] -= 1.0
+    assert fp64_ok(rel_err(got, ref))
+    el.close()
+    for name, kind in (("astro", 2), ("simple_astro", 0)):
+        el = pl.load(name, {"theta": 0.9, "e": 0.25, "precision": "fp64"})
+        acc = zero_acc(len(s))
+        el.transform(s, acc)
+        if kind:
+            ref, _, _ = O.bh_transform(s, 0.9, 0.25, dim=2)
+        else:
+            ref = O.simple_transform(s, 0.25)
+        assert fp64_ok(rel_err(acc.view(np.float64).reshape(-1, 3), ref))
+        el.close()
+
+
+# -------------------------------------------------------------- edge cases
+def test_small_and_special_states(engine):
+    one = P.gen_star(x=0.3, y=0.1, z=0.2, mass=2.0)
+    a = P.Sim(engine, one, theta=1.0, e=0.5).accel()
+    assert np.all(a == 0.0)
+    two = P.concat([P.gen_star(x=0.0, mass=2.0), P.gen_star(x=3.0, y=4.0, mass=5.0)])
+    ref, _, _ = O.bh_transform(two, 1.0, 0.5)
+    for prec in ("fp64", "mixed"):
+        a = P.Sim(engine, two, theta=1.0, e=0.5, precision=prec).accel()
+        assert rel_err(a, ref).max() <= (1e-15 if prec == "fp64" else 1e-6)
+    # fixed bodies get no acceleration; zero-mass bodies get NaN (as the reference)
+    s = small_cube(500, seed=2)
+    s["fixed"][:10] = 1
+    s["mass"][10] = 0.0
+    ref, _, _ = O.bh_transform(s, 1.0, 0.5)
+    for prec in ("fp64", "mixed"):
+        a = P.Sim(engine, s, theta=1.0, e=0.5, precision=prec).accel()
+        assert np.all(a[:10] == 0.0)
+        assert np.all(np.isnan(a[10])) and np.all(np.isnan(ref[10]))
+        ok = np.arange(len(s)) > 10
+        r = rel_err(a[ok], ref[ok])
+        assert fp64_ok(r) if prec == "fp64" else r.max() <= 1e-3
+
+
+def test_errors_are_reported(engine):
+    s = small_cube(300, seed=3)
+    bad = s.copy()
+    bad["x"][5] = np.nan
+    with pytest.raises(P.PgError) as ei:
+        P.Sim(engine, bad, theta=1.0, e=0.5).accel()
+    assert ei.value.code == 7
+    dup = s.copy()
+    dup["x"][7], dup["y"][7], dup["z"][7] = dup["x"][8], dup["y"][8], dup["z"][8]
+    with pytest.raises(P.PgError) as ei:
+        P.Sim(engine, dup, theta=1.0, e=0.5).accel()
+    assert ei.value.code in (5, 6)
+    # the engine keeps working after a failed simulation
+    a = P.Sim(engine, s, theta=1.0, e=0.5, precision="fp64").accel()
+    ref, _, _ = O.bh_transform(s, 1.0, 0.5)
+    assert fp64_ok(rel_err(a, ref))
