@@ -1,0 +1,178 @@
+"""Throughput benchmark: astro2 (Barnes-Hut octree, theta 1.5, e 0.5) +
+Stormer verlet (dt 1e-5) on the 1,048,576-body cube + 2 stars workload,
+state resident on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
+                    [--bodies N] [--no-cpu-baseline] [--profile-steps P]
+
+Prints one JSON line: body-steps/s over K timed steps (inputs already in HBM
+when the clock starts), the roofline of the dominant kernel (its algorithmic
+bytes per launch over its HIP-event-timed average duration, from an eager
+pass over the same workload after the timed region) and the CPU baseline (the
+oracle's reference-faithful restatement of astro2 + verlet, single thread, on
+a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(kernel, n, nodes, passes, interactions):
+    """Compulsory bytes one launch of `kernel` moves for n bodies (each
+    input read once, each output written once; see DESIGN.md's table)."""
+    internal = max(nodes - n, 0)
+    table = {
+        "prep": 0.0,
+        "extent": 24.0 * n,                         # read x, y, z
+        "keys": 36.0 * n,                           # read x, y, z; write key, index
+        "sort_pass": 24.0 * n,                      # read + write (key, index)
+        "scan_gather": (8 + 4 + 85 + 85 + 4 + 1) * n,  # keys, perm, state in+out, offsets, level
+        "nodes": 49.0 * n + 80.0 * nodes,           # keys, offsets, leaf bodies; node records
+        "com": 4.0 * n + (80.0 + 32.0) * internal,  # climb + children read + parent write
+        "walk": (33.0 + 24.0 + 72.0) * n + 64.0 * nodes,  # own state, Stormer in/out, tree
+        "direct": 32.0 * n,
+    }
+    return table.get(kernel, 0.0)
+
+
+def cpu_baseline(n_cube, theta, e, dt, budget_s):
+    """The oracle's single-thread astro2 + verlet step on the same workload,
+    timed for whole steps until about budget_s seconds."""
+    from oracle import oracle as O
+
+    state = O.cube_and_stars(n_cube, seed=1, spin=1000.0)
+    v = O.Verlet()
+    t0 = time.perf_counter()
+    O.pipeline_steps_timed(state, theta, e, dt, 1, v)  # first (Taylor) step, untimed
+    first = time.perf_counter() - t0
+    steps = max(1, min(30, int(budget_s / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    O.pipeline_steps_timed(state, theta, e, dt, steps, v)
+    dt_s = time.perf_counter() - t0
+    n = len(state)
+    return {
+        "value": n * steps / dt_s,
+        "unit": "body-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} timed steps (after 1 untimed) of the same {n}-body workload, "
+                  f"oracle astro2 + verlet, 1 thread, {dt_s:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--bodies", type=int, default=1 << 20, help="cube bodies (2 stars added)")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import physim_amd as P
+
+    theta, e, dt = 1.5, 0.5, 1e-5
+    state = P.cube_and_stars(args.bodies, seed=1, spin=1000.0)
+    n = len(state)
+    eng = P.Engine(0)
+    sim = P.Sim(eng, state, kind="astro2", theta=theta, e=e, dt=dt, precision=args.precision)
+
+    if args.warmup:
+        sim.step(args.warmup)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    eng.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = sim.stats()
+
+    # eager pass with per-kernel HIP events on the engine's stream
+    eng.profile_reset()
+    eng.set_profiling(True)
+    sim.step(args.profile_steps)
+    eng.set_profiling(False)
+    prof = eng.profile()
+    # one more step with per-body interaction counting for the statistics
+    sim.set_counting(True)
+    sim.step(1)
+    sim.set_counting(False)
+    st2 = st = sim.stats()
+    passes = prof.get("sort_pass", (0, 0))[1] // max(args.profile_steps, 1)
+    per_step_ms = {k: v[0] / args.profile_steps for k, v in prof.items()}
+    dom = max(per_step_ms, key=per_step_ms.get)
+    avg_ms = prof[dom][0] / prof[dom][1]
+    abytes = algorithmic_bytes(dom, n, st2["nodes"], passes, st2["interactions"])
+    achieved = abytes / (avg_ms * 1e-3) / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        tr = json.load(open(tfile))
+        if tr.get("kernel") == dom and tr.get("bodies") == n:
+            traffic = tr.get("bytes_per_launch")
+
+    value = n * args.steps / elapsed
+    out = {
+        "metric": "body-steps/s at 1M bodies (BH θ=1.5); p99 solve latency, served vs plain GPU",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair forces + f64 tree/state" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "cube n=1048576 seed=1 spin=1000 + 2 stars (m=1e5 at (+-0.2,+-0.2,0.5)); "
+                        "astro2 theta=1.5 e=0.5; verlet dt=1e-5; state resident on device",
+            "bodies": n,
+            "precision": args.precision,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "avg_launch_ms": avg_ms,
+            "algorithmic_bytes_per_launch": abytes,
+        },
+        "extra": {
+            "interactions_per_step": st["interactions"],
+            "interactions_per_s": st["interactions"] * args.steps / elapsed,
+            "tree_nodes": st["nodes"],
+            "max_leaf_level": st["max_level"],
+            "kernel_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step_ms.items())},
+            "eager_ms_per_step": sum(per_step_ms.values()),
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.bodies, theta, e, dt, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,6 +1,7 @@
 """ctypes bindings of libphysim_gpu.so (include/physim_gpu.h)."""
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -231,9 +232,14 @@ class Engine:
         h = ctypes.c_void_p()
         check(lib().pg_engine_create(device, budget, ctypes.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()
 
     def close(self):
+        """Destroys the engine; servers and simulations still open on it are
+        closed first (they must not outlive it)."""
         if getattr(self, "h", None):
+            for child in sorted(self._children, key=lambda c: not isinstance(c, Server)):
+                child.close()
             lib().pg_engine_destroy(self.h)
             self.h = None
 
@@ -291,6 +297,7 @@ class Sim:
         h = ctypes.c_void_p()
         check(lib().pg_sim_create(engine.h, self.n, ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
+        engine._children.add(self)
         if state is not None:
             self.upload(state)
 
@@ -372,6 +379,7 @@ class Server:
         h = ctypes.c_void_p()
         check(lib().pg_server_create(engine.h, max_active, ctypes.byref(h)))
         self.h = h
+        engine._children.add(self)
 
     def close(self):
         if getattr(self, "h", None):

@@ -59,6 +59,7 @@ struct pg_engine {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
+    std::vector<pg_sim*> sims;  // live simulations; freed with the engine
     uint64_t sims_bytes = 0;
     uint64_t scratch_bytes = 0;
     std::recursive_mutex mu;
@@ -82,13 +83,19 @@ struct pg_engine {
     uint64_t* seg_off = nullptr;
     uint32_t* blk_seg = nullptr;
     uint32_t* blk_tile = nullptr;
+    uint32_t* ext_seg = nullptr;
+    uint32_t* ext_chunk = nullptr;
+    uint64_t* ext_partial = nullptr;
+    uint32_t* top_list = nullptr;
     StepCounters* sc = nullptr;
     uint32_t* err_global = nullptr;
+    uint32_t cap_chunks = 0;
+    uint64_t cap_top = 0;
 
     // host mirrors
     std::vector<SegDesc> h_desc[2];
     std::vector<uint64_t> h_seg_off;
-    std::vector<uint32_t> h_blk_seg, h_blk_tile;
+    std::vector<uint32_t> h_blk_seg, h_blk_tile, h_ext_seg, h_ext_chunk;
 
     // graph cache (two entries: the two parities of an odd step count)
     struct GraphEntry {
@@ -204,8 +211,9 @@ static void free_scratch(pg_engine* e) {
 extern "C" void pg_engine_destroy(pg_engine* e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
-    if (e->xform) pg_sim_destroy(e->xform);
     (void)hipStreamSynchronize(e->stream);
+    e->xform = nullptr;
+    while (!e->sims.empty()) pg_sim_destroy(e->sims.back());
     free_scratch(e);
     for (auto& ev : e->ev_pool) {
         (void)hipEventDestroy(ev.a);
@@ -306,6 +314,12 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_so = take((segs + 1) * 8);
     const uint64_t o_bs = take(blocks * 4), o_bt = take(blocks * 4);
     const uint64_t o_sc = take(sizeof(StepCounters)), o_eg = take(256);
+    // extent chunks never straddle segments: at most bodies/C + segs of them
+    const uint64_t chunks = bodies / kExtentChunk + segs + 1;
+    const uint64_t o_es = take(chunks * 4), o_ec = take(chunks * 4), o_ep = take(chunks * 8);
+    // a tile boundary is crossed by at most one node per level (levels 0..32)
+    const uint64_t top = (bodies / kComTile + 2) * 33;
+    const uint64_t o_tl = take(top * 4);
     const uint64_t total = off;
     if (e->sims_bytes + total > e->budget)
         return pg_set_error(PG_ERR_BUDGET, "scratch of %llu bytes exceeds the HBM budget",
@@ -338,6 +352,12 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->blk_tile = (uint32_t*)(b + o_bt);
     e->sc = (StepCounters*)(b + o_sc);
     e->err_global = (uint32_t*)(b + o_eg);
+    e->ext_seg = (uint32_t*)(b + o_es);
+    e->ext_chunk = (uint32_t*)(b + o_ec);
+    e->ext_partial = (uint64_t*)(b + o_ep);
+    e->top_list = (uint32_t*)(b + o_tl);
+    e->cap_chunks = (uint32_t)chunks;
+    e->cap_top = top;
     // Status words are versioned; zero them once so no stale tag can match.
     HIP_TRY(hipMemsetAsync(e->sort_status, 0, sort_tiles * 256 * 8, e->stream));
     HIP_TRY(hipMemsetAsync(e->scan_status, 0, scan_tiles * 8, e->stream));
@@ -399,6 +419,7 @@ struct BatchPlan {
     uint32_t nseg = 0;
     uint64_t total = 0;
     uint32_t nblocks = 0;
+    uint32_t nchunks = 0;
     bool integrate = true;
 };
 
@@ -417,7 +438,11 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         k_prep<<<1, kBlock, 0, st>>>(desc, bp.nseg, e->sc, INTEG ? 1 : 0);
     });
     launch(e, cap, "extent", [&] {
-        k_extent<<<grid_for(total, 2048), kBlock, 0, st>>>(desc, e->seg_off, bp.nseg, total);
+        k_extent_partial<<<bp.nchunks, kBlock, 0, st>>>(desc, e->ext_seg, e->ext_chunk,
+                                                        e->ext_partial);
+    });
+    launch(e, cap, "extent_final", [&] {
+        k_extent_final<<<bp.nseg, kBlock, 0, st>>>(desc, e->ext_partial);
     });
     launch(e, cap, "keys", [&] {
         k_keys<DIM><<<grid_for(total, 512), kBlock, 0, st>>>(desc, e->seg_off, bp.nseg, total,
@@ -446,9 +471,12 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
             desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->first_node, e->lo_level, e->hot,
             e->cold, e->cap_nodes);
     });
-    launch(e, cap, "com", [&] {
-        k_com<<<grid_for(total, 1u << 20), kBlock, 0, st>>>(total, e->first_node, e->hot,
-                                                           e->cold, e->cap_nodes);
+    launch(e, cap, "com_local", [&] {
+        k_com_local<<<(unsigned)((total + kComTile - 1) / kComTile), kBlock, 0, st>>>(
+            total, e->first_node, e->hot, e->cold, e->cap_nodes, e->top_list, e->sc);
+    });
+    launch(e, cap, "com_top", [&] {
+        k_com_top<<<1, 1024, 0, st>>>(e->hot, e->top_list, e->sc, bp.lmax);
     });
     launch(e, cap, "walk", [&] {
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
@@ -571,7 +599,22 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         }
         bp.nblocks = (uint32_t)e->h_blk_seg.size();
     }
+    e->h_ext_seg.clear();
+    e->h_ext_chunk.clear();
+    std::vector<uint32_t> chunk0(bp.nseg), nchunk(bp.nseg);
+    for (uint32_t s = 0; s < bp.nseg; ++s) {
+        const uint32_t nc = (uint32_t)((sims[s]->n + kExtentChunk - 1) / kExtentChunk);
+        chunk0[s] = (uint32_t)e->h_ext_seg.size();
+        nchunk[s] = nc;
+        for (uint32_t c = 0; c < nc; ++c) {
+            e->h_ext_seg.push_back(s);
+            e->h_ext_chunk.push_back(c);
+        }
+    }
+    bp.nchunks = (uint32_t)e->h_ext_seg.size();
     PG_TRY(ensure_scratch(e, bp.total, nodes, bp.nseg, bp.nblocks));
+    if (bp.nchunks > e->cap_chunks)
+        return pg_set_error(PG_ERR_INTERNAL, "extent chunk table overflow");
 
     // descriptors for both parities
     e->h_seg_off.assign(bp.nseg + 1, 0);
@@ -601,6 +644,8 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
             d.count = sm->counting ? 1u : 0u;
             d.counts = sm->counts;
             d.st = sm->st;
+            d.chunk0 = chunk0[s];
+            d.nchunks = nchunk[s];
         }
         off += sm->n;
     }
@@ -617,6 +662,10 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         HIP_TRY(hipMemcpyAsync(e->blk_tile, e->h_blk_tile.data(), bp.nblocks * 4,
                                hipMemcpyHostToDevice, e->stream));
     }
+    HIP_TRY(hipMemcpyAsync(e->ext_seg, e->h_ext_seg.data(), bp.nchunks * 4,
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->ext_chunk, e->h_ext_chunk.data(), bp.nchunks * 4,
+                           hipMemcpyHostToDevice, e->stream));
 
     const bool flips = bp.kind != PG_KIND_DIRECT || integrate;
     uint32_t done = 0;
@@ -732,6 +781,7 @@ extern "C" int pg_sim_create(pg_engine* e, uint64_t n, const pg_params* p, pg_si
     s->st = (SegState*)take(sizeof(SegState));
     HIP_TRY(hipMemset(s->st, 0, sizeof(SegState)));
     e->sims_bytes += bytes;
+    e->sims.push_back(s);
     *out = s;
     return PG_OK;
 }
@@ -755,6 +805,8 @@ extern "C" void pg_sim_destroy(pg_sim* s) {
         }
     }
     if (e->last_sim == s) e->last_sim = nullptr;
+    if (e->xform == s) e->xform = nullptr;
+    e->sims.erase(std::remove(e->sims.begin(), e->sims.end(), s), e->sims.end());
     (void)hipFree(s->block);
     e->sims_bytes -= s->bytes;
     delete s;

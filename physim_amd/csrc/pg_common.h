@@ -14,6 +14,8 @@ constexpr int kScanItems = 8;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr uint32_t kSpinLimit = 1u << 22;  // bound of every device-side wait
+constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
+constexpr int kComTile = 2048;     // sorted bodies per centre-of-mass tile
 
 // Device-side error bits (sticky per segment), mapped to pg_status on the host.
 enum : uint32_t {
@@ -42,9 +44,11 @@ struct SegDesc {
     uint64_t n;
     double theta, e, dt;
     uint32_t seg;
-    uint32_t count; // per-body interaction counting into counts[] (debug)
+    uint32_t count; // per-body interaction counts into counts[] and step statistics
     int64_t* counts;
     struct SegState* st;
+    uint32_t chunk0;  // first extent chunk of this segment
+    uint32_t nchunks;
 };
 
 // Per-segment device scalars.
@@ -78,6 +82,7 @@ struct StepCounters {
     uint32_t gen;           // bumped per step: versions look-back status words
     uint32_t sort_ticket[kMaxPasses];
     uint32_t scan_ticket;
+    uint32_t top_count;     // tree nodes spanning centre-of-mass tiles
     uint32_t hist[kMaxPasses * 256];
 };
 

@@ -123,6 +123,7 @@ __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCoun
         sc->gen += 1;
         for (int p = 0; p < kMaxPasses; ++p) sc->sort_ticket[p] = 0;
         sc->scan_ticket = 0;
+        sc->top_count = 0;
     }
     for (uint32_t i = t; i < kMaxPasses * 256; i += stride) sc->hist[i] = 0;
     for (uint32_t s = t; s < nseg; s += stride) {
@@ -135,40 +136,59 @@ __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCoun
     }
 }
 
-// transformers.rs:126-131: max |coord| over the segment (exact), NaN/Inf flagged.
-__global__ void k_extent(const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off,
-                         uint32_t nseg, uint64_t total) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
-        const uint64_t g = base + threadIdx.x;
-        const bool valid = g < total;
-        uint32_t s = 0;
-        uint64_t bits = 0;
-        if (valid) {
-            s = seg_of(seg_off, nseg, g);
-            const SegDesc& d = desc[s];
-            const uint64_t j = g - d.off;
-            const double x = d.sx[j], y = d.sy[j], z = d.sz[j];
-            if (!isfinite(x) || !isfinite(y) || !isfinite(z)) atomicOr(&d.st->err, kErrNonFinite);
-            bits = f64_bits(fmax(fmax(fabs(x), fabs(y)), fabs(z)));
-        }
-        const uint64_t vmask = __ballot(valid);
-        const int lead = __ffsll((long long)vmask) - 1;
-        const uint32_t s0 = __shfl(s, lead < 0 ? 0 : lead);
-        const bool uniform = __ballot(valid && s != s0) == 0;
-        if (uniform) {
-            uint64_t m = bits;
+// Block-wide max of one u64 per thread (256 threads).
+__device__ __forceinline__ uint64_t block_max_u64(uint64_t v, uint64_t* s_wave) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                uint64_t t2 = __shfl_xor(m, o);
-                m = t2 > m ? t2 : m;
-            }
-            if ((int)(threadIdx.x & 63) == lead) atomicMax((unsigned long long*)&desc[s0].st->extent_bits,
-                                                      (unsigned long long)m);
-        } else if (valid) {
-            atomicMax((unsigned long long*)&desc[s].st->extent_bits, (unsigned long long)bits);
-        }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t t = __shfl_xor(v, o);
+        v = t > v ? t : v;
     }
+    if (lane == 0) s_wave[wave] = v;
+    __syncthreads();
+    uint64_t m = s_wave[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) m = s_wave[w] > m ? s_wave[w] : m;
+    __syncthreads();
+    return m;
+}
+
+// transformers.rs:126-131: max |coord| over a segment (exact). Each block
+// reduces one chunk of kExtentChunk bodies of one segment into partial[];
+// NaN/Inf coordinates are flagged.
+__global__ __launch_bounds__(kBlock) void k_extent_partial(const SegDesc* __restrict__ desc,
+                                                           const uint32_t* __restrict__ chunk_seg,
+                                                           const uint32_t* __restrict__ chunk_idx,
+                                                           uint64_t* __restrict__ partial) {
+    __shared__ uint64_t s_wave[kBlock / 64];
+    const SegDesc& d = desc[chunk_seg[blockIdx.x]];
+    const uint64_t j0 = (uint64_t)chunk_idx[blockIdx.x] * kExtentChunk;
+    const uint64_t j1 = min(j0 + kExtentChunk, d.n);
+    uint64_t m = 0;
+    bool bad = false;
+    for (uint64_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
+        const double x = d.sx[j], y = d.sy[j], z = d.sz[j];
+        bad |= !isfinite(x) || !isfinite(y) || !isfinite(z);
+        const uint64_t b = f64_bits(fmax(fmax(fabs(x), fabs(y)), fabs(z)));
+        m = b > m ? b : m;
+    }
+    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(&d.st->err, kErrNonFinite);
+    m = block_max_u64(m, s_wave);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+// One block per segment: the segment's extent from its chunk partials.
+__global__ __launch_bounds__(kBlock) void k_extent_final(const SegDesc* __restrict__ desc,
+                                                         const uint64_t* __restrict__ partial) {
+    __shared__ uint64_t s_wave[kBlock / 64];
+    const SegDesc& d = desc[blockIdx.x];
+    uint64_t m = 0;
+    for (uint32_t c = threadIdx.x; c < d.nchunks; c += kBlock) {
+        const uint64_t b = partial[d.chunk0 + c];
+        m = b > m ? b : m;
+    }
+    m = block_max_u64(m, s_wave);
+    if (threadIdx.x == 0) d.st->extent_bits = m;
 }
 
 // Morton key by exact FP64 descent (octree.rs:160-215 / quadtree.rs:161-194):
@@ -519,55 +539,86 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
                 ext = h2;
             }
         }
-        atomicMax(&d.st->max_level, (uint32_t)hi);
+        if (d.count) atomicMax(&d.st->max_level, (uint32_t)hi);
     }
 }
 
-// Bottom-up mass and centre of mass: each leaf climbs; at every parent the
-// thread that completes the parent's body count sums the children (octant
-// order, so the result does not depend on which thread arrives last).
-__global__ __launch_bounds__(kBlock) void k_com(uint64_t total,
-                                                const uint32_t* __restrict__ first_node,
-                                                NodeHot* hot, NodeCold* cold, uint64_t cap) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        uint32_t nd = first_node[i + 1] - 1u;
-        if (nd >= cap) continue;
-        for (;;) {
-            const int32_t p = cold[nd].parent;
-            if (p < 0 || (uint32_t)p >= nd) break;  // parents precede children in preorder
-            const uint32_t nb = cold[nd].last - cold[nd].first + 1u;
-            const uint32_t need = cold[p].last - cold[p].first + 1u;
-            __threadfence();
-            const uint32_t old = atomicAdd(&cold[p].arrivals, nb);
-            if (old + nb != need) break;
-            __threadfence();
-            double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
-            const uint32_t send = __hip_atomic_load(&hot[p].skip, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-            for (uint32_t c = (uint32_t)p + 1u; c < send;) {
-                const double m = __hip_atomic_load(&hot[c].mass, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                const double x = __hip_atomic_load(&hot[c].mx, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                const double y = __hip_atomic_load(&hot[c].my, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                const double z = __hip_atomic_load(&hot[c].mz, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                M += m;
-                X += m * x;
-                Y += m * y;
-                Z += m * z;
-                const uint32_t nxt = hot[c].skip;
-                c = nxt > c ? nxt : c + 1u;
-            }
-            const double inv = 1.0 / M;
-            __hip_atomic_store(&hot[p].mx, X * inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&hot[p].my, Y * inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&hot[p].mz, Z * inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&hot[p].mass, M, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            nd = (uint32_t)p;
+// Mass and centre of mass of an internal node from its children, in octant
+// order (children are p+1 and then each child's skip until p's skip).
+__device__ __forceinline__ void com_from_children(NodeHot* hot, uint32_t p) {
+    double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+    const uint32_t send = hot[p].skip;
+    for (uint32_t c = p + 1u; c < send;) {
+        const double m = hot[c].mass;
+        M += m;
+        X += m * hot[c].mx;
+        Y += m * hot[c].my;
+        Z += m * hot[c].mz;
+        const uint32_t nxt = hot[c].skip;
+        c = nxt > c ? nxt : c + 1u;
+    }
+    const double inv = 1.0 / M;
+    hot[p].mx = X * inv;
+    hot[p].my = Y * inv;
+    hot[p].mz = Z * inv;
+    hot[p].mass = M;
+}
+
+// Bottom-up centre of mass, tile pass. The bodies of tile t own the node range
+// [first_node[b0], first_node[b1]) (preorder); an internal node whose body
+// range ends inside the tile has its whole subtree there, so the block reduces
+// those level by level, deepest first, with barriers between levels. Nodes
+// spanning into a later tile are listed for the single-block top pass.
+__global__ __launch_bounds__(kBlock) void k_com_local(uint64_t total,
+                                                      const uint32_t* __restrict__ first_node,
+                                                      NodeHot* hot,
+                                                      const NodeCold* __restrict__ cold,
+                                                      uint64_t cap, uint32_t* __restrict__ top_list,
+                                                      StepCounters* sc) {
+    __shared__ uint64_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_scan[kBlock / 64];
+    __shared__ uint32_t s_base;
+    const uint64_t b0 = (uint64_t)blockIdx.x * kComTile;
+    const uint64_t b1 = min(b0 + (uint64_t)kComTile, total);
+    const uint32_t n0 = first_node[b0];
+    const uint32_t n1 = (uint32_t)min((uint64_t)first_node[b1], cap);
+    uint64_t ml = 0;
+    uint32_t nspan = 0;
+    for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock) {
+        const uint32_t L = hot[nd].level;
+        ml = L > ml ? L : ml;
+        nspan += hot[nd].skip != nd + 1u && cold[nd].last >= b1;
+    }
+    const int maxl = (int)block_max_u64(ml, s_wave);
+    uint32_t nsp_total;
+    uint32_t slot = block_exclusive_scan(nspan, s_scan, &nsp_total);
+    if (threadIdx.x == 0) s_base = nsp_total ? atomicAdd(&sc->top_count, nsp_total) : 0u;
+    __syncthreads();
+    slot += s_base;
+    for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock)
+        if (hot[nd].skip != nd + 1u && cold[nd].last >= b1) top_list[slot++] = nd;
+    for (int L = maxl - 1; L >= 0; --L) {
+        for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock) {
+            if ((int)hot[nd].level == L && hot[nd].skip != nd + 1u && cold[nd].last < b1)
+                com_from_children(hot, nd);
         }
+        __syncthreads();
+    }
+}
+
+// Bottom-up centre of mass, top pass: one block reduces the listed
+// tile-spanning nodes level by level (their children are either local nodes,
+// finished by the tile pass, or listed nodes of a deeper level).
+__global__ __launch_bounds__(1024) void k_com_top(NodeHot* hot,
+                                                  const uint32_t* __restrict__ top_list,
+                                                  const StepCounters* __restrict__ sc, int lmax) {
+    const uint32_t cnt = sc->top_count;
+    for (int L = lmax; L >= 0; --L) {
+        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+            const uint32_t nd = top_list[k];
+            if ((int)hot[nd].level == L) com_from_children(hot, nd);
+        }
+        __syncthreads();
     }
 }
 
@@ -706,8 +757,10 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
             az = (double)ka.sz - (double)ka.cz;
         }
     }
-    if (d.count) d.counts[d.did[j]] = count;
-    atomicAdd((unsigned long long*)&d.st->interactions, (unsigned long long)count);
+    if (d.count) {
+        d.counts[d.did[j]] = count;
+        atomicAdd((unsigned long long*)&d.st->interactions, (unsigned long long)count);
+    }
     if (INTEGRATE) {
         const double dt = d.dt;
         const double dt2 = dt * dt;
