@@ -86,7 +86,8 @@ struct pg_engine {
     uint32_t* ext_seg = nullptr;
     uint32_t* ext_chunk = nullptr;
     uint64_t* ext_partial = nullptr;
-    uint32_t* top_list = nullptr;
+    uint64_t* top_slot = nullptr;  // generation-tagged tile-spanning node per (tile, level)
+    uint64_t* top_kids = nullptr;  // its children by octant, generation-tagged
     StepCounters* sc = nullptr;
     uint32_t* err_global = nullptr;
     uint32_t cap_chunks = 0;
@@ -135,6 +136,7 @@ struct pg_sim {
     bool counting = false;
     int cur = 0;
     bool uploaded = false;
+    double dt_last = 0.0;  // dt of the last integrating step
 };
 
 // ---------------------------------------------------------------- sizing
@@ -318,8 +320,8 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t chunks = bodies / kExtentChunk + segs + 1;
     const uint64_t o_es = take(chunks * 4), o_ec = take(chunks * 4), o_ep = take(chunks * 8);
     // a tile boundary is crossed by at most one node per level (levels 0..32)
-    const uint64_t top = (bodies / kComTile + 2) * 33;
-    const uint64_t o_tl = take(top * 4);
+    const uint64_t top = (bodies / kComTile + 2) * kTopSlots;
+    const uint64_t o_tl = take(top * 8), o_tk = take(top * 8 * 8);
     const uint64_t total = off;
     if (e->sims_bytes + total > e->budget)
         return pg_set_error(PG_ERR_BUDGET, "scratch of %llu bytes exceeds the HBM budget",
@@ -355,7 +357,8 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->ext_seg = (uint32_t*)(b + o_es);
     e->ext_chunk = (uint32_t*)(b + o_ec);
     e->ext_partial = (uint64_t*)(b + o_ep);
-    e->top_list = (uint32_t*)(b + o_tl);
+    e->top_slot = (uint64_t*)(b + o_tl);
+    e->top_kids = (uint64_t*)(b + o_tk);
     e->cap_chunks = (uint32_t)chunks;
     e->cap_top = top;
     // Status words are versioned; zero them once so no stale tag can match.
@@ -363,6 +366,8 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     HIP_TRY(hipMemsetAsync(e->scan_status, 0, scan_tiles * 8, e->stream));
     HIP_TRY(hipMemsetAsync(e->sc, 0, sizeof(StepCounters), e->stream));
     HIP_TRY(hipMemsetAsync(e->err_global, 0, 256, e->stream));
+    HIP_TRY(hipMemsetAsync(e->top_slot, 0, top * 8, e->stream));
+    HIP_TRY(hipMemsetAsync(e->top_kids, 0, top * 64, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->cap_bodies = bodies;
     e->cap_nodes = nodes;
@@ -467,16 +472,17 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
             e->sc, e->scan_status, e->err_global);
     });
     launch(e, cap, "nodes", [&] {
-        k_nodes<DIM><<<grid_for(total, 1u << 20), kBlock, 0, st>>>(
+        k_nodes<DIM><<<(unsigned)((total + kNodeTile - 1) / kNodeTile), kNodeThreads, 0, st>>>(
             desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->first_node, e->lo_level, e->hot,
-            e->cold, e->cap_nodes);
+            e->cold, e->cap_nodes, e->top_slot, e->top_kids, e->sc);
     });
+    const unsigned com_tiles = (unsigned)((total + kComTile - 1) / kComTile);
     launch(e, cap, "com_local", [&] {
-        k_com_local<<<(unsigned)((total + kComTile - 1) / kComTile), kBlock, 0, st>>>(
-            total, e->first_node, e->hot, e->cold, e->cap_nodes, e->top_list, e->sc);
+        k_com_local<<<com_tiles, kBlock, 0, st>>>(total, e->first_node, e->hot, e->cold,
+                                                  e->cap_nodes);
     });
     launch(e, cap, "com_top", [&] {
-        k_com_top<<<1, 1024, 0, st>>>(e->hot, e->top_list, e->sc, bp.lmax);
+        k_com_top<<<1, 1024, 0, st>>>(e->hot, e->top_slot, e->top_kids, e->sc, com_tiles);
     });
     launch(e, cap, "walk", [&] {
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
@@ -707,6 +713,8 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
 
     if (flips && (steps & 1))
         for (pg_sim* s : sims) s->cur ^= 1;
+    if (integrate)
+        for (pg_sim* s : sims) s->dt_last = s->p.dt;
 
     e->last_sim = bp.nseg == 1 && bp.kind != PG_KIND_DIRECT ? sims[0] : nullptr;
     e->last_total = bp.total;
@@ -848,6 +856,19 @@ extern "C" int pg_sim_upload(pg_sim* s, const Entity* state, uint64_t n) {
     return PG_OK;
 }
 
+// Where the current state's velocities come from: the stored arrays up to and
+// right after the Taylor first step, the backward difference afterwards.
+static int velocity_source(pg_sim* s, VelocitySource* vs) {
+    SegState st;
+    HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
+    const int c = s->cur;
+    vs->vx = s->vx[c]; vs->vy = s->vy[c]; vs->vz = s->vz[c];
+    vs->px = s->px[c]; vs->py = s->py[c]; vs->pz = s->pz[c];
+    vs->derive = st.steps_done >= 2 ? 1 : 0;
+    vs->dt = s->dt_last;
+    return PG_OK;
+}
+
 extern "C" int pg_sim_download(pg_sim* s, Entity* out, uint64_t n) {
     if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
     if (n != s->n) return pg_set_error(PG_ERR_ARG, "n mismatch");
@@ -858,8 +879,10 @@ extern "C" int pg_sim_download(pg_sim* s, Entity* out, uint64_t n) {
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     HIP_TRY(hipSetDevice(e->device));
     const int c = s->cur;
+    VelocitySource vs;
+    PG_TRY(velocity_source(s, &vs));
     k_pack<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(s->aos, n, s->x[c], s->y[c], s->z[c],
-                                                           s->vx[c], s->vy[c], s->vz[c], s->id[c]);
+                                                           vs, s->id[c]);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, s->aos, n * sizeof(Entity), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -930,8 +953,14 @@ extern "C" int pg_sim_energy(pg_sim* s, double out[6]) {
     double* part = nullptr;
     HIP_TRY(hipMalloc(&part, (size_t)nb * 5 * 8));
     const int c = s->cur;
-    k_energy<<<nb, kBlock, 0, e->stream>>>(s->x[c], s->y[c], s->z[c], s->vx[c], s->vy[c],
-                                          s->vz[c], s->m[c], n, std::fabs(s->p.e), part);
+    VelocitySource vs;
+    int vrc = velocity_source(s, &vs);
+    if (vrc != PG_OK) {
+        (void)hipFree(part);
+        return vrc;
+    }
+    k_energy<<<nb, kBlock, 0, e->stream>>>(s->x[c], s->y[c], s->z[c], vs, s->m[c], n,
+                                          std::fabs(s->p.e), part);
     hipError_t le = hipGetLastError();
     std::vector<double> h((size_t)nb * 5);
     hipError_t ce = hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, e->stream);

@@ -16,6 +16,10 @@ constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr uint32_t kSpinLimit = 1u << 22;  // bound of every device-side wait
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
 constexpr int kComTile = 2048;     // sorted bodies per centre-of-mass tile
+constexpr int kNodeTile = 2048;    // sorted bodies per node-building block
+constexpr int kNodeThreads = 1024; // threads per node-building block
+constexpr int kComLds = 6144;      // node counters per tile kept in LDS
+constexpr int kTopSlots = 33;      // tree levels 0..32: one tile-spanning node each
 
 // Device-side error bits (sticky per segment), mapped to pg_status on the host.
 enum : uint32_t {
@@ -59,7 +63,7 @@ struct SegState {
     uint32_t history;       // 1 when the Stormer history is valid for this step
     uint32_t err;
     uint32_t max_level;
-    uint32_t pad;
+    uint32_t vperm;         // velocities must travel with the state (not yet derivable)
 };
 
 // Tree node, split into the part the walk reads (64 bytes, one line) and the
@@ -69,7 +73,9 @@ struct NodeHot {
     double mx, my, mz;      // centre of mass (leaf: the body's exact position)
     double mass;
     uint32_t skip;          // index of the next node after this subtree
-    uint32_t level;
+    uint16_t level;
+    uint8_t octant;         // child index within the parent (0 for a root)
+    uint8_t pad;
 };
 struct NodeCold {
     uint32_t first, last;   // sorted-body range

@@ -132,6 +132,10 @@ __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCoun
         st->interactions = 0;
         st->max_level = 0;
         st->history = st->steps_done > 0 ? 1u : 0u;
+        // v is (x - x_prev) / dt after any step but the Taylor first one
+        // (verlet.rs:68-70), so it only has to be carried before and right
+        // after that step.
+        st->vperm = st->steps_done <= 1 ? 1u : 0u;
         if (integrate) st->steps_done += 1;
     }
 }
@@ -342,8 +346,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
 // Per sorted body: its common-prefix lengths with its neighbours give the
 // tree levels it owns (lo..hi, hi = its leaf); the exclusive scan of the
 // per-body node counts lays the tree out in preorder. Also permutes the
-// state into the new Morton order and flags equal keys (depth) and the
-// reference's 1e-9 merge rule (octree.rs:69-80).
+// state into the new Morton order (velocities only while they are not
+// derivable from the two positions) and flags equal keys (depth). Bodies are
+// read and written striped across the block (coalesced); the tile's counts
+// are scanned through LDS in index order.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys,
@@ -353,19 +359,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_excl;
+    __shared__ uint32_t s_cnt[kScanTile];
     const int tid = threadIdx.x;
     if (tid == 0) s_tile = atomicAdd(&sc->scan_ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t tag = (sc->gen << 4) | 15u;
-    const uint64_t base = (uint64_t)tile * kScanTile + (uint64_t)tid * kScanItems;
+    const uint64_t tbase = (uint64_t)tile * kScanTile;
 
-    uint32_t cnt[kScanItems];
-    uint32_t sum = 0;
-#pragma unroll
+#pragma unroll 2
     for (int k = 0; k < kScanItems; ++k) {
-        const uint64_t i = base + k;
-        cnt[k] = 0;
+        const uint32_t li = (uint32_t)k * kBlock + tid;
+        const uint64_t i = tbase + li;
+        uint32_t c = 0;
         if (i < total) {
             const uint64_t key = skeys[i];
             const uint32_t s = seg_of_key(key, segbits);
@@ -373,38 +379,36 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
             if (s >= nseg || src >= desc[s].n || i < desc[s].off || i >= desc[s].off + desc[s].n) {
                 atomicOr(err, kErrCapacity);
                 lo_level[i] = 0;
-                cnt[k] = 1;
-                sum += 1;
-                continue;
-            }
-            const SegDesc& d = desc[s];
-            const uint64_t end = d.off + d.n;
-            int lcpL = i > d.off ? key_lcp<DIM>(skeys[i - 1], key, lmax) : -1;
-            int lcpR = i + 1 < end ? key_lcp<DIM>(key, skeys[i + 1], lmax) : -1;
-            if (lcpR >= lmax) {
-                atomicOr(&d.st->err, kErrDepth);
-                lcpR = lmax - 1;
-            }
-            if (lcpL >= lmax) lcpL = lmax - 1;
-            cnt[k] = 1u + (uint32_t)max(0, lcpR - lcpL);
-            lo_level[i] = (uint8_t)(lcpL + 1);
-            const uint64_t j = i - d.off;
-            const double x = d.sx[src], y = d.sy[src], z = d.sz[src];
-            d.dx[j] = x; d.dy[j] = y; d.dz[j] = z;
-            d.dpx[j] = d.spx[src]; d.dpy[j] = d.spy[src]; d.dpz[j] = d.spz[src];
-            d.dvx[j] = d.svx[src]; d.dvy[j] = d.svy[src]; d.dvz[j] = d.svz[src];
-            d.dm[j] = d.sm[src];
-            d.dfix[j] = d.sfix[src];
-            d.did[j] = d.sid[src];
-            if (i + 1 < end) {
-                const uint64_t src2 = min((uint64_t)svals[i + 1] - d.off, d.n - 1);
-                if (fabs(x - d.sx[src2]) < 1e-9 && fabs(y - d.sy[src2]) < 1e-9 &&
-                    fabs(z - d.sz[src2]) < 1e-9)
-                    atomicOr(&d.st->err, kErrCoincident);
+                c = 1;
+            } else {
+                const SegDesc& d = desc[s];
+                const uint64_t end = d.off + d.n;
+                int lcpL = i > d.off ? key_lcp<DIM>(skeys[i - 1], key, lmax) : -1;
+                int lcpR = i + 1 < end ? key_lcp<DIM>(key, skeys[i + 1], lmax) : -1;
+                if (lcpR >= lmax) {
+                    atomicOr(&d.st->err, kErrDepth);
+                    lcpR = lmax - 1;
+                }
+                if (lcpL >= lmax) lcpL = lmax - 1;
+                c = 1u + (uint32_t)max(0, lcpR - lcpL);
+                lo_level[i] = (uint8_t)(lcpL + 1);
+                const uint64_t j = i - d.off;
+                d.dx[j] = d.sx[src]; d.dy[j] = d.sy[src]; d.dz[j] = d.sz[src];
+                d.dpx[j] = d.spx[src]; d.dpy[j] = d.spy[src]; d.dpz[j] = d.spz[src];
+                if (d.st->vperm) {
+                    d.dvx[j] = d.svx[src]; d.dvy[j] = d.svy[src]; d.dvz[j] = d.svz[src];
+                }
+                d.dm[j] = d.sm[src];
+                d.dfix[j] = d.sfix[src];
+                d.did[j] = d.sid[src];
             }
         }
-        sum += cnt[k];
+        s_cnt[li] = c;
     }
+    __syncthreads();
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) sum += s_cnt[tid * kScanItems + k];
     uint32_t tile_total;
     const uint32_t texcl = block_exclusive_scan(sum, s_scan, &tile_total);
     if (tid == 0) {
@@ -423,17 +427,43 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     uint32_t run = s_excl + texcl;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const uint64_t i = base + k;
-        if (i < total) first_node[i] = run;
-        run += cnt[k];
-        if (i + 1 == total) first_node[total] = run;
+        const uint32_t li = tid * kScanItems + k;
+        const uint32_t c = s_cnt[li];
+        s_cnt[li] = run;
+        run += c;
+        if (tbase + li + 1 == total) first_node[total] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t li = (uint32_t)k * kBlock + tid;
+        if (tbase + li < total) first_node[tbase + li] = s_cnt[li];
     }
 }
 
+// Sorted-key reads through an LDS window: the block's tile [b0, b1) is
+// staged in LDS, anything outside comes from global memory.
+struct KeyWindow {
+    const uint64_t* __restrict__ g;
+    const uint64_t* s;
+    uint64_t b0, b1;
+    __device__ __forceinline__ uint64_t operator[](uint64_t j) const {
+        return (j >= b0 && j < b1) ? s[j - b0] : g[j];
+    }
+};
+struct U32Window {
+    const uint32_t* __restrict__ g;
+    const uint32_t* s;
+    uint64_t b0, b1;
+    __device__ __forceinline__ uint32_t operator[](uint64_t j) const {
+        return (j >= b0 && j < b1) ? s[j - b0] : g[j];
+    }
+};
+
 // Last sorted index j >= i whose key shares i's prefix at `level`.
-template <int DIM>
-__device__ __forceinline__ uint64_t range_end(const uint64_t* __restrict__ skeys, uint64_t i,
-                                              uint64_t end, int lmax, int level) {
+template <int DIM, class Keys>
+__device__ __forceinline__ uint64_t range_end(const Keys& skeys, uint64_t i, uint64_t end, int lmax,
+                                              int level) {
     const uint64_t P = key_prefix<DIM>(skeys[i], lmax, level);
     uint64_t last = i, step = 1;
     for (;;) {
@@ -451,9 +481,9 @@ __device__ __forceinline__ uint64_t range_end(const uint64_t* __restrict__ skeys
 }
 
 // First sorted index j <= i (j >= begin) whose key shares i's prefix at `level`.
-template <int DIM>
-__device__ __forceinline__ uint64_t range_begin(const uint64_t* __restrict__ skeys, uint64_t i,
-                                                uint64_t begin, int lmax, int level) {
+template <int DIM, class Keys>
+__device__ __forceinline__ uint64_t range_begin(const Keys& skeys, uint64_t i, uint64_t begin,
+                                                int lmax, int level) {
     const uint64_t P = key_prefix<DIM>(skeys[i], lmax, level);
     uint64_t first = i, step = 1;
     for (;;) {
@@ -474,64 +504,120 @@ __device__ __forceinline__ uint64_t range_begin(const uint64_t* __restrict__ ske
 }
 
 // Writes every node a sorted body owns: geometric centre by the reference's
-// descent, half-width level, sorted range, skip (= first node of the body
-// after the range), parent; a leaf takes its body's exact position and mass.
+// descent, half-width level, octant, sorted range, skip (= first node of the
+// body after the range), parent; a leaf takes its body's exact position and
+// mass. One block per tile of kNodeTile sorted bodies, whose keys, node
+// offsets and levels are staged in LDS for the range searches. For the
+// centre-of-mass passes, an internal node whose range leaves its owner's
+// centre-of-mass tile is recorded (generation-tagged) at slot
+// tile * kTopSlots + level, and a node whose parent is such a node
+// registers itself in the parent's kid slot (octant order).
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_nodes(
-    const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys,
-    uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ first_node,
+__global__ __launch_bounds__(kNodeThreads) void k_nodes(
+    const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys_g,
+    uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ first_g,
     const uint8_t* __restrict__ lo_level, NodeHot* __restrict__ hot, NodeCold* __restrict__ cold,
-    uint64_t cap) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const uint64_t key = skeys[i];
+    uint64_t cap, uint64_t* __restrict__ top_slot, uint64_t* __restrict__ top_kids,
+    const StepCounters* __restrict__ sc) {
+    __shared__ uint64_t s_keys[kNodeTile];
+    __shared__ uint32_t s_first[kNodeTile + 1];
+    __shared__ uint8_t s_lo[kNodeTile];
+    const uint64_t gen = (uint64_t)sc->gen << 32;
+    const uint64_t b0 = (uint64_t)blockIdx.x * kNodeTile;
+    const uint64_t b1 = min(b0 + (uint64_t)kNodeTile, total);
+    for (uint64_t j = b0 + threadIdx.x; j < b1; j += kNodeThreads) {
+        s_keys[j - b0] = skeys_g[j];
+        s_lo[j - b0] = lo_level[j];
+    }
+    for (uint64_t j = b0 + threadIdx.x; j <= b1; j += kNodeThreads) s_first[j - b0] = first_g[j];
+    __syncthreads();
+    const KeyWindow skeys{skeys_g, s_keys, b0, b1};
+    const U32Window first_node{first_g, s_first, b0, b1 + 1};
+    const uint64_t dmask = (1u << DIM) - 1u;
+    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kNodeThreads) {
+        const uint64_t key = s_keys[i - b0];
         const uint32_t s = seg_of_key(key, segbits);
         if (s >= nseg) continue;
         const SegDesc& d = desc[s];
         const uint64_t end = d.off + d.n;
-        const int lo = lo_level[i];
-        const uint32_t nbase = first_node[i];
-        const int hi = lo + (int)(first_node[i + 1] - nbase) - 1;
+        const int lo = s_lo[i - b0];
+        const uint32_t nbase = s_first[i - b0];
+        const int cnt = (int)(s_first[i - b0 + 1] - nbase);
+        const int hi = lo + cnt - 1;
+        // centre of the first owned node: the reference's descent, lo levels down
         double ext = bits_f64(d.st->extent_bits);
         double cx = 0.0, cy = 0.0, cz = 0.0;
-        for (int L = 0; L <= hi; ++L) {
-            if (L >= lo) {
-                const uint64_t nd = (uint64_t)nbase + (uint64_t)(L - lo);
-                if (nd >= cap) {
-                    atomicOr(&d.st->err, kErrCapacity);
-                    break;
-                }
-                const bool leaf = L == hi;
-                const uint64_t last = leaf ? i : range_end<DIM>(skeys, i, end, lmax, L);
-                int32_t parent;
-                if (L > lo) parent = (int32_t)(nd - 1);
-                else if (L == 0) parent = -1;
-                else {
-                    const uint64_t a = range_begin<DIM>(skeys, i, d.off, lmax, L - 1);
-                    parent = (int32_t)(first_node[a] + (uint32_t)(L - 1 - lo_level[a]));
-                }
-                NodeHot h;
-                h.cx = cx; h.cy = cy; h.cz = cz;
-                if (leaf) {
-                    const uint64_t j = i - d.off;
-                    h.mx = d.dx[j]; h.my = d.dy[j]; h.mz = d.dz[j];
-                    h.mass = d.dm[j];
-                } else {
-                    h.mx = h.my = h.mz = 0.0;
-                    h.mass = 0.0;
-                }
-                h.skip = first_node[last + 1];
-                h.level = (uint32_t)L;
-                hot[nd] = h;
-                NodeCold c;
-                c.first = (uint32_t)i;
-                c.last = (uint32_t)last;
-                c.parent = parent;
-                c.arrivals = 0;
-                cold[nd] = c;
+        for (int L = 0; L < lo; ++L) {
+            const uint32_t o = (uint32_t)((key >> (DIM * (lmax - 1 - L))) & dmask);
+            const double h2 = ext / 2.0;
+            cx = (o & 1u) ? cx + h2 : cx - h2;
+            cy = (o & 2u) ? cy + h2 : cy - h2;
+            if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+            ext = h2;
+        }
+        // the owned nodes, all lanes of a wave in step
+        for (int k = 0; k < cnt; ++k) {
+            const int L = lo + k;
+            const uint64_t nd = (uint64_t)nbase + (uint64_t)k;
+            if (nd >= cap) {
+                atomicOr(&d.st->err, kErrCapacity);
+                break;
             }
-            if (L < hi) {
-                const uint32_t o = (uint32_t)(key >> (DIM * (lmax - 1 - L))) & ((1u << DIM) - 1u);
+            const bool leaf = L == hi;
+            const uint64_t last = leaf ? i : range_end<DIM>(skeys, i, end, lmax, L);
+            int32_t parent;
+            uint64_t pfirst = i;  // first body of the parent
+            if (k > 0) {
+                parent = (int32_t)(nd - 1);
+            } else if (L == 0) {
+                parent = -1;
+            } else {
+                const uint64_t a = range_begin<DIM>(skeys, i, d.off, lmax, L - 1);
+                const uint8_t loa = (a >= b0 && a < b1) ? s_lo[a - b0] : lo_level[a];
+                parent = (int32_t)(first_node[a] + (uint32_t)(L - 1 - loa));
+                pfirst = a;
+            }
+            NodeHot h;
+            h.cx = cx; h.cy = cy; h.cz = cz;
+            if (leaf) {
+                const uint64_t j = i - d.off;
+                h.mx = d.dx[j]; h.my = d.dy[j]; h.mz = d.dz[j];
+                h.mass = d.dm[j];
+                // the reference merges bodies within 1e-9 on every axis
+                // (octree.rs:69-80); the device reports them instead
+                if (j + 1 < d.n && fabs(h.mx - d.dx[j + 1]) < 1e-9 &&
+                    fabs(h.my - d.dy[j + 1]) < 1e-9 && fabs(h.mz - d.dz[j + 1]) < 1e-9)
+                    atomicOr(&d.st->err, kErrCoincident);
+            } else {
+                h.mx = h.my = h.mz = 0.0;
+                h.mass = 0.0;
+            }
+            h.skip = first_node[last + 1];
+            h.level = (uint16_t)L;
+            h.octant = L > 0 ? (uint8_t)((key >> (DIM * (lmax - L))) & dmask) : 0;
+            h.pad = 0;
+            hot[nd] = h;
+            NodeCold c;
+            c.first = (uint32_t)i;
+            c.last = (uint32_t)last;
+            c.parent = parent;
+            c.arrivals = 0;
+            cold[nd] = c;
+            const uint64_t tile = i / (uint64_t)kComTile;
+            if (!leaf && last >= (tile + 1) * (uint64_t)kComTile && L < kTopSlots)
+                top_slot[tile * kTopSlots + L] = gen | (uint64_t)nd;
+            if (parent >= 0) {
+                // the parent spans its tile iff the first body past that tile
+                // still shares the parent's prefix
+                const uint64_t pt = pfirst / (uint64_t)kComTile;
+                const uint64_t te = (pt + 1) * (uint64_t)kComTile;
+                if (te < end &&
+                    key_prefix<DIM>(skeys[te], lmax, L - 1) == key_prefix<DIM>(key, lmax, L - 1))
+                    top_kids[(pt * kTopSlots + (uint64_t)(L - 1)) * 8 + h.octant] =
+                        gen | (uint64_t)nd;
+            }
+            if (k + 1 < cnt) {
+                const uint32_t o = (uint32_t)((key >> (DIM * (lmax - 1 - L))) & dmask);
                 const double h2 = ext / 2.0;
                 cx = (o & 1u) ? cx + h2 : cx - h2;
                 cy = (o & 2u) ? cy + h2 : cy - h2;
@@ -566,57 +652,99 @@ __device__ __forceinline__ void com_from_children(NodeHot* hot, uint32_t p) {
 
 // Bottom-up centre of mass, tile pass. The bodies of tile t own the node range
 // [first_node[b0], first_node[b1]) (preorder); an internal node whose body
-// range ends inside the tile has its whole subtree there, so the block reduces
-// those level by level, deepest first, with barriers between levels. Nodes
-// spanning into a later tile are listed for the single-block top pass.
+// range ends inside the tile has its whole subtree there. Each thread climbs
+// from its bodies' leaves; per local node an LDS counter sums the bodies of
+// the children that have arrived, and the thread completing it computes the
+// node from its children (octant order, so the result does not depend on
+// which thread arrives last) and climbs on. Nodes spanning tiles are left to
+// the top pass.
 __global__ __launch_bounds__(kBlock) void k_com_local(uint64_t total,
                                                       const uint32_t* __restrict__ first_node,
                                                       NodeHot* hot,
                                                       const NodeCold* __restrict__ cold,
-                                                      uint64_t cap, uint32_t* __restrict__ top_list,
-                                                      StepCounters* sc) {
+                                                      uint64_t cap) {
+    __shared__ uint32_t s_cnt[kComLds];
     __shared__ uint64_t s_wave[kBlock / 64];
-    __shared__ uint32_t s_scan[kBlock / 64];
-    __shared__ uint32_t s_base;
     const uint64_t b0 = (uint64_t)blockIdx.x * kComTile;
     const uint64_t b1 = min(b0 + (uint64_t)kComTile, total);
     const uint32_t n0 = first_node[b0];
     const uint32_t n1 = (uint32_t)min((uint64_t)first_node[b1], cap);
-    uint64_t ml = 0;
-    uint32_t nspan = 0;
-    for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock) {
-        const uint32_t L = hot[nd].level;
-        ml = L > ml ? L : ml;
-        nspan += hot[nd].skip != nd + 1u && cold[nd].last >= b1;
-    }
-    const int maxl = (int)block_max_u64(ml, s_wave);
-    uint32_t nsp_total;
-    uint32_t slot = block_exclusive_scan(nspan, s_scan, &nsp_total);
-    if (threadIdx.x == 0) s_base = nsp_total ? atomicAdd(&sc->top_count, nsp_total) : 0u;
+    const uint32_t nloc = n1 > n0 ? n1 - n0 : 0u;
+    const bool in_lds = nloc <= (uint32_t)kComLds;
+    if (in_lds)
+        for (uint32_t k = threadIdx.x; k < nloc; k += kBlock) s_cnt[k] = 0u;
     __syncthreads();
-    slot += s_base;
-    for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock)
-        if (hot[nd].skip != nd + 1u && cold[nd].last >= b1) top_list[slot++] = nd;
-    for (int L = maxl - 1; L >= 0; --L) {
-        for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock) {
-            if ((int)hot[nd].level == L && hot[nd].skip != nd + 1u && cold[nd].last < b1)
-                com_from_children(hot, nd);
+    if (in_lds) {
+        for (uint64_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
+            uint32_t nd = first_node[i + 1] - 1u;
+            if (nd >= cap) continue;
+            for (;;) {
+                const int32_t p = cold[nd].parent;
+                if (p < (int32_t)n0 || (uint32_t)p >= nd) break;  // owned by an earlier tile: top pass
+                const NodeCold cp = cold[p];
+                if (cp.last >= b1) break;                         // spans a later tile: top pass
+                const uint32_t nb = cold[nd].last - cold[nd].first + 1u;
+                __threadfence_block();
+                const uint32_t old = atomicAdd(&s_cnt[(uint32_t)p - n0], nb);
+                if (old + nb != cp.last - cp.first + 1u) break;
+                __threadfence_block();
+                com_from_children(hot, (uint32_t)p);
+                nd = (uint32_t)p;
+            }
         }
-        __syncthreads();
+    } else {
+        // more nodes than LDS counters: level by level, deepest first
+        uint64_t ml = 0;
+        for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock) {
+            const uint32_t L = hot[nd].level;
+            ml = L > ml ? L : ml;
+        }
+        const int maxl = (int)block_max_u64(ml, s_wave);
+        for (int L = maxl - 1; L >= 0; --L) {
+            for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock) {
+                if ((int)hot[nd].level == L && hot[nd].skip != nd + 1u && cold[nd].last < b1)
+                    com_from_children(hot, nd);
+            }
+            __syncthreads();
+        }
     }
 }
 
-// Bottom-up centre of mass, top pass: one block reduces the listed
-// tile-spanning nodes level by level (their children are either local nodes,
-// finished by the tile pass, or listed nodes of a deeper level).
+// Bottom-up centre of mass, top pass (one block): the tile-spanning nodes,
+// level by level, deepest first. Each one's kids are read from its kid slots
+// (octant order; a kid is a local node or leaf finished earlier, or a
+// spanning node of the previous level).
 __global__ __launch_bounds__(1024) void k_com_top(NodeHot* hot,
-                                                  const uint32_t* __restrict__ top_list,
-                                                  const StepCounters* __restrict__ sc, int lmax) {
-    const uint32_t cnt = sc->top_count;
-    for (int L = lmax; L >= 0; --L) {
-        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
-            const uint32_t nd = top_list[k];
-            if ((int)hot[nd].level == L) com_from_children(hot, nd);
+                                                  const uint64_t* __restrict__ top_slot,
+                                                  const uint64_t* __restrict__ top_kids,
+                                                  const StepCounters* __restrict__ sc,
+                                                  uint32_t ntiles) {
+    const uint32_t gen = sc->gen;
+    for (int L = kTopSlots - 1; L >= 0; --L) {
+        for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
+            const uint64_t slot = (uint64_t)t * kTopSlots + (uint64_t)L;
+            const uint64_t v = top_slot[slot];
+            if ((uint32_t)(v >> 32) != gen) continue;
+            const uint32_t nd = (uint32_t)v;
+            uint64_t kid[8];
+#pragma unroll
+            for (int o = 0; o < 8; ++o) kid[o] = top_kids[slot * 8 + o];
+            double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                if ((uint32_t)(kid[o] >> 32) != gen) continue;
+                const NodeHot& c = hot[(uint32_t)kid[o]];
+                const double m = c.mass;
+                M += m;
+                X += m * c.mx;
+                Y += m * c.my;
+                Z += m * c.mz;
+            }
+            const double inv = 1.0 / M;
+            hot[nd].mx = X * inv;
+            hot[nd].my = Y * inv;
+            hot[nd].mz = Z * inv;
+            hot[nd].mass = M;
         }
         __syncthreads();
     }
@@ -762,28 +890,24 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
         atomicAdd((unsigned long long*)&d.st->interactions, (unsigned long long)count);
     }
     if (INTEGRATE) {
+        // verlet.rs:24-82. After the Taylor first step the velocity is
+        // (x' - x) / dt of the stored positions and is formed on download.
         const double dt = d.dt;
         const double dt2 = dt * dt;
-        double nx, ny, nz, vx, vy, vz;
         if (d.st->history == 0) {
             const double ux = d.dvx[j], uy = d.dvy[j], uz = d.dvz[j];
-            nx = x + ux * dt + 0.5 * ax * dt2;
-            ny = y + uy * dt + 0.5 * ay * dt2;
-            nz = z + uz * dt + 0.5 * az * dt2;
-            vx = ux + ax * dt;
-            vy = uy + ay * dt;
-            vz = uz + az * dt;
+            d.dx[j] = x + ux * dt + 0.5 * ax * dt2;
+            d.dy[j] = y + uy * dt + 0.5 * ay * dt2;
+            d.dz[j] = z + uz * dt + 0.5 * az * dt2;
+            d.dvx[j] = ux + ax * dt;
+            d.dvy[j] = uy + ay * dt;
+            d.dvz[j] = uz + az * dt;
         } else {
-            nx = 2.0 * x - d.dpx[j] + ax * dt2;
-            ny = 2.0 * y - d.dpy[j] + ay * dt2;
-            nz = 2.0 * z - d.dpz[j] + az * dt2;
-            vx = (nx - x) / dt;
-            vy = (ny - y) / dt;
-            vz = (nz - z) / dt;
+            d.dx[j] = 2.0 * x - d.dpx[j] + ax * dt2;
+            d.dy[j] = 2.0 * y - d.dpy[j] + ay * dt2;
+            d.dz[j] = 2.0 * z - d.dpz[j] + az * dt2;
         }
         d.dpx[j] = x; d.dpy[j] = y; d.dpz[j] = z;
-        d.dx[j] = nx; d.dy[j] = ny; d.dz[j] = nz;
-        d.dvx[j] = vx; d.dvy[j] = vy; d.dvz[j] = vz;
     } else {
         double* o = acc_out + 3 * (d.off + d.did[j]);
         o[0] = ax; o[1] = ay; o[2] = az;
@@ -859,28 +983,23 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
         }
     }
     if (INTEGRATE) {
+        // verlet.rs:24-82; velocities are stored only by the Taylor first step
         const double dt = d.dt;
         const double dt2 = dt * dt;
-        double nx, ny, nz, vx, vy, vz;
         if (d.st->history == 0) {
             const double ux = d.svx[i], uy = d.svy[i], uz = d.svz[i];
-            nx = x + ux * dt + 0.5 * ax * dt2;
-            ny = y + uy * dt + 0.5 * ay * dt2;
-            nz = z + uz * dt + 0.5 * az * dt2;
-            vx = ux + ax * dt;
-            vy = uy + ay * dt;
-            vz = uz + az * dt;
+            d.dx[i] = x + ux * dt + 0.5 * ax * dt2;
+            d.dy[i] = y + uy * dt + 0.5 * ay * dt2;
+            d.dz[i] = z + uz * dt + 0.5 * az * dt2;
+            d.dvx[i] = ux + ax * dt;
+            d.dvy[i] = uy + ay * dt;
+            d.dvz[i] = uz + az * dt;
         } else {
-            nx = 2.0 * x - d.spx[i] + ax * dt2;
-            ny = 2.0 * y - d.spy[i] + ay * dt2;
-            nz = 2.0 * z - d.spz[i] + az * dt2;
-            vx = (nx - x) / dt;
-            vy = (ny - y) / dt;
-            vz = (nz - z) / dt;
+            d.dx[i] = 2.0 * x - d.spx[i] + ax * dt2;
+            d.dy[i] = 2.0 * y - d.spy[i] + ay * dt2;
+            d.dz[i] = 2.0 * z - d.spz[i] + az * dt2;
         }
         d.dpx[i] = x; d.dpy[i] = y; d.dpz[i] = z;
-        d.dx[i] = nx; d.dy[i] = ny; d.dz[i] = nz;
-        d.dvx[i] = vx; d.dvy[i] = vy; d.dvz[i] = vz;
         d.dm[i] = m;
         d.dfix[i] = d.sfix[i];
         d.did[i] = d.sid[i];
@@ -890,15 +1009,33 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
     }
 }
 
+// Velocity of body j: stored (before and right after the Taylor first step)
+// or the reference's backward difference (x - x_prev) / dt (verlet.rs:68-70).
+struct VelocitySource {
+    const double *vx, *vy, *vz, *px, *py, *pz;
+    int derive;
+    double dt;
+    __device__ __forceinline__ void at(uint64_t j, double x, double y, double z, double& ox,
+                                       double& oy, double& oz) const {
+        if (derive) {
+            ox = (x - px[j]) / dt;
+            oy = (y - py[j]) / dt;
+            oz = (z - pz[j]) / dt;
+        } else {
+            ox = vx[j];
+            oy = vy[j];
+            oz = vz[j];
+        }
+    }
+};
+
 // FP64 energy terms of one state: per block partial sums of
 // {V (half of the all-pairs sum), K, Px, Py, Pz}. U(r) = -(ma mb / sqrt(e))
 // atan(sqrt(e) / r) is the potential of |F| = ma mb / (r^2 + e).
 __global__ __launch_bounds__(kBlock) void k_energy(const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z,
-                                                   const double* __restrict__ vx,
-                                                   const double* __restrict__ vy,
-                                                   const double* __restrict__ vz,
+                                                   VelocitySource vs,
                                                    const double* __restrict__ m, uint64_t n,
                                                    double e, double* __restrict__ partial) {
     __shared__ double4 sj[kBlock];
@@ -928,8 +1065,10 @@ __global__ __launch_bounds__(kBlock) void k_energy(const double* __restrict__ x,
     }
     double K = 0, Px = 0, Py = 0, Pz = 0;
     if (valid) {
-        K = 0.5 * mi * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
-        Px = mi * vx[i]; Py = mi * vy[i]; Pz = mi * vz[i];
+        double vx, vy, vz;
+        vs.at(i, xi, yi, zi, vx, vy, vz);
+        K = 0.5 * mi * (vx * vx + vy * vy + vz * vz);
+        Px = mi * vx; Py = mi * vy; Pz = mi * vz;
     }
     red[0][threadIdx.x] = 0.5 * V;
     red[1][threadIdx.x] = K;
@@ -961,13 +1100,12 @@ __global__ void k_unpack(const Entity* __restrict__ aos, uint64_t n, double* x, 
 }
 
 __global__ void k_pack(Entity* __restrict__ aos, uint64_t n, const double* x, const double* y,
-                       const double* z, const double* vx, const double* vy, const double* vz,
-                       const uint32_t* id) {
+                       const double* z, VelocitySource v, const uint32_t* id) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     Entity& e = aos[id[j]];
     e.x = x[j]; e.y = y[j]; e.z = z[j];
-    e.vx = vx[j]; e.vy = vy[j]; e.vz = vz[j];
+    v.at(j, e.x, e.y, e.z, e.vx, e.vy, e.vz);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
