@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel eagerly")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the streaming-copy bandwidth probe on BYTES and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -86,10 +89,18 @@ def main():
 
     import physim_amd as P
 
+    if args.copy_probe:
+        eng = P.Engine(0)
+        gbps = P._lib.copy_probe(eng, args.copy_probe, reps=10)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "read_plus_write_GBps": gbps}))
+        return
+
     theta, e, dt = 1.5, 0.5, 1e-5
     state = P.cube_and_stars(args.bodies, seed=1, spin=1000.0)
     n = len(state)
     eng = P.Engine(0)
+    if args.no_graphs:
+        eng.set_graphs(False)
     sim = P.Sim(eng, state, kind="astro2", theta=theta, e=e, dt=dt, precision=args.precision)
 
     if args.warmup:

@@ -97,6 +97,11 @@ int pg_engine_profile_read(pg_engine* eng, char* names, size_t cap, double* ms,
                            uint64_t* launches, uint32_t max_kernels, uint32_t* count);
 int pg_engine_profile_reset(pg_engine* eng);
 int pg_engine_synchronize(pg_engine* eng);
+/* A plain streaming copy of `bytes` device bytes, `reps` times; *gbps
+ * receives (read + write) bytes per second of the timed launches. A
+ * calibration probe for memory counters and the achievable-bandwidth
+ * figure; no reference counterpart. */
+int pg_engine_copy_probe(pg_engine* eng, uint64_t bytes, uint32_t reps, double* gbps);
 
 /* ---- simulations ------------------------------------------------------ */
 int pg_sim_create(pg_engine* eng, uint64_t n, const pg_params* params, pg_sim** out);

@@ -115,6 +115,7 @@ def lib():
         "pg_engine_profile_read": (I, [P, ctypes.c_char_p, SZ, P, P, U32, ctypes.POINTER(U32)]),
         "pg_engine_profile_reset": (I, [P]),
         "pg_engine_synchronize": (I, [P]),
+        "pg_engine_copy_probe": (I, [P, U64, U32, ctypes.POINTER(D)]),
         "pg_sim_create": (I, [P, U64, ctypes.POINTER(Params), PP]),
         "pg_sim_destroy": (None, [P]),
         "pg_sim_set_params": (I, [P, ctypes.POINTER(Params)]),
@@ -415,3 +416,10 @@ class Server:
 
     def clock(self):
         return lib().pg_server_clock(self.h)
+
+
+def copy_probe(engine, nbytes, reps=20):
+    """Achieved (read + write) GB/s of a plain streaming copy of nbytes."""
+    g = ctypes.c_double()
+    check(lib().pg_engine_copy_probe(engine.h, nbytes, reps, ctypes.byref(g)))
+    return g.value

@@ -283,6 +283,40 @@ extern "C" int pg_engine_synchronize(pg_engine* e) {
     return PG_OK;
 }
 
+extern "C" int pg_engine_copy_probe(pg_engine* e, uint64_t bytes, uint32_t reps, double* gbps) {
+    if (!e || bytes < 16 || reps == 0) return pg_set_error(PG_ERR_ARG, "bad copy probe arguments");
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    const uint64_t n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, n16 * 16));
+    hipError_t he = hipMalloc(&b, n16 * 16);
+    if (he != hipSuccess) {
+        (void)hipFree(a);
+        return pg_set_error(PG_ERR_NOMEM, "copy probe: %s", hipGetErrorString(he));
+    }
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    const unsigned grid = 4096;
+    (void)hipMemsetAsync(a, 1, n16 * 16, e->stream);
+    k_copy<<<grid, kBlock, 0, e->stream>>>((const uint4*)a, (uint4*)b, n16);  // warm-up
+    (void)hipEventRecord(t0, e->stream);
+    for (uint32_t r = 0; r < reps; ++r)
+        k_copy<<<grid, kBlock, 0, e->stream>>>((const uint4*)a, (uint4*)b, n16);
+    (void)hipEventRecord(t1, e->stream);
+    he = hipEventSynchronize(t1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (he != hipSuccess) return pg_set_error(PG_ERR_HIP, "copy probe: %s", hipGetErrorString(he));
+    if (gbps) *gbps = 2.0 * (double)(n16 * 16) * reps / (ms * 1e-3) / 1e9;
+    return PG_OK;
+}
+
 // Grows the batch scratch to hold `bodies` bodies, `nodes` tree nodes, `segs`
 // segments and `blocks` direct-sum blocks. One allocation, carved in place.
 static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_t segs,

@@ -625,7 +625,11 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 ext = h2;
             }
         }
-        if (d.count) atomicMax(&d.st->max_level, (uint32_t)hi);
+        // read first: once the running maximum is reached almost no lane
+        // needs the atomic (a stale read only costs an extra atomic)
+        if (d.count && (uint32_t)hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&d.st->max_level, (uint32_t)hi);
     }
 }
 
@@ -871,6 +875,20 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
         }
         pending = pending && !mine;
     }
+    // interaction totals (debug counting): one atomic per wave and segment,
+    // every lane converged here
+    bool cpend = valid && desc[s].count;
+    while (__ballot(cpend)) {
+        const int lead = __ffsll((long long)__ballot(cpend)) - 1;
+        const uint32_t s0 = uniform_u32(__shfl(s, lead));
+        const bool mine = cpend && s == s0;
+        unsigned long long c = mine ? (unsigned long long)count : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if ((int)(threadIdx.x & 63) == lead)
+            atomicAdd((unsigned long long*)&desc[s0].st->interactions, c);
+        cpend = cpend && !mine;
+    }
     if (!valid) return;
     const SegDesc& d = desc[s];
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -885,10 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
             az = (double)ka.sz - (double)ka.cz;
         }
     }
-    if (d.count) {
-        d.counts[d.did[j]] = count;
-        atomicAdd((unsigned long long*)&d.st->interactions, (unsigned long long)count);
-    }
+    if (d.count) d.counts[d.did[j]] = count;
     if (INTEGRATE) {
         // verlet.rs:24-82. After the Taylor first step the velocity is
         // (x' - x) / dt of the stored positions and is formed on download.
@@ -1106,6 +1121,14 @@ __global__ void k_pack(Entity* __restrict__ aos, uint64_t n, const double* x, co
     Entity& e = aos[id[j]];
     e.x = x[j]; e.y = y[j]; e.z = z[j];
     v.at(j, e.x, e.y, e.z, e.vx, e.vy, e.vz);
+}
+
+// Streaming copy (16-byte vectors, grid-stride): the bandwidth probe.
+__global__ __launch_bounds__(kBlock) void k_copy(const uint4* __restrict__ src,
+                                                 uint4* __restrict__ dst, uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
