@@ -511,10 +511,6 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
             e->cold, e->cap_nodes, e->top_slot, e->top_kids, e->sc);
     });
     const unsigned com_tiles = (unsigned)((total + kComTile - 1) / kComTile);
-    launch(e, cap, "com_local", [&] {
-        k_com_local<<<com_tiles, kBlock, 0, st>>>(total, e->first_node, e->hot, e->cold,
-                                                  e->cap_nodes);
-    });
     launch(e, cap, "com_top", [&] {
         k_com_top<<<1, 1024, 0, st>>>(e->hot, e->top_slot, e->top_kids, e->sc, com_tiles);
     });

@@ -15,10 +15,10 @@ constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr uint32_t kSpinLimit = 1u << 22;  // bound of every device-side wait
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
-constexpr int kComTile = 2048;     // sorted bodies per centre-of-mass tile
 constexpr int kNodeTile = 2048;    // sorted bodies per node-building block
+constexpr int kComTile = kNodeTile; // the centre-of-mass tiles are the node tiles
 constexpr int kNodeThreads = 1024; // threads per node-building block
-constexpr int kComLds = 6144;      // node counters per tile kept in LDS
+constexpr int kNodeLds = 3584;     // nodes per tile whose centre of mass is reduced in LDS
 constexpr int kTopSlots = 33;      // tree levels 0..32: one tile-spanning node each
 
 // Device-side error bits (sticky per segment), mapped to pg_status on the host.
@@ -79,8 +79,6 @@ struct NodeHot {
 };
 struct NodeCold {
     uint32_t first, last;   // sorted-body range
-    int32_t parent;
-    uint32_t arrivals;      // bottom-up reduction counter
 };
 
 // Batch-wide counters reset at the start of every step.
@@ -88,7 +86,7 @@ struct StepCounters {
     uint32_t gen;           // bumped per step: versions look-back status words
     uint32_t sort_ticket[kMaxPasses];
     uint32_t scan_ticket;
-    uint32_t top_count;     // tree nodes spanning centre-of-mass tiles
+    uint32_t top_levels;    // 1 + deepest level of a tile-spanning node (0: none)
     uint32_t hist[kMaxPasses * 256];
 };
 

@@ -123,7 +123,7 @@ __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCoun
         sc->gen += 1;
         for (int p = 0; p < kMaxPasses; ++p) sc->sort_ticket[p] = 0;
         sc->scan_ticket = 0;
-        sc->top_count = 0;
+        sc->top_levels = 0;
     }
     for (uint32_t i = t; i < kMaxPasses * 256; i += stride) sc->hist[i] = 0;
     for (uint32_t s = t; s < nseg; s += stride) {
@@ -503,37 +503,90 @@ __device__ __forceinline__ uint64_t range_begin(const Keys& skeys, uint64_t i, u
     return first;
 }
 
-// Writes every node a sorted body owns: geometric centre by the reference's
-// descent, half-width level, octant, sorted range, skip (= first node of the
-// body after the range), parent; a leaf takes its body's exact position and
-// mass. One block per tile of kNodeTile sorted bodies, whose keys, node
-// offsets and levels are staged in LDS for the range searches. For the
-// centre-of-mass passes, an internal node whose range leaves its owner's
-// centre-of-mass tile is recorded (generation-tagged) at slot
-// tile * kTopSlots + level, and a node whose parent is such a node
-// registers itself in the parent's kid slot (octant order).
+// Mass and centre of mass of an internal node from its children, in octant
+// order (children are p+1 and then each child's skip until p's skip).
+__device__ __forceinline__ void com_from_children(NodeHot* hot, uint32_t p) {
+    double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+    const uint32_t send = hot[p].skip;
+    for (uint32_t c = p + 1u; c < send;) {
+        const double m = hot[c].mass;
+        M += m;
+        X += m * hot[c].mx;
+        Y += m * hot[c].my;
+        Z += m * hot[c].mz;
+        const uint32_t nxt = hot[c].skip;
+        c = nxt > c ? nxt : c + 1u;
+    }
+    const double inv = 1.0 / M;
+    hot[p].mx = X * inv;
+    hot[p].my = Y * inv;
+    hot[p].mz = Z * inv;
+    hot[p].mass = M;
+}
+
+// Local-node tags of the LDS reduction (low 6 bits: level).
+constexpr uint8_t kTagInternal = 0x80;  // internal node whose subtree lies in the tile
+constexpr uint8_t kTagNone = 0xFF;      // no node written (never matches a level)
+
+// Builds the tree and its masses and centres of mass, one block per tile of
+// kNodeTile sorted bodies (keys, node offsets and levels staged in LDS for
+// the range searches).
+//
+// Every node a sorted body owns gets its geometric centre by the reference's
+// descent, half-width level, octant, sorted range and skip (= first node of
+// the body after the range); a leaf takes its body's exact position and mass.
+// The tile's bodies own the preorder node range [first[b0], first[b1]); an
+// internal node whose bodies all lie in the tile has its whole subtree there,
+// so its mass and centre of mass are reduced in LDS, deepest level first, from
+// its children in octant order (the arithmetic of com_from_children). A tile
+// with more than kNodeLds nodes does the same level loop on the global records.
+//
+// An internal node whose range leaves the tile is left to k_com_top: it is
+// recorded (generation-tagged) at slot tile * kTopSlots + level, a node whose
+// parent is such a node registers itself in the parent's kid slot (octant
+// order), and sc->top_levels bounds the levels k_com_top visits.
 template <int DIM>
 __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys_g,
     uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ first_g,
     const uint8_t* __restrict__ lo_level, NodeHot* __restrict__ hot, NodeCold* __restrict__ cold,
     uint64_t cap, uint64_t* __restrict__ top_slot, uint64_t* __restrict__ top_kids,
-    const StepCounters* __restrict__ sc) {
+    StepCounters* __restrict__ sc) {
     __shared__ uint64_t s_keys[kNodeTile];
     __shared__ uint32_t s_first[kNodeTile + 1];
     __shared__ uint8_t s_lo[kNodeTile];
+    __shared__ double s_m[kNodeLds], s_x[kNodeLds], s_y[kNodeLds], s_z[kNodeLds];
+    __shared__ uint16_t s_skip[kNodeLds];
+    __shared__ uint8_t s_tag[kNodeLds];
+    __shared__ uint64_t s_prev, s_next;  // keys just before and just after the tile
+    __shared__ uint32_t s_maxl, s_topl;
     const uint64_t gen = (uint64_t)sc->gen << 32;
     const uint64_t b0 = (uint64_t)blockIdx.x * kNodeTile;
     const uint64_t b1 = min(b0 + (uint64_t)kNodeTile, total);
+    const uint32_t n0 = first_g[b0];
+    const uint32_t n1 = (uint32_t)min((uint64_t)first_g[b1], cap);
+    const uint32_t nloc = n1 > n0 ? n1 - n0 : 0u;
+    const bool in_lds = nloc <= (uint32_t)kNodeLds;
     for (uint64_t j = b0 + threadIdx.x; j < b1; j += kNodeThreads) {
         s_keys[j - b0] = skeys_g[j];
         s_lo[j - b0] = lo_level[j];
     }
     for (uint64_t j = b0 + threadIdx.x; j <= b1; j += kNodeThreads) s_first[j - b0] = first_g[j];
+    if (in_lds)
+        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) s_tag[k] = kTagNone;
+    if (threadIdx.x == 0) {
+        // segment bits differ across segments, so these never match a prefix
+        // of another segment
+        s_prev = b0 > 0 ? skeys_g[b0 - 1] : ~0ull;
+        s_next = b1 < total ? skeys_g[b1] : ~0ull;
+        s_maxl = 0;
+        s_topl = 0;
+    }
     __syncthreads();
     const KeyWindow skeys{skeys_g, s_keys, b0, b1};
     const U32Window first_node{first_g, s_first, b0, b1 + 1};
     const uint64_t dmask = (1u << DIM) - 1u;
+    const uint64_t tile = b0 / (uint64_t)kNodeTile;
     for (uint64_t i = b0 + threadIdx.x; i < b1; i += kNodeThreads) {
         const uint64_t key = s_keys[i - b0];
         const uint32_t s = seg_of_key(key, segbits);
@@ -565,18 +618,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             }
             const bool leaf = L == hi;
             const uint64_t last = leaf ? i : range_end<DIM>(skeys, i, end, lmax, L);
-            int32_t parent;
-            uint64_t pfirst = i;  // first body of the parent
-            if (k > 0) {
-                parent = (int32_t)(nd - 1);
-            } else if (L == 0) {
-                parent = -1;
-            } else {
-                const uint64_t a = range_begin<DIM>(skeys, i, d.off, lmax, L - 1);
-                const uint8_t loa = (a >= b0 && a < b1) ? s_lo[a - b0] : lo_level[a];
-                parent = (int32_t)(first_node[a] + (uint32_t)(L - 1 - loa));
-                pfirst = a;
-            }
+            const bool spans = !leaf && last >= b1;
             NodeHot h;
             h.cx = cx; h.cy = cy; h.cz = cz;
             if (leaf) {
@@ -597,22 +639,37 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             h.octant = L > 0 ? (uint8_t)((key >> (DIM * (lmax - L))) & dmask) : 0;
             h.pad = 0;
             hot[nd] = h;
-            NodeCold c;
-            c.first = (uint32_t)i;
-            c.last = (uint32_t)last;
-            c.parent = parent;
-            c.arrivals = 0;
-            cold[nd] = c;
-            const uint64_t tile = i / (uint64_t)kComTile;
-            if (!leaf && last >= (tile + 1) * (uint64_t)kComTile && L < kTopSlots)
+            cold[nd] = NodeCold{(uint32_t)i, (uint32_t)last};
+            if (in_lds) {
+                const uint32_t loc = (uint32_t)nd - n0;
+                s_tag[loc] = (uint8_t)L | (leaf || spans ? 0 : kTagInternal);
+                s_skip[loc] = (uint16_t)min(h.skip - n0, nloc);
+                if (leaf) {
+                    s_m[loc] = h.mass;
+                    s_x[loc] = h.mx; s_y[loc] = h.my; s_z[loc] = h.mz;
+                }
+            }
+            if (spans && L < kTopSlots) {
                 top_slot[tile * kTopSlots + L] = gen | (uint64_t)nd;
-            if (parent >= 0) {
-                // the parent spans its tile iff the first body past that tile
-                // still shares the parent's prefix
-                const uint64_t pt = pfirst / (uint64_t)kComTile;
-                const uint64_t te = (pt + 1) * (uint64_t)kComTile;
-                if (te < end &&
-                    key_prefix<DIM>(skeys[te], lmax, L - 1) == key_prefix<DIM>(key, lmax, L - 1))
+                atomicMax(&s_topl, (uint32_t)L + 1u);
+            }
+            if (L > 0) {
+                // Kid registration when the parent (level L-1) leaves its
+                // tile. The parent is node nd-1 owned by body i when k > 0;
+                // for k == 0 it starts at an earlier body (body i-1 shares
+                // L-1 levels with i), in this tile unless the body before
+                // the tile shares its prefix, and then it spans for sure.
+                const uint64_t P = key_prefix<DIM>(key, lmax, L - 1);
+                const bool p_before = k == 0 && b0 > d.off && key_prefix<DIM>(s_prev, lmax, L - 1) == P;
+                uint64_t pt = tile;
+                bool pspans;
+                if (p_before) {
+                    pt = range_begin<DIM>(skeys, i, d.off, lmax, L - 1) / (uint64_t)kNodeTile;
+                    pspans = true;
+                } else {
+                    pspans = b1 < end && key_prefix<DIM>(s_next, lmax, L - 1) == P;
+                }
+                if (pspans)
                     top_kids[(pt * kTopSlots + (uint64_t)(L - 1)) * 8 + h.octant] =
                         gen | (uint64_t)nd;
             }
@@ -625,90 +682,59 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 ext = h2;
             }
         }
+        atomicMax(&s_maxl, (uint32_t)hi);
         // read first: once the running maximum is reached almost no lane
         // needs the atomic (a stale read only costs an extra atomic)
         if (d.count && (uint32_t)hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(&d.st->max_level, (uint32_t)hi);
     }
-}
-
-// Mass and centre of mass of an internal node from its children, in octant
-// order (children are p+1 and then each child's skip until p's skip).
-__device__ __forceinline__ void com_from_children(NodeHot* hot, uint32_t p) {
-    double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
-    const uint32_t send = hot[p].skip;
-    for (uint32_t c = p + 1u; c < send;) {
-        const double m = hot[c].mass;
-        M += m;
-        X += m * hot[c].mx;
-        Y += m * hot[c].my;
-        Z += m * hot[c].mz;
-        const uint32_t nxt = hot[c].skip;
-        c = nxt > c ? nxt : c + 1u;
-    }
-    const double inv = 1.0 / M;
-    hot[p].mx = X * inv;
-    hot[p].my = Y * inv;
-    hot[p].mz = Z * inv;
-    hot[p].mass = M;
-}
-
-// Bottom-up centre of mass, tile pass. The bodies of tile t own the node range
-// [first_node[b0], first_node[b1]) (preorder); an internal node whose body
-// range ends inside the tile has its whole subtree there. Each thread climbs
-// from its bodies' leaves; per local node an LDS counter sums the bodies of
-// the children that have arrived, and the thread completing it computes the
-// node from its children (octant order, so the result does not depend on
-// which thread arrives last) and climbs on. Nodes spanning tiles are left to
-// the top pass.
-__global__ __launch_bounds__(kBlock) void k_com_local(uint64_t total,
-                                                      const uint32_t* __restrict__ first_node,
-                                                      NodeHot* hot,
-                                                      const NodeCold* __restrict__ cold,
-                                                      uint64_t cap) {
-    __shared__ uint32_t s_cnt[kComLds];
-    __shared__ uint64_t s_wave[kBlock / 64];
-    const uint64_t b0 = (uint64_t)blockIdx.x * kComTile;
-    const uint64_t b1 = min(b0 + (uint64_t)kComTile, total);
-    const uint32_t n0 = first_node[b0];
-    const uint32_t n1 = (uint32_t)min((uint64_t)first_node[b1], cap);
-    const uint32_t nloc = n1 > n0 ? n1 - n0 : 0u;
-    const bool in_lds = nloc <= (uint32_t)kComLds;
-    if (in_lds)
-        for (uint32_t k = threadIdx.x; k < nloc; k += kBlock) s_cnt[k] = 0u;
     __syncthreads();
+    if (threadIdx.x == 0 && s_topl) atomicMax(&sc->top_levels, s_topl);
+    // internal nodes sit above the deepest leaf
+    const int maxl = (int)min(s_maxl, (uint32_t)kTopSlots);
     if (in_lds) {
-        for (uint64_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
-            uint32_t nd = first_node[i + 1] - 1u;
-            if (nd >= cap) continue;
-            for (;;) {
-                const int32_t p = cold[nd].parent;
-                if (p < (int32_t)n0 || (uint32_t)p >= nd) break;  // owned by an earlier tile: top pass
-                const NodeCold cp = cold[p];
-                if (cp.last >= b1) break;                         // spans a later tile: top pass
-                const uint32_t nb = cold[nd].last - cold[nd].first + 1u;
-                __threadfence_block();
-                const uint32_t old = atomicAdd(&s_cnt[(uint32_t)p - n0], nb);
-                if (old + nb != cp.last - cp.first + 1u) break;
-                __threadfence_block();
-                com_from_children(hot, (uint32_t)p);
-                nd = (uint32_t)p;
+        for (int L = maxl - 1; L >= 0; --L) {
+            const uint8_t want = (uint8_t)L | kTagInternal;
+            for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
+                if (s_tag[k] != want) continue;
+                double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+                const uint32_t send = s_skip[k];
+                for (uint32_t c = k + 1u; c < send;) {
+                    const double m = s_m[c];
+                    M += m;
+                    X += m * s_x[c];
+                    Y += m * s_y[c];
+                    Z += m * s_z[c];
+                    const uint32_t nxt = s_skip[c];
+                    c = nxt > c ? nxt : c + 1u;
+                }
+                const double inv = 1.0 / M;
+                s_m[k] = M;
+                s_x[k] = X * inv;
+                s_y[k] = Y * inv;
+                s_z[k] = Z * inv;
+            }
+            __syncthreads();
+        }
+        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
+            if ((s_tag[k] & kTagInternal) && s_tag[k] != kTagNone) {
+                NodeHot& h = hot[n0 + k];
+                h.mx = s_x[k];
+                h.my = s_y[k];
+                h.mz = s_z[k];
+                h.mass = s_m[k];
             }
         }
     } else {
-        // more nodes than LDS counters: level by level, deepest first
-        uint64_t ml = 0;
-        for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock) {
-            const uint32_t L = hot[nd].level;
-            ml = L > ml ? L : ml;
-        }
-        const int maxl = (int)block_max_u64(ml, s_wave);
+        // more nodes than LDS holds: the same level loop on the global
+        // records, made visible across the block by device-scope fences
         for (int L = maxl - 1; L >= 0; --L) {
-            for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kBlock) {
+            for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kNodeThreads) {
                 if ((int)hot[nd].level == L && hot[nd].skip != nd + 1u && cold[nd].last < b1)
                     com_from_children(hot, nd);
             }
+            __threadfence();
             __syncthreads();
         }
     }
@@ -724,7 +750,8 @@ __global__ __launch_bounds__(1024) void k_com_top(NodeHot* hot,
                                                   const StepCounters* __restrict__ sc,
                                                   uint32_t ntiles) {
     const uint32_t gen = sc->gen;
-    for (int L = kTopSlots - 1; L >= 0; --L) {
+    const int levels = (int)min(sc->top_levels, (uint32_t)kTopSlots);
+    for (int L = levels - 1; L >= 0; --L) {
         for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
             const uint64_t slot = (uint64_t)t * kTopSlots + (uint64_t)L;
             const uint64_t v = top_slot[slot];
