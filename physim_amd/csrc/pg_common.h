@@ -14,6 +14,8 @@ constexpr int kScanItems = 8;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr uint32_t kSpinLimit = 1u << 22;  // bound of every device-side wait
+constexpr int kSortLookWindow = 8;  // look-back status loads in flight per digit
+constexpr int kScanLookWindow = 8;  // look-back status loads in flight (node-offset scan)
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
 constexpr int kNodeTile = 2048;    // sorted bodies per node-building block
 constexpr int kComTile = kNodeTile; // the centre-of-mass tiles are the node tiles
@@ -84,8 +86,6 @@ struct NodeCold {
 // Batch-wide counters reset at the start of every step.
 struct StepCounters {
     uint32_t gen;           // bumped per step: versions look-back status words
-    uint32_t sort_ticket[kMaxPasses];
-    uint32_t scan_ticket;
     uint32_t top_levels;    // 1 + deepest level of a tile-spanning node (0: none)
     uint32_t hist[kMaxPasses * 256];
 };

@@ -89,27 +89,53 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 }
 
 // Decoupled look-back: exclusive prefix of tile `tile` for one value lane.
+// Status words travel between XCDs through memory, so each load costs a
+// round trip of the order of a microsecond and the walk is latency-bound: the
+// words of W predecessors are loaded together (their latencies overlap), then
+// consumed nearest first: aggregates add up until an inclusive prefix ends the
+// walk; a predecessor that has not published yet restarts the window there.
+// With every tile walking at W tiles per round trip, the last of T tiles
+// resolves after about sqrt(2T / W) round trips.
+//
+// Tiles are numbered by blockIdx: blocks are dispatched in index order, so
+// the lowest unfinished tile is always resident and never waits. Every wait
+// is bounded by kSpinLimit regardless (kErrSpin).
+template <int W>
 __device__ __forceinline__ uint32_t lookback(const uint64_t* status, uint32_t tile,
                                              uint32_t stride, uint32_t col, uint32_t tag,
                                              uint32_t* err) {
     uint32_t excl = 0;
     int64_t t = (int64_t)tile - 1;
     uint32_t spins = 0;
+    const uint32_t want = tag & 0x3FFFFFFFu;
     while (t >= 0) {
-        uint64_t w = status_load(status + (uint64_t)t * stride + col);
-        uint32_t wtag = (uint32_t)(w >> 34);
-        uint64_t flag = (w >> 32) & 3ull;
-        if (wtag != (tag & 0x3FFFFFFFu) || flag == 0) {
+        uint64_t w[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+            w[k] = t - k >= 0 ? status_load(status + (uint64_t)(t - k) * stride + col) : 0ull;
+        bool done = false, stalled = false;
+        int used = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            if (done || stalled || t - k < 0) continue;
+            const uint64_t flag = (w[k] >> 32) & 3ull;
+            if ((uint32_t)(w[k] >> 34) != want || flag == 0) {
+                stalled = true;
+                continue;
+            }
+            excl += (uint32_t)w[k];
+            ++used;
+            if (flag == kFlagPrefix) done = true;
+        }
+        if (done) break;
+        t -= used;
+        if (stalled) {
             if (++spins > kSpinLimit) {
                 atomicOr(err, kErrSpin);
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
-            continue;
         }
-        excl += (uint32_t)w;
-        if (flag == kFlagPrefix) break;
-        --t;
     }
     return excl;
 }
@@ -121,8 +147,6 @@ __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCoun
     const uint32_t stride = gridDim.x * blockDim.x;
     if (t == 0) {
         sc->gen += 1;
-        for (int p = 0; p < kMaxPasses; ++p) sc->sort_ticket[p] = 0;
-        sc->scan_ticket = 0;
         sc->top_levels = 0;
     }
     for (uint32_t i = t; i < kMaxPasses * 256; i += stride) sc->hist[i] = 0;
@@ -248,7 +272,6 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
                                                       uint32_t* __restrict__ vout, uint64_t n,
                                                       int pass, StepCounters* sc,
                                                       uint64_t* status, uint32_t* err) {
-    __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wcnt[kBlock / 64][256];
     __shared__ uint32_t s_lstart[256];
     __shared__ uint32_t s_gbase[256];
@@ -257,10 +280,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
     __shared__ uint32_t s_vals[kSortTile];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_tile = atomicAdd(&sc->sort_ticket[pass], 1u);
     for (int i = tid; i < (kBlock / 64) * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const uint32_t tag = (sc->gen << 4) | (uint32_t)(pass + 1);
     const uint64_t base = (uint64_t)tile * kSortTile;
     const int shift = 8 * pass;
@@ -315,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
     const uint32_t gbase = block_exclusive_scan(ghist, s_scan, &dummy);
     uint32_t excl = 0;
     if (tile > 0) {
-        excl = lookback(status, tile, 256, d, tag, err);
+        excl = lookback<kSortLookWindow>(status, tile, 256, d, tag, err);
         status_store(status + (uint64_t)tile * 256 + d, status_pack(tag, kFlagPrefix, excl + cnt));
     }
     s_lstart[d] = lstart;
@@ -356,14 +378,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
     uint32_t* __restrict__ first_node, uint8_t* __restrict__ lo_level, StepCounters* sc,
     uint64_t* status, uint32_t* err) {
-    __shared__ uint32_t s_tile;
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_excl;
     __shared__ uint32_t s_cnt[kScanTile];
     const int tid = threadIdx.x;
-    if (tid == 0) s_tile = atomicAdd(&sc->scan_ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const uint32_t tag = (sc->gen << 4) | 15u;
     const uint64_t tbase = (uint64_t)tile * kScanTile;
 
@@ -418,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
             s_excl = 0;
         } else {
             status_store(my, status_pack(tag, kFlagAgg, tile_total));
-            const uint32_t ex = lookback(status, tile, 1, 0, tag, err);
+            const uint32_t ex = lookback<kScanLookWindow>(status, tile, 1, 0, tag, err);
             status_store(my, status_pack(tag, kFlagPrefix, ex + tile_total));
             s_excl = ex;
         }
