@@ -24,19 +24,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def algorithmic_bytes(kernel, n, nodes, passes, interactions):
-    """Compulsory bytes one launch of `kernel` moves for n bodies (each
-    input read once, each output written once; see DESIGN.md's table)."""
-    internal = max(nodes - n, 0)
+def algorithmic_bytes(kernel, n, nodes, tiles):
+    """Compulsory bytes one launch of `kernel` moves for n bodies and the
+    given tree size: each input read once, each output written once."""
     table = {
-        "prep": 0.0,
-        "extent": 24.0 * n,                         # read x, y, z
-        "keys": 36.0 * n,                           # read x, y, z; write key, index
-        "sort_pass": 24.0 * n,                      # read + write (key, index)
-        "scan_gather": (8 + 4 + 85 + 85 + 4 + 1) * n,  # keys, perm, state in+out, offsets, level
-        "nodes": 49.0 * n + 80.0 * nodes,           # keys, offsets, leaf bodies; node records
-        "com": 4.0 * n + (80.0 + 32.0) * internal,  # climb + children read + parent write
-        "walk": (33.0 + 24.0 + 72.0) * n + 64.0 * nodes,  # own state, Stormer in/out, tree
+        "extent": 24.0 * n,                          # read x, y, z
+        "keys": 36.0 * n + 1024.0 * tiles,           # read x, y, z; write key, index; digit counts
+        "sort_count": 8.0 * n + 1024.0 * tiles,      # read keys; write digit counts
+        "sort_scan": 2048.0 * tiles,                 # read + write the count matrix
+        "sort_scatter": 24.0 * n + 1024.0 * tiles,   # read + write (key, index); read offsets
+        # keys, sorted index, state read (x y z px py pz m id fixed) and written
+        # (+ velocities on the first two steps only), node offsets, levels
+        "scan_gather": (8 + 4 + 65 + 65 + 4 + 1) * n,
+        # keys, node offsets, levels, leaf x y z m; node records written once
+        "nodes": 45.0 * n + (64.0 + 8.0) * nodes,
+        # own x y z m fixed + Stormer x_prev in, x and x_prev out; tree read once
+        "walk": (33.0 + 24.0 + 48.0) * n + 64.0 * nodes,
         "direct": 32.0 * n,
     }
     return table.get(kernel, 0.0)
@@ -83,8 +86,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              "the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this engine runs on one GPU only (asked for {args.gpus}); "
+              "a multi-GPU run is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import physim_amd as P
@@ -122,20 +125,22 @@ def main():
     sim.set_counting(True)
     sim.step(1)
     sim.set_counting(False)
-    st2 = st = sim.stats()
-    passes = prof.get("sort_pass", (0, 0))[1] // max(args.profile_steps, 1)
+    st = sim.stats()
+    tiles = (n + 2047) // 2048
     per_step_ms = {k: v[0] / args.profile_steps for k, v in prof.items()}
     dom = max(per_step_ms, key=per_step_ms.get)
     avg_ms = prof[dom][0] / prof[dom][1]
-    abytes = algorithmic_bytes(dom, n, st2["nodes"], passes, st2["interactions"])
+    abytes = algorithmic_bytes(dom, n, st["nodes"], tiles)
     achieved = abytes / (avg_ms * 1e-3) / 1e9
 
+    # HBM bytes per launch of that kernel, measured with rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE on this workload (scaled by a copy kernel of known size)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         tr = json.load(open(tfile))
-        if tr.get("kernel") == dom and tr.get("bodies") == n:
-            traffic = tr.get("bytes_per_launch")
+        if tr.get("bodies") == n and args.precision == "mixed":
+            traffic = tr.get("bytes_per_launch", {}).get(dom)
 
     value = n * args.steps / elapsed
     out = {

@@ -74,7 +74,8 @@ struct pg_engine {
     uint32_t* vals[2] = {nullptr, nullptr};
     uint32_t* first_node = nullptr;
     uint8_t* lo_level = nullptr;
-    uint64_t* sort_status = nullptr;
+    uint32_t* sort_cnt = nullptr;  // per-tile digit counts, digit-major
+    uint32_t* sort_tot = nullptr;  // per-digit totals of one pass
     uint64_t* scan_status = nullptr;
     NodeHot* hot = nullptr;
     NodeCold* cold = nullptr;
@@ -343,7 +344,8 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_k0 = take(bodies * 8), o_k1 = take(bodies * 8);
     const uint64_t o_v0 = take(bodies * 4), o_v1 = take(bodies * 4);
     const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies);
-    const uint64_t o_ss = take(sort_tiles * 256 * 8), o_cs = take(scan_tiles * 8);
+    const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
+    const uint64_t o_cs = take(scan_tiles * 8);
     const uint64_t o_hot = take(nodes * sizeof(NodeHot)), o_cold = take(nodes * sizeof(NodeCold));
     const uint64_t o_acc = take(bodies * 24);
     const uint64_t o_d0 = take(segs * sizeof(SegDesc)), o_d1 = take(segs * sizeof(SegDesc));
@@ -376,7 +378,8 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->vals[1] = (uint32_t*)(b + o_v1);
     e->first_node = (uint32_t*)(b + o_fn);
     e->lo_level = (uint8_t*)(b + o_lo);
-    e->sort_status = (uint64_t*)(b + o_ss);
+    e->sort_cnt = (uint32_t*)(b + o_ss);
+    e->sort_tot = (uint32_t*)(b + o_st);
     e->scan_status = (uint64_t*)(b + o_cs);
     e->hot = (NodeHot*)(b + o_hot);
     e->cold = (NodeCold*)(b + o_cold);
@@ -396,7 +399,6 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->cap_chunks = (uint32_t)chunks;
     e->cap_top = top;
     // Status words are versioned; zero them once so no stale tag can match.
-    HIP_TRY(hipMemsetAsync(e->sort_status, 0, sort_tiles * 256 * 8, e->stream));
     HIP_TRY(hipMemsetAsync(e->scan_status, 0, scan_tiles * 8, e->stream));
     HIP_TRY(hipMemsetAsync(e->sc, 0, sizeof(StepCounters), e->stream));
     HIP_TRY(hipMemsetAsync(e->err_global, 0, 256, e->stream));
@@ -483,18 +485,26 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "extent_final", [&] {
         k_extent_final<<<bp.nseg, kBlock, 0, st>>>(desc, e->ext_partial);
     });
-    launch(e, cap, "keys", [&] {
-        k_keys<DIM><<<grid_for(total, 512), kBlock, 0, st>>>(desc, e->seg_off, bp.nseg, total,
-                                                            bp.segbits, bp.lmax, bp.passes,
-                                                            e->keys[0], e->vals[0], e->sc);
-    });
     const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
+    launch(e, cap, "keys", [&] {
+        k_keys<DIM><<<sort_tiles, kBlock, 0, st>>>(desc, e->seg_off, bp.nseg, total, bp.segbits,
+                                                   bp.lmax, e->keys[0], e->vals[0], e->sort_cnt,
+                                                   sort_tiles);
+    });
     for (int p = 0; p < bp.passes; ++p) {
         const int a = p & 1, b = a ^ 1;
-        launch(e, cap, "sort_pass", [&] {
-            k_sort_pass<<<sort_tiles, kBlock, 0, st>>>(e->keys[a], e->vals[a], e->keys[b],
-                                                       e->vals[b], total, p, e->sc,
-                                                       e->sort_status, e->err_global);
+        if (p > 0)
+            launch(e, cap, "sort_count", [&] {
+                k_sort_count<<<sort_tiles, kBlock, 0, st>>>(e->keys[a], total, 8 * p, e->sort_cnt,
+                                                            sort_tiles);
+            });
+        launch(e, cap, "sort_scan", [&] {
+            k_sort_scan<<<256, kScanThreads, 0, st>>>(e->sort_cnt, sort_tiles, e->sort_tot);
+        });
+        launch(e, cap, "sort_scatter", [&] {
+            k_sort_scatter<<<sort_tiles, kBlock, 0, st>>>(e->keys[a], e->vals[a], e->keys[b],
+                                                          e->vals[b], total, 8 * p, e->sort_cnt,
+                                                          sort_tiles, e->sort_tot, e->err_global);
         });
     }
     const uint64_t* skeys = e->keys[bp.passes & 1];

@@ -14,8 +14,8 @@ constexpr int kScanItems = 8;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr uint32_t kSpinLimit = 1u << 22;  // bound of every device-side wait
-constexpr int kSortLookWindow = 8;  // look-back status loads in flight per digit
 constexpr int kScanLookWindow = 8;  // look-back status loads in flight (node-offset scan)
+constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort tiles
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
 constexpr int kNodeTile = 2048;    // sorted bodies per node-building block
 constexpr int kComTile = kNodeTile; // the centre-of-mass tiles are the node tiles
@@ -87,7 +87,6 @@ struct NodeCold {
 struct StepCounters {
     uint32_t gen;           // bumped per step: versions look-back status words
     uint32_t top_levels;    // 1 + deepest level of a tile-spanning node (0: none)
-    uint32_t hist[kMaxPasses * 256];
 };
 
 }  // namespace pg

@@ -64,7 +64,9 @@ __device__ __forceinline__ int key_lcp(uint64_t a, uint64_t b, int lmax) {
     return lz / DIM;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
+// Exclusive scan of one value per thread over a block of NT threads
+// (s_wave holds NT / 64 values).
+template <int NT = kBlock>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave,
                                                          uint32_t* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -78,7 +80,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     __syncthreads();
     uint32_t wbase = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
+    for (int w = 0; w < NT / 64; ++w) {
         uint32_t c = s_wave[w];
         if (w < wave) wbase += c;
         tot += c;
@@ -149,7 +151,6 @@ __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCoun
         sc->gen += 1;
         sc->top_levels = 0;
     }
-    for (uint32_t i = t; i < kMaxPasses * 256; i += stride) sc->hist[i] = 0;
     for (uint32_t s = t; s < nseg; s += stride) {
         SegState* st = desc[s].st;
         st->extent_bits = 0;
@@ -219,59 +220,133 @@ __global__ __launch_bounds__(kBlock) void k_extent_final(const SegDesc* __restri
     if (threadIdx.x == 0) d.st->extent_bits = m;
 }
 
+// Adds one digit per lane (256 = none) to the block's LDS digit counters:
+// lanes of a wave that hold the same digit are found with match-any ballots
+// and their leader adds the group's size. Every lane of the wave must call.
+__device__ __forceinline__ void wave_count_digit(uint32_t* s_h, uint32_t d) {
+    const bool valid = d < 256u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+    }
+    if (valid && (int)(threadIdx.x & 63) == __ffsll((long long)peers) - 1)
+        atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
+}
+
 // Morton key by exact FP64 descent (octree.rs:160-215 / quadtree.rs:161-194):
 // at each level the octant is (x > cx) | (y > cy) << 1 [| (z > cz) << 2] and
 // the child centre is c -/+ extent/2, rounded as the reference rounds it.
-// Also accumulates the digit histograms of every radix pass.
+// One block per radix-sort tile of kSortTile bodies (the batch's segments are
+// concatenated); also writes the tile's digit counts of the first radix pass
+// (digit-major: cnt[digit * tiles + tile]).
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_keys(const SegDesc* __restrict__ desc,
                                                  const uint64_t* __restrict__ seg_off,
                                                  uint32_t nseg, uint64_t total, int segbits,
-                                                 int lmax, int passes, uint64_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals, StepCounters* sc) {
-    __shared__ uint32_t h[kMaxPasses * 256];
-    for (int i = threadIdx.x; i < passes * 256; i += kBlock) h[i] = 0;
+                                                 int lmax, uint64_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals,
+                                                 uint32_t* __restrict__ cnt, uint32_t tiles) {
+    __shared__ uint32_t s_h[256];
+    s_h[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
-        const uint32_t s = seg_of(seg_off, nseg, g);
-        const SegDesc& d = desc[s];
-        const uint64_t j = g - d.off;
-        const double px = d.sx[j], py = d.sy[j], pz = d.sz[j];
-        double ext = bits_f64(d.st->extent_bits);
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        uint64_t key = 0;
-        for (int l = 0; l < lmax; ++l) {
-            uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
-            if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
-            const double h2 = ext / 2.0;
-            cx = (o & 1u) ? cx + h2 : cx - h2;
-            cy = (o & 2u) ? cy + h2 : cy - h2;
-            if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-            ext = h2;
-            key = (key << DIM) | o;
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t g = base + (uint64_t)k * kBlock + threadIdx.x;
+        uint32_t dig = 256u;
+        if (g < total) {
+            const uint32_t s = seg_of(seg_off, nseg, g);
+            const SegDesc& d = desc[s];
+            const uint64_t j = g - d.off;
+            const double px = d.sx[j], py = d.sy[j], pz = d.sz[j];
+            double ext = bits_f64(d.st->extent_bits);
+            double cx = 0.0, cy = 0.0, cz = 0.0;
+            uint64_t key = 0;
+            for (int l = 0; l < lmax; ++l) {
+                uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
+                if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
+                const double h2 = ext / 2.0;
+                cx = (o & 1u) ? cx + h2 : cx - h2;
+                cy = (o & 2u) ? cy + h2 : cy - h2;
+                if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+                ext = h2;
+                key = (key << DIM) | o;
+            }
+            if (segbits) key |= (uint64_t)s << (64 - segbits);
+            keys[g] = key;
+            vals[g] = (uint32_t)g;
+            dig = (uint32_t)(key & 255u);
         }
-        if (segbits) key |= (uint64_t)s << (64 - segbits);
-        keys[g] = key;
-        vals[g] = (uint32_t)g;
-        for (int p = 0; p < passes; ++p) atomicAdd(&h[p * 256 + ((key >> (8 * p)) & 255u)], 1u);
+        wave_count_digit(s_h, dig);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < passes * 256; i += kBlock)
-        if (h[i]) atomicAdd(&sc->hist[i], h[i]);
+    cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
-// One stable LSD radix pass (8-bit digit) over (key, value) pairs: the
-// single-pass "onesweep" scheme. Ranks inside a tile come from wave64
-// match-any ballots and per-wave LDS counters, the tile's global offsets from
-// the precomputed digit histogram plus a decoupled look-back over tiles;
-// the tile is reordered in LDS and written out in runs per digit.
-__global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict__ kin,
-                                                      const uint32_t* __restrict__ vin,
-                                                      uint64_t* __restrict__ kout,
-                                                      uint32_t* __restrict__ vout, uint64_t n,
-                                                      int pass, StepCounters* sc,
-                                                      uint64_t* status, uint32_t* err) {
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits, three kernels
+// per pass and no inter-tile waiting: (1) each tile counts its digits,
+// (2) one block per digit scans that digit's counts over the tiles,
+// (3) each tile ranks its keys and scatters them. (Tiles that wait on their
+// predecessors, as in a single-pass look-back scheme, pay a chain of
+// cross-XCD round trips when every tile of a pass is resident at once.)
+
+// (1) Digit counts of one tile: cnt[digit * tiles + tile].
+__global__ __launch_bounds__(kBlock) void k_sort_count(const uint64_t* __restrict__ kin, uint64_t n,
+                                                       int shift, uint32_t* __restrict__ cnt,
+                                                       uint32_t tiles) {
+    __shared__ uint32_t s_h[256];
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    uint64_t key[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t idx = base + (uint64_t)k * kBlock + threadIdx.x;
+        key[k] = idx < n ? kin[idx] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t idx = base + (uint64_t)k * kBlock + threadIdx.x;
+        wave_count_digit(s_h, idx < n ? (uint32_t)((key[k] >> shift) & 255u) : 256u);
+    }
+    __syncthreads();
+    cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
+}
+
+// (2) One block per digit: exclusive scan of the digit's per-tile counts (in
+// place) and the digit's total.
+__global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict__ cnt,
+                                                            uint32_t tiles,
+                                                            uint32_t* __restrict__ tot) {
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    uint32_t* row = cnt + (uint64_t)blockIdx.x * tiles;
+    const uint32_t per = (tiles + kScanThreads - 1) / kScanThreads;
+    const uint32_t b = threadIdx.x * per;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b + k < tiles) s += row[b + k];
+    uint32_t total;
+    uint32_t run = block_exclusive_scan<kScanThreads>(s, s_wave, &total);
+    for (uint32_t k = 0; k < per; ++k) {
+        if (b + k < tiles) {
+            const uint32_t c = row[b + k];
+            row[b + k] = run;
+            run += c;
+        }
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = total;
+}
+
+// (3) Ranks inside the tile come from wave64 match-any ballots and per-wave
+// LDS counters; a key's output index is the digit's global base (scan of the
+// digit totals) + the tile's offset within the digit (pass 2) + its rank in
+// the tile. The tile is reordered in LDS and written out in runs per digit.
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, uint64_t n, int shift, const uint32_t* __restrict__ cnt,
+    uint32_t tiles, const uint32_t* __restrict__ tot, uint32_t* err) {
     __shared__ uint32_t s_wcnt[kBlock / 64][256];
     __shared__ uint32_t s_lstart[256];
     __shared__ uint32_t s_gbase[256];
@@ -283,9 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
     for (int i = tid; i < (kBlock / 64) * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = blockIdx.x;
-    const uint32_t tag = (sc->gen << 4) | (uint32_t)(pass + 1);
     const uint64_t base = (uint64_t)tile * kSortTile;
-    const int shift = 8 * pass;
 
     uint64_t key[kSortItems];
     uint32_t val[kSortItems], dig[kSortItems], rank[kSortItems];
@@ -322,26 +395,18 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
     __syncthreads();
     // Per digit (one thread each): exclusive offsets over waves, tile count.
     const uint32_t d = tid;
-    uint32_t cnt = 0;
+    uint32_t c = 0;
 #pragma unroll
     for (int w = 0; w < kBlock / 64; ++w) {
-        const uint32_t c = s_wcnt[w][d];
-        s_wcnt[w][d] = cnt;
-        cnt += c;
+        const uint32_t cw = s_wcnt[w][d];
+        s_wcnt[w][d] = c;
+        c += cw;
     }
-    status_store(status + (uint64_t)tile * 256 + d,
-                 status_pack(tag, tile == 0 ? kFlagPrefix : kFlagAgg, cnt));
     uint32_t dummy;
-    const uint32_t lstart = block_exclusive_scan(cnt, s_scan, &dummy);
-    const uint32_t ghist = sc->hist[pass * 256 + d];
-    const uint32_t gbase = block_exclusive_scan(ghist, s_scan, &dummy);
-    uint32_t excl = 0;
-    if (tile > 0) {
-        excl = lookback<kSortLookWindow>(status, tile, 256, d, tag, err);
-        status_store(status + (uint64_t)tile * 256 + d, status_pack(tag, kFlagPrefix, excl + cnt));
-    }
+    const uint32_t lstart = block_exclusive_scan(c, s_scan, &dummy);
+    const uint32_t gbase = block_exclusive_scan(tot[d], s_scan, &dummy);
     s_lstart[d] = lstart;
-    s_gbase[d] = gbase + excl - lstart;
+    s_gbase[d] = gbase + cnt[(uint64_t)d * tiles + tile] - lstart;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
