@@ -76,7 +76,11 @@ struct pg_engine {
     uint8_t* lo_level = nullptr;
     uint32_t* sort_cnt = nullptr;  // per-tile digit counts, digit-major
     uint32_t* sort_tot = nullptr;  // per-digit totals of one pass
-    uint64_t* scan_status = nullptr;
+    uint32_t* node_local = nullptr;  // per-body node offset within its tile
+    uint32_t* tile_tot = nullptr;    // nodes per tile
+    uint32_t* tile_base = nullptr;   // first node of each tile (+ total)
+    uint32_t* lvl_last = nullptr;    // per tile and level: last node-starting body (+1), scanned
+    uint32_t* lvl_first = nullptr;   // per tile and level: first node-starting body, scanned
     NodeHot* hot = nullptr;
     NodeCold* cold = nullptr;
     double* acc_out = nullptr;
@@ -345,7 +349,10 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_v0 = take(bodies * 4), o_v1 = take(bodies * 4);
     const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies);
     const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
-    const uint64_t o_cs = take(scan_tiles * 8);
+    const uint64_t o_nl = take((bodies + 1) * 4), o_tt = take((scan_tiles + 2) * 4);
+    const uint64_t o_tb = take((scan_tiles + 2) * 4);
+    const uint64_t o_ll = take((scan_tiles + 2) * kTopSlots * 4);
+    const uint64_t o_lf = take((scan_tiles + 2) * kTopSlots * 4);
     const uint64_t o_hot = take(nodes * sizeof(NodeHot)), o_cold = take(nodes * sizeof(NodeCold));
     const uint64_t o_acc = take(bodies * 24);
     const uint64_t o_d0 = take(segs * sizeof(SegDesc)), o_d1 = take(segs * sizeof(SegDesc));
@@ -380,7 +387,11 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->lo_level = (uint8_t*)(b + o_lo);
     e->sort_cnt = (uint32_t*)(b + o_ss);
     e->sort_tot = (uint32_t*)(b + o_st);
-    e->scan_status = (uint64_t*)(b + o_cs);
+    e->node_local = (uint32_t*)(b + o_nl);
+    e->tile_tot = (uint32_t*)(b + o_tt);
+    e->tile_base = (uint32_t*)(b + o_tb);
+    e->lvl_last = (uint32_t*)(b + o_ll);
+    e->lvl_first = (uint32_t*)(b + o_lf);
     e->hot = (NodeHot*)(b + o_hot);
     e->cold = (NodeCold*)(b + o_cold);
     e->acc_out = (double*)(b + o_acc);
@@ -399,7 +410,6 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->cap_chunks = (uint32_t)chunks;
     e->cap_top = top;
     // Status words are versioned; zero them once so no stale tag can match.
-    HIP_TRY(hipMemsetAsync(e->scan_status, 0, scan_tiles * 8, e->stream));
     HIP_TRY(hipMemsetAsync(e->sc, 0, sizeof(StepCounters), e->stream));
     HIP_TRY(hipMemsetAsync(e->err_global, 0, 256, e->stream));
     HIP_TRY(hipMemsetAsync(e->top_slot, 0, top * 8, e->stream));
@@ -512,13 +522,19 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     const unsigned scan_tiles = (unsigned)((total + kScanTile - 1) / kScanTile);
     launch(e, cap, "scan_gather", [&] {
         k_scan_gather<DIM><<<scan_tiles, kBlock, 0, st>>>(
-            desc, total, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->first_node, e->lo_level,
-            e->sc, e->scan_status, e->err_global);
+            desc, total, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_tot,
+            e->lo_level, e->lvl_last, e->lvl_first, e->err_global);
+    });
+    launch(e, cap, "tree_scan", [&] {
+        k_tree_scan<<<1 + kTopSlots, kScanThreads, 0, st>>>(e->tile_tot, e->tile_base, scan_tiles,
+                                                             e->lvl_last, e->lvl_first,
+                                                             e->first_node, total);
     });
     launch(e, cap, "nodes", [&] {
-        k_nodes<DIM><<<(unsigned)((total + kNodeTile - 1) / kNodeTile), kNodeThreads, 0, st>>>(
-            desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->first_node, e->lo_level, e->hot,
-            e->cold, e->cap_nodes, e->top_slot, e->top_kids, e->sc);
+        k_nodes<DIM><<<scan_tiles, kNodeThreads, 0, st>>>(
+            desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_base,
+            e->lvl_last, e->lvl_first, e->first_node, e->lo_level, e->hot, e->cold, e->cap_nodes,
+            e->top_slot, e->top_kids, e->sc);
     });
     const unsigned com_tiles = (unsigned)((total + kComTile - 1) / kComTile);
     launch(e, cap, "com_top", [&] {

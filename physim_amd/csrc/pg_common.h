@@ -22,6 +22,8 @@ constexpr int kComTile = kNodeTile; // the centre-of-mass tiles are the node til
 constexpr int kNodeThreads = 1024; // threads per node-building block
 constexpr int kNodeLds = 3584;     // nodes per tile whose centre of mass is reduced in LDS
 constexpr int kTopSlots = 33;      // tree levels 0..32: one tile-spanning node each
+static_assert(kScanTile == kNodeTile, "node offsets and node building share their tiles");
+static_assert(kTopSlots <= kBlock, "per-level tables are filled by one thread per level");
 
 // Device-side error bits (sticky per segment), mapped to pg_status on the host.
 enum : uint32_t {

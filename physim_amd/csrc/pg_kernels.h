@@ -431,25 +431,38 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
 }
 
 // Per sorted body: its common-prefix lengths with its neighbours give the
-// tree levels it owns (lo..hi, hi = its leaf); the exclusive scan of the
-// per-body node counts lays the tree out in preorder. Also permutes the
-// state into the new Morton order (velocities only while they are not
-// derivable from the two positions) and flags equal keys (depth). Bodies are
-// read and written striped across the block (coalesced); the tile's counts
-// are scanned through LDS in index order.
+// tree levels it owns (lo..hi, hi = its leaf; lo = 1 + common prefix with the
+// previous body, 0 at a segment start); the exclusive scan of the per-body
+// node counts lays the tree out in preorder. This kernel writes each body's
+// offset within its tile (node_local) and the tile's total; k_tree_scan adds
+// the tiles up. Also permutes the state into the new Morton order
+// (velocities only while they are not derivable from the two positions) and
+// flags equal keys (depth).
+//
+// Node ranges: the node at level L containing body i starts at the last body
+// j <= i with lo[j] <= L and ends before the first body j > i with lo[j] <= L.
+// Per tile and level this kernel records the last such body (+1, 0 = none)
+// and the first one (~0 = none); k_tree_scan turns them into a prefix maximum
+// and a suffix minimum over the tiles, so the node builder answers both
+// queries across tiles in one load.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys,
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
-    uint32_t* __restrict__ first_node, uint8_t* __restrict__ lo_level, StepCounters* sc,
-    uint64_t* status, uint32_t* err) {
+    uint32_t* __restrict__ node_local, uint32_t* __restrict__ tile_tot,
+    uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_last,
+    uint32_t* __restrict__ lvl_first, uint32_t* err) {
     __shared__ uint32_t s_scan[kBlock / 64];
-    __shared__ uint32_t s_excl;
     __shared__ uint32_t s_cnt[kScanTile];
+    __shared__ uint32_t s_lastlo[kTopSlots], s_firstlo[kTopSlots];
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
-    const uint32_t tag = (sc->gen << 4) | 15u;
     const uint64_t tbase = (uint64_t)tile * kScanTile;
+    if (tid < kTopSlots) {
+        s_lastlo[tid] = 0u;
+        s_firstlo[tid] = ~0u;
+    }
+    __syncthreads();
 
 #pragma unroll 2
     for (int k = 0; k < kScanItems; ++k) {
@@ -460,9 +473,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
             const uint64_t key = skeys[i];
             const uint32_t s = seg_of_key(key, segbits);
             const uint64_t src = s < nseg ? (uint64_t)svals[i] - desc[s].off : ~0ull;
+            int lo = 0;
             if (s >= nseg || src >= desc[s].n || i < desc[s].off || i >= desc[s].off + desc[s].n) {
                 atomicOr(err, kErrCapacity);
-                lo_level[i] = 0;
                 c = 1;
             } else {
                 const SegDesc& d = desc[s];
@@ -475,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 }
                 if (lcpL >= lmax) lcpL = lmax - 1;
                 c = 1u + (uint32_t)max(0, lcpR - lcpL);
-                lo_level[i] = (uint8_t)(lcpL + 1);
+                lo = lcpL + 1;
                 const uint64_t j = i - d.off;
                 d.dx[j] = d.sx[src]; d.dy[j] = d.sy[src]; d.dz[j] = d.sz[src];
                 d.dpx[j] = d.spx[src]; d.dpy[j] = d.spy[src]; d.dpz[j] = d.spz[src];
@@ -486,6 +499,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 d.dfix[j] = d.sfix[src];
                 d.did[j] = d.sid[src];
             }
+            lo_level[i] = (uint8_t)lo;
+            atomicMax(&s_lastlo[lo], (uint32_t)i + 1u);
+            atomicMin(&s_firstlo[lo], (uint32_t)i);
         }
         s_cnt[li] = c;
     }
@@ -494,97 +510,140 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) sum += s_cnt[tid * kScanItems + k];
     uint32_t tile_total;
-    const uint32_t texcl = block_exclusive_scan(sum, s_scan, &tile_total);
-    if (tid == 0) {
-        uint64_t* my = status + tile;
-        if (tile == 0) {
-            status_store(my, status_pack(tag, kFlagPrefix, tile_total));
-            s_excl = 0;
-        } else {
-            status_store(my, status_pack(tag, kFlagAgg, tile_total));
-            const uint32_t ex = lookback<kScanLookWindow>(status, tile, 1, 0, tag, err);
-            status_store(my, status_pack(tag, kFlagPrefix, ex + tile_total));
-            s_excl = ex;
-        }
-    }
-    __syncthreads();
-    uint32_t run = s_excl + texcl;
+    uint32_t run = block_exclusive_scan(sum, s_scan, &tile_total);
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t li = tid * kScanItems + k;
         const uint32_t c = s_cnt[li];
         s_cnt[li] = run;
         run += c;
-        if (tbase + li + 1 == total) first_node[total] = run;
     }
+    if (tid < kTopSlots) {
+        uint32_t mx = 0u, mn = ~0u;
+        for (int v = 0; v <= tid; ++v) {
+            mx = max(mx, s_lastlo[v]);
+            mn = min(mn, s_firstlo[v]);
+        }
+        lvl_last[(uint64_t)tile * kTopSlots + tid] = mx;
+        lvl_first[(uint64_t)tile * kTopSlots + tid] = mn;
+    }
+    if (tid == 0) tile_tot[tile] = tile_total;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t li = (uint32_t)k * kBlock + tid;
-        if (tbase + li < total) first_node[tbase + li] = s_cnt[li];
+        if (tbase + li < total) node_local[tbase + li] = s_cnt[li];
     }
 }
 
-// Sorted-key reads through an LDS window: the block's tile [b0, b1) is
-// staged in LDS, anything outside comes from global memory.
-struct KeyWindow {
-    const uint64_t* __restrict__ g;
-    const uint64_t* s;
-    uint64_t b0, b1;
-    __device__ __forceinline__ uint64_t operator[](uint64_t j) const {
-        return (j >= b0 && j < b1) ? s[j - b0] : g[j];
+// Inclusive scan over a block of NT threads with an associative operation.
+template <int NT, class Op>
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s_wave, Op op) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o);
+        if (lane >= o) v = op(t, v);
     }
-};
-struct U32Window {
-    const uint32_t* __restrict__ g;
-    const uint32_t* s;
-    uint64_t b0, b1;
-    __device__ __forceinline__ uint32_t operator[](uint64_t j) const {
-        return (j >= b0 && j < b1) ? s[j - b0] : g[j];
-    }
-};
+    if (lane == 63) s_wave[wave] = v;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) v = op(s_wave[w], v);
+    __syncthreads();
+    return v;
+}
 
-// Last sorted index j >= i whose key shares i's prefix at `level`.
-template <int DIM, class Keys>
-__device__ __forceinline__ uint64_t range_end(const Keys& skeys, uint64_t i, uint64_t end, int lmax,
-                                              int level) {
-    const uint64_t P = key_prefix<DIM>(skeys[i], lmax, level);
-    uint64_t last = i, step = 1;
+// Block 0: exclusive scan of the tile node totals into tile_base[0..tiles]
+// (tile_base[tiles] = all nodes) and first_node[total] = all nodes. Block
+// 1 + L: over the tiles, prefix maximum of lvl_last[., L] and suffix minimum
+// of lvl_first[., L], in place.
+__global__ __launch_bounds__(kScanThreads) void k_tree_scan(
+    const uint32_t* __restrict__ tile_tot, uint32_t* __restrict__ tile_base, uint32_t tiles,
+    uint32_t* __restrict__ lvl_last, uint32_t* __restrict__ lvl_first,
+    uint32_t* __restrict__ first_node, uint64_t total) {
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    const uint32_t per = (tiles + kScanThreads - 1) / kScanThreads;
+    const uint32_t b = threadIdx.x * per;
+    if (blockIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t k = 0; k < per; ++k)
+            if (b + k < tiles) s += tile_tot[b + k];
+        uint32_t all;
+        uint32_t run = block_exclusive_scan<kScanThreads>(s, s_wave, &all);
+        for (uint32_t k = 0; k < per; ++k) {
+            if (b + k < tiles) {
+                tile_base[b + k] = run;
+                run += tile_tot[b + k];
+            }
+        }
+        if (threadIdx.x == 0) {
+            tile_base[tiles] = all;
+            first_node[total] = all;
+        }
+        return;
+    }
+    const uint32_t L = blockIdx.x - 1;
+    auto opmax = [](uint32_t a, uint32_t c) { return a > c ? a : c; };
+    auto opmin = [](uint32_t a, uint32_t c) { return a < c ? a : c; };
+    // prefix maximum in tile order
+    uint32_t m = 0u;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b + k < tiles) m = max(m, lvl_last[(uint64_t)(b + k) * kTopSlots + L]);
+    uint32_t run = block_inclusive_scan<kScanThreads>(m, s_wave, opmax);
+    run = __shfl_up(run, 1);
+    {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) run = 0u;
+        for (int w = 0; w < wave; ++w) run = max(run, s_wave[w]);
+    }
+    // (s_wave still holds the per-wave maxima of the inclusive scan)
+    for (uint32_t k = 0; k < per; ++k) {
+        if (b + k < tiles) {
+            uint32_t* p = lvl_last + (uint64_t)(b + k) * kTopSlots + L;
+            run = max(run, *p);
+            *p = run;
+        }
+    }
+    __syncthreads();
+    // suffix minimum: thread r walks the tiles from the end
+    const uint32_t rb = threadIdx.x * per;
+    uint32_t mn = ~0u;
+    for (uint32_t k = 0; k < per; ++k)
+        if (rb + k < tiles) mn = min(mn, lvl_first[(uint64_t)(tiles - 1 - (rb + k)) * kTopSlots + L]);
+    uint32_t rrun = block_inclusive_scan<kScanThreads>(mn, s_wave, opmin);
+    rrun = __shfl_up(rrun, 1);
+    {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) rrun = ~0u;
+        for (int w = 0; w < wave; ++w) rrun = min(rrun, s_wave[w]);
+    }
+    for (uint32_t k = 0; k < per; ++k) {
+        if (rb + k < tiles) {
+            uint32_t* p = lvl_first + (uint64_t)(tiles - 1 - (rb + k)) * kTopSlots + L;
+            rrun = min(rrun, *p);
+            *p = rrun;
+        }
+    }
+}
+
+// Last index j >= i (j < end) whose key shares key i's prefix at `level`,
+// over keys staged in LDS (galloping, then bisection).
+template <int DIM>
+__device__ __forceinline__ uint32_t range_end_lds(const uint64_t* s, uint32_t i, uint32_t end,
+                                                  int lmax, int level) {
+    const uint64_t P = key_prefix<DIM>(s[i], lmax, level);
+    uint32_t last = i, step = 1;
     for (;;) {
-        const uint64_t probe = last + step;
-        if (probe >= end || key_prefix<DIM>(skeys[probe], lmax, level) != P) break;
+        const uint32_t probe = last + step;
+        if (probe >= end || key_prefix<DIM>(s[probe], lmax, level) != P) break;
         last = probe;
         step <<= 1;
     }
-    uint64_t hi = min(last + step, end);
+    uint32_t hi = min(last + step, end);
     while (hi - last > 1) {
-        const uint64_t mid = (last + hi) >> 1;
-        if (key_prefix<DIM>(skeys[mid], lmax, level) == P) last = mid; else hi = mid;
+        const uint32_t mid = (last + hi) >> 1;
+        if (key_prefix<DIM>(s[mid], lmax, level) == P) last = mid; else hi = mid;
     }
     return last;
-}
-
-// First sorted index j <= i (j >= begin) whose key shares i's prefix at `level`.
-template <int DIM, class Keys>
-__device__ __forceinline__ uint64_t range_begin(const Keys& skeys, uint64_t i, uint64_t begin,
-                                                int lmax, int level) {
-    const uint64_t P = key_prefix<DIM>(skeys[i], lmax, level);
-    uint64_t first = i, step = 1;
-    for (;;) {
-        if (first < begin + step) break;
-        const uint64_t probe = first - step;
-        if (key_prefix<DIM>(skeys[probe], lmax, level) != P) break;
-        first = probe;
-        step <<= 1;
-    }
-    // keys[first] matches; keys[lo] (if lo >= begin and lo < first) may not.
-    uint64_t lo = first >= begin + step ? first - step : begin;
-    if (key_prefix<DIM>(skeys[lo], lmax, level) == P) return lo;
-    while (first - lo > 1) {
-        const uint64_t mid = (lo + first) >> 1;
-        if (key_prefix<DIM>(skeys[mid], lmax, level) == P) first = mid; else lo = mid;
-    }
-    return first;
 }
 
 // Mass and centre of mass of an internal node from its children, in octant
@@ -613,17 +672,23 @@ constexpr uint8_t kTagInternal = 0x80;  // internal node whose subtree lies in t
 constexpr uint8_t kTagNone = 0xFF;      // no node written (never matches a level)
 
 // Builds the tree and its masses and centres of mass, one block per tile of
-// kNodeTile sorted bodies (keys, node offsets and levels staged in LDS for
-// the range searches).
+// kNodeTile sorted bodies (the tiles of k_scan_gather).
 //
-// Every node a sorted body owns gets its geometric centre by the reference's
-// descent, half-width level, octant, sorted range and skip (= first node of
-// the body after the range); a leaf takes its body's exact position and mass.
-// The tile's bodies own the preorder node range [first[b0], first[b1]); an
-// internal node whose bodies all lie in the tile has its whole subtree there,
-// so its mass and centre of mass are reduced in LDS, deepest level first, from
-// its children in octant order (the arithmetic of com_from_children). A tile
-// with more than kNodeLds nodes does the same level loop on the global records.
+// The tile's node offsets (tile base + offsets within the tile) are staged in
+// LDS with its keys and levels, and written out as first_node. Every node a
+// sorted body owns gets its geometric centre by the reference's descent,
+// half-width level, octant, sorted range and skip (= first node of the body
+// after the range); a leaf takes its body's exact position and mass. A node's
+// range end is found by bisection over the staged keys; a node that runs past
+// the tile ends before the first later body starting a node at its level or
+// above (the scanned per-level table), so no search leaves the tile.
+//
+// The tile's bodies own the preorder node range [tile_base[t],
+// tile_base[t+1]); an internal node whose bodies all lie in the tile has its
+// whole subtree there, so its mass and centre of mass are reduced in LDS,
+// deepest level first, from its children in octant order (the arithmetic of
+// com_from_children). A tile with more than kNodeLds nodes does the same level
+// loop on the global records.
 //
 // An internal node whose range leaves the tile is left to k_com_top: it is
 // recorded (generation-tagged) at slot tile * kTopSlots + level, a node whose
@@ -632,7 +697,9 @@ constexpr uint8_t kTagNone = 0xFF;      // no node written (never matches a leve
 template <int DIM>
 __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys_g,
-    uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ first_g,
+    uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ node_local,
+    const uint32_t* __restrict__ tile_base, const uint32_t* __restrict__ lvl_last,
+    const uint32_t* __restrict__ lvl_first, uint32_t* __restrict__ first_node,
     const uint8_t* __restrict__ lo_level, NodeHot* __restrict__ hot, NodeCold* __restrict__ cold,
     uint64_t cap, uint64_t* __restrict__ top_slot, uint64_t* __restrict__ top_kids,
     StepCounters* __restrict__ sc) {
@@ -645,32 +712,31 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     __shared__ uint64_t s_prev, s_next;  // keys just before and just after the tile
     __shared__ uint32_t s_maxl, s_topl;
     const uint64_t gen = (uint64_t)sc->gen << 32;
-    const uint64_t b0 = (uint64_t)blockIdx.x * kNodeTile;
+    const uint32_t tile = blockIdx.x, tiles = gridDim.x;
+    const uint64_t b0 = (uint64_t)tile * kNodeTile;
     const uint64_t b1 = min(b0 + (uint64_t)kNodeTile, total);
-    const uint32_t n0 = first_g[b0];
-    const uint32_t n1 = (uint32_t)min((uint64_t)first_g[b1], cap);
+    const uint32_t n0 = tile_base[tile];
+    const uint32_t n1 = (uint32_t)min((uint64_t)tile_base[tile + 1], cap);
     const uint32_t nloc = n1 > n0 ? n1 - n0 : 0u;
     const bool in_lds = nloc <= (uint32_t)kNodeLds;
     for (uint64_t j = b0 + threadIdx.x; j < b1; j += kNodeThreads) {
         s_keys[j - b0] = skeys_g[j];
         s_lo[j - b0] = lo_level[j];
+        const uint32_t f = node_local[j] + n0;
+        s_first[j - b0] = f;
+        first_node[j] = f;
     }
-    for (uint64_t j = b0 + threadIdx.x; j <= b1; j += kNodeThreads) s_first[j - b0] = first_g[j];
-    if (in_lds)
-        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) s_tag[k] = kTagNone;
     if (threadIdx.x == 0) {
-        // segment bits differ across segments, so these never match a prefix
-        // of another segment
-        s_prev = b0 > 0 ? skeys_g[b0 - 1] : ~0ull;
-        s_next = b1 < total ? skeys_g[b1] : ~0ull;
+        s_first[b1 - b0] = tile_base[tile + 1];
+        s_prev = b0 > 0 ? skeys_g[b0 - 1] : 0ull;  // read only when the tile does not start a segment
+        s_next = b1 < total ? skeys_g[b1] : 0ull;  // read only when the segment continues past b1
         s_maxl = 0;
         s_topl = 0;
     }
+    if (in_lds)
+        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) s_tag[k] = kTagNone;
     __syncthreads();
-    const KeyWindow skeys{skeys_g, s_keys, b0, b1};
-    const U32Window first_node{first_g, s_first, b0, b1 + 1};
     const uint64_t dmask = (1u << DIM) - 1u;
-    const uint64_t tile = b0 / (uint64_t)kNodeTile;
     for (uint64_t i = b0 + threadIdx.x; i < b1; i += kNodeThreads) {
         const uint64_t key = s_keys[i - b0];
         const uint32_t s = seg_of_key(key, segbits);
@@ -701,8 +767,19 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 break;
             }
             const bool leaf = L == hi;
-            const uint64_t last = leaf ? i : range_end<DIM>(skeys, i, end, lmax, L);
-            const bool spans = !leaf && last >= b1;
+            uint64_t last = i;
+            bool spans = false;
+            if (!leaf) {
+                const uint64_t bound = min(end, b1);
+                last = b0 + range_end_lds<DIM>(s_keys, (uint32_t)(i - b0), (uint32_t)(bound - b0),
+                                               lmax, L);
+                if (last + 1 == b1 && b1 < end &&
+                    key_prefix<DIM>(s_next, lmax, L) == key_prefix<DIM>(key, lmax, L)) {
+                    spans = true;
+                    const uint32_t nx = lvl_first[(uint64_t)(tile + 1) * kTopSlots + L];
+                    last = (nx == ~0u ? total : (uint64_t)nx) - 1;
+                }
+            }
             NodeHot h;
             h.cx = cx; h.cy = cy; h.cz = cz;
             if (leaf) {
@@ -718,7 +795,13 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 h.mx = h.my = h.mz = 0.0;
                 h.mass = 0.0;
             }
-            h.skip = first_node[last + 1];
+            const uint64_t nxt = last + 1;
+            if (!spans)
+                h.skip = s_first[nxt - b0];
+            else if (nxt >= total)
+                h.skip = tile_base[tiles];
+            else
+                h.skip = node_local[nxt] + tile_base[nxt / kNodeTile];
             h.level = (uint16_t)L;
             h.octant = L > 0 ? (uint8_t)((key >> (DIM * (lmax - L))) & dmask) : 0;
             h.pad = 0;
@@ -734,7 +817,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 }
             }
             if (spans && L < kTopSlots) {
-                top_slot[tile * kTopSlots + L] = gen | (uint64_t)nd;
+                top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
                 atomicMax(&s_topl, (uint32_t)L + 1u);
             }
             if (L > 0) {
@@ -742,13 +825,17 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 // tile. The parent is node nd-1 owned by body i when k > 0;
                 // for k == 0 it starts at an earlier body (body i-1 shares
                 // L-1 levels with i), in this tile unless the body before
-                // the tile shares its prefix, and then it spans for sure.
+                // the tile shares its prefix: then it spans for sure, and it
+                // starts at the last earlier body that starts a node at
+                // level L-1 or above.
                 const uint64_t P = key_prefix<DIM>(key, lmax, L - 1);
-                const bool p_before = k == 0 && b0 > d.off && key_prefix<DIM>(s_prev, lmax, L - 1) == P;
+                const bool p_before =
+                    k == 0 && b0 > d.off && key_prefix<DIM>(s_prev, lmax, L - 1) == P;
                 uint64_t pt = tile;
                 bool pspans;
                 if (p_before) {
-                    pt = range_begin<DIM>(skeys, i, d.off, lmax, L - 1) / (uint64_t)kNodeTile;
+                    const uint32_t a1 = lvl_last[(uint64_t)(tile - 1) * kTopSlots + (L - 1)];
+                    pt = a1 ? (uint64_t)(a1 - 1u) / kNodeTile : (uint64_t)tile;
                     pspans = true;
                 } else {
                     pspans = b1 < end && key_prefix<DIM>(s_next, lmax, L - 1) == P;
