@@ -14,6 +14,11 @@ namespace pg {
 __device__ __forceinline__ double bits_f64(uint64_t b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ uint64_t f64_bits(double d) { return (uint64_t)__double_as_longlong(d); }
 
+// The reference's `extent / 2.0` (octree.rs:97-100). Halving by a multiply
+// rounds exactly as the IEEE division does (scaling by a power of two, also
+// into the subnormal range), without relying on the compiler to see that.
+__device__ __forceinline__ double half(double x) { return x * 0.5; }
+
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
@@ -267,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(const SegDesc* __restrict__ des
             for (int l = 0; l < lmax; ++l) {
                 uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
                 if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
-                const double h2 = ext / 2.0;
+                const double h2 = half(ext);
                 cx = (o & 1u) ? cx + h2 : cx - h2;
                 cy = (o & 2u) ? cy + h2 : cy - h2;
                 if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
@@ -752,7 +757,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
         double cx = 0.0, cy = 0.0, cz = 0.0;
         for (int L = 0; L < lo; ++L) {
             const uint32_t o = (uint32_t)((key >> (DIM * (lmax - 1 - L))) & dmask);
-            const double h2 = ext / 2.0;
+            const double h2 = half(ext);
             cx = (o & 1u) ? cx + h2 : cx - h2;
             cy = (o & 2u) ? cy + h2 : cy - h2;
             if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
@@ -846,7 +851,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             }
             if (k + 1 < cnt) {
                 const uint32_t o = (uint32_t)((key >> (DIM * (lmax - 1 - L))) & dmask);
-                const double h2 = ext / 2.0;
+                const double h2 = half(ext);
                 cx = (o & 1u) ? cx + h2 : cx - h2;
                 cy = (o & 2u) ? cy + h2 : cy - h2;
                 if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
