@@ -70,6 +70,231 @@ def cpu_baseline(n_cube, theta, e, dt, budget_s):
     }
 
 
+def _pct(xs, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(xs, float), q))
+
+
+def served_workload(P, args):
+    """Served requests with Poisson arrivals at args.rate per second. Request
+    `id` is seeded by id: a cube (spin 1000) or a virial Plummer sphere
+    (a 0.1), 50/50; N log-uniform in [4096, 262144]; steps uniform in
+    [100, 5000]; theta in {0.5, 0.7, 1.5}; e 0.01; dt 1e-5; 200 GB budget.
+    The served path runs them with continuous batching on one engine; the
+    plain path runs each alone, first come first served, at the same arrival
+    times (its latencies follow from the measured solo run times)."""
+    import numpy as np
+
+    reqs, t = [], 0.0
+    arrivals = np.random.default_rng(12345)
+    for rid in range(args.requests):
+        rng = np.random.default_rng(rid)
+        n = int(round(np.exp(np.log(4096) + rng.random() * (np.log(262144) - np.log(4096)))))
+        steps = int(rng.integers(100, 5001))
+        theta = float(rng.choice([0.5, 0.7, 1.5]))
+        if rng.random() < 0.5:
+            state, what = P.gen_cube(n, seed=rid, spin=1000.0), "cube"
+        else:
+            state, what = P.gen_plummer_virial(n, seed=rid, mass=1.0, a=0.1), "plummer"
+        t += float(arrivals.exponential(1.0 / args.rate))
+        reqs.append(dict(n=n, steps=steps, theta=theta, state=state, what=what, arrival=t))
+    kw = dict(kind="astro2", e=0.01, dt=1e-5, precision="mixed")
+    work = sum(r["n"] * r["steps"] for r in reqs)
+    eng = P.Engine(0, budget=200 << 30)
+
+    # plain: each request alone
+    solo = []
+    for r in reqs:
+        sim = P.Sim(eng, r["state"], theta=r["theta"], **kw)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        sim.step(r["steps"])
+        eng.synchronize()
+        solo.append(time.perf_counter() - t0)
+        sim.close()
+    end, plain_lat, plain_end = 0.0, [], []
+    for r, s in zip(reqs, solo):
+        end = max(r["arrival"], end) + s
+        plain_end.append(end)
+        plain_lat.append(end - r["arrival"])
+
+    print(f"served workload: plain path done ({sum(solo):.1f} s busy)", file=sys.stderr, flush=True)
+    # served: real-time arrivals into the continuous-batching server
+    srv = P.Server(eng)
+    c0, w0 = srv.clock(), time.perf_counter()
+    ids, k, active, queued = [], 0, 0, 0
+    while k < len(reqs) or active or queued:
+        now = time.perf_counter() - w0
+        while k < len(reqs) and reqs[k]["arrival"] <= now:
+            r = reqs[k]
+            ids.append(srv.submit(r["state"], r["steps"], theta=r["theta"], **kw))
+            k += 1
+        if not active and not queued and k < len(reqs):
+            time.sleep(max(0.0, reqs[k]["arrival"] - (time.perf_counter() - w0)))
+            continue
+        active, queued = srv.tick(args.tick_steps)
+    served_lat, served_end = [], []
+    for r, rid in zip(reqs, ids):
+        info, ok = srv.info(rid)
+        assert ok and info.status == 0, info.status
+        served_end.append(info.complete_s - c0)
+        served_lat.append(info.complete_s - c0 - r["arrival"])
+    srv.close()
+    first = reqs[0]["arrival"]
+    out = {
+        "metric": "served completion latency p50/p99 and aggregate body-steps/s, served vs plain GPU",
+        "unit": "s, body-steps/s",
+        "n_gpus": 1,
+        "config": {"workload": f"{len(reqs)} requests, Poisson arrivals {args.rate}/s; cube or virial "
+                               "Plummer 50/50 by request id; N log-uniform [4096, 262144]; steps "
+                               "uniform [100, 5000]; theta {0.5, 0.7, 1.5}; e 0.01; dt 1e-5; mixed; "
+                               f"server ticks of {args.tick_steps} co-batched steps",
+                   "requests": len(reqs), "body_steps": work},
+        "served": {"p50_s": _pct(served_lat, 50), "p99_s": _pct(served_lat, 99),
+                   "body_steps_per_s": work / (max(served_end) - first)},
+        "plain": {"p50_s": _pct(plain_lat, 50), "p99_s": _pct(plain_lat, 99),
+                  "body_steps_per_s": work / (max(plain_end) - first),
+                  "busy_body_steps_per_s": work / sum(solo)},
+    }
+    out["served_over_plain_throughput"] = (out["served"]["body_steps_per_s"]
+                                           / out["plain"]["body_steps_per_s"])
+    eng.close()
+    return out
+
+
+def ensembles_workload(P, args):
+    """512 independent virial Plummer spheres (a 0.1, M 1), N_i log-uniform in
+    [1024, 65536] from seed i, theta 0.7, e 0.01, dt 1e-4, advanced together
+    (one launch sequence per step for all, graph-captured) against each
+    stepped alone (solo rate measured over a sample of steps per ensemble)."""
+    import numpy as np
+
+    ns = [int(round(np.exp(np.log(1024) + np.random.default_rng(i).random() * np.log(64))))
+          for i in range(args.ensembles)]
+    eng = P.Engine(0)
+    kw = dict(kind="astro2", theta=0.7, e=0.01, dt=1e-4, precision="mixed")
+    sims = [P.Sim(eng, P.gen_plummer_virial(n, seed=i, mass=1.0, a=0.1), **kw)
+            for i, n in enumerate(ns)]
+    total = sum(ns)
+    eng.batch_step(sims, 2)
+    eng.synchronize()
+    print(f"ensembles: {len(ns)} simulations, {total} bodies uploaded", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    eng.batch_step(sims, args.ensemble_steps)
+    eng.synchronize()
+    batched = time.perf_counter() - t0
+    solo_time = 0.0
+    sample = 10
+    for sim, n in zip(sims, ns):
+        sim.step(2)
+        eng.synchronize()
+        t1 = time.perf_counter()
+        sim.step(sample)
+        eng.synchronize()
+        solo_time += (time.perf_counter() - t1) / sample * args.ensemble_steps
+    out = {
+        "metric": "batched ensembles: aggregate body-steps/s, batched vs each alone",
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "config": {"workload": f"{len(ns)} virial Plummer spheres (a 0.1), N log-uniform "
+                               "[1024, 65536] from seed i; astro2 theta 0.7 e 0.01; verlet dt 1e-4; mixed",
+                   "bodies": total, "steps": args.ensemble_steps},
+        "batched_body_steps_per_s": total * args.ensemble_steps / batched,
+        "alone_body_steps_per_s": total * args.ensemble_steps / solo_time,
+        "alone_sample": f"each ensemble alone for {sample} steps after 2, scaled to {args.ensemble_steps}",
+    }
+    out["batched_over_alone"] = out["batched_body_steps_per_s"] / out["alone_body_steps_per_s"]
+    for s in sims:
+        s.close()
+    eng.close()
+    return out
+
+
+def precision_workload(P, args):
+    """Two colliding virial Plummer spheres (mass ratio 10:1, a 0.1, centres
+    1.0 apart along x, closing speed 0.5, seeds 7 and 8), theta 0.5, e 1e-3,
+    dt 1e-5: relative energy (force-consistent softened potential) and
+    momentum drift of mixed precision and of FP64 mode over the same steps."""
+    import numpy as np
+
+    n = args.precision_bodies
+    n1 = int(round(n * 10 / 11))
+    n2 = n - n1
+    m1, m2 = 1.0, 0.1
+    v1, v2 = 0.5 * m2 / (m1 + m2), -0.5 * m1 / (m1 + m2)
+    state = P.concat([
+        P.gen_plummer_virial(n1, seed=7, mass=m1, a=0.1, centre=(-0.5, 0.0, 0.0), v=(v1, 0.0, 0.0)),
+        P.gen_plummer_virial(n2, seed=8, mass=m2, a=0.1, centre=(0.5, 0.0, 0.0), v=(v2, 0.0, 0.0)),
+    ])
+    eng = P.Engine(0)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        sim = P.Sim(eng, state, kind="astro2", theta=0.5, e=1e-3, dt=1e-5, precision=prec)
+        E0 = sim.energy()
+        pscale = float(np.sum(state["mass"] * np.sqrt(state["vx"] ** 2 + state["vy"] ** 2
+                                                      + state["vz"] ** 2)))
+        drift, elapsed = [], 0.0
+        chunk = max(1, args.precision_steps // 4)
+        done = 0
+        while done < args.precision_steps:
+            k = min(chunk, args.precision_steps - done)
+            eng.synchronize()
+            t0 = time.perf_counter()
+            sim.step(k)
+            eng.synchronize()
+            elapsed += time.perf_counter() - t0
+            done += k
+            E = sim.energy()
+            drift.append((E[2] - E0[2]) / abs(E0[2]))
+            print(f"precision {prec}: {done} steps", file=sys.stderr, flush=True)
+        res[prec] = {"energy_drift": drift[-1], "max_abs_energy_drift": float(np.max(np.abs(drift))),
+                     "momentum_drift": float(np.max(np.abs(E[3:] - E0[3:])) / pscale),
+                     "ms_per_step": 1e3 * elapsed / args.precision_steps,
+                     "body_steps_per_s": n * args.precision_steps / elapsed}
+        sim.close()
+    eng.close()
+    return {
+        "metric": "relative energy and momentum drift, mixed precision vs FP64",
+        "n_gpus": 1,
+        "config": {"workload": f"two virial Plummer spheres, {n1} + {n2} bodies (mass 1.0 : 0.1, "
+                               "a 0.1), centres 1.0 apart on x, closing speed 0.5, seeds 7/8; "
+                               "astro2 theta 0.5 e 1e-3; verlet dt 1e-5",
+                   "bodies": n, "steps": args.precision_steps},
+        "mixed": res["mixed"], "fp64": res["fp64"],
+        "energy_drift_difference": abs(res["mixed"]["energy_drift"] - res["fp64"]["energy_drift"]),
+    }
+
+
+def direct_workload(P, args):
+    """simple_astro direct sum over the cube n=131072 + 2 stars (131,074
+    bodies), FP32 (mixed) and FP64: kernel time per evaluation from HIP
+    events, counted at 20 flops per pair interaction, N(N-1) interactions,
+    against the MI355X vector peaks."""
+    state = P.cube_and_stars(131072, seed=1, spin=1000.0)
+    n = len(state)
+    eng = P.Engine(0)
+    eng.set_graphs(False)
+    out = {"metric": "direct-sum FLOP/s (20 flops per interaction) vs VALU peak", "n_gpus": 1,
+           "config": {"workload": "cube n=131072 seed=1 spin=1000 + 2 stars; simple_astro e 0.5",
+                      "bodies": n}}
+    for prec, peak in (("mixed", 157.3), ("fp64", 78.6)):
+        sim = P.Sim(eng, state, kind="simple_astro", theta=1.0, e=0.5, dt=1e-5, precision=prec)
+        sim.step(1)
+        eng.profile_reset()
+        eng.set_profiling(True)
+        sim.step(5)
+        eng.set_profiling(False)
+        ms, calls = eng.profile()["direct"]
+        avg = ms / calls * 1e-3
+        tflops = 20.0 * n * (n - 1) / avg / 1e12
+        out[prec] = {"kernel_ms": avg * 1e3, "interactions_per_s": n * (n - 1) / avg,
+                     "tflops": tflops, "peak_tflops": peak, "frac": tflops / peak}
+        sim.close()
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +308,17 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel eagerly")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the streaming-copy bandwidth probe on BYTES and exit")
+    ap.add_argument("--workload", default="cube", choices=["cube", "served", "ensembles",
+                                                           "precision", "direct"],
+                    help="cube: the 1M-body throughput line (default); the others print "
+                         "their own JSON line")
+    ap.add_argument("--requests", type=int, default=32)
+    ap.add_argument("--rate", type=float, default=4.0, help="served arrivals per second")
+    ap.add_argument("--tick-steps", type=int, default=50)
+    ap.add_argument("--ensembles", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--precision-bodies", type=int, default=1 << 18)
+    ap.add_argument("--precision-steps", type=int, default=1000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -96,6 +332,11 @@ def main():
         eng = P.Engine(0)
         gbps = P._lib.copy_probe(eng, args.copy_probe, reps=10)
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "read_plus_write_GBps": gbps}))
+        return
+    if args.workload != "cube":
+        fn = {"served": served_workload, "ensembles": ensembles_workload,
+              "precision": precision_workload, "direct": direct_workload}[args.workload]
+        print(json.dumps(fn(P, args)), flush=True)
         return
 
     theta, e, dt = 1.5, 0.5, 1e-5

@@ -81,6 +81,7 @@ struct pg_engine {
     uint32_t* tile_base = nullptr;   // first node of each tile (+ total)
     uint32_t* lvl_last = nullptr;    // per tile and level: last node-starting body (+1), scanned
     uint32_t* lvl_first = nullptr;   // per tile and level: first node-starting body, scanned
+    uint64_t* k2 = nullptr;          // second key words of bodies whose keys tie
     NodeHot* hot = nullptr;
     NodeCold* cold = nullptr;
     double* acc_out = nullptr;
@@ -167,9 +168,12 @@ static uint64_t state_bytes(uint64_t n) {
            align256(n * 8) + 256;
 }
 
+// Nodes of a tree over n bodies with keys of 2 * lmax levels (a second key
+// word for bodies whose first words tie): n leaves plus the total ascent of
+// the common-prefix sequence, at most half its total variation.
 static uint64_t nodes_bound(uint64_t n, int dim, int lmax) {
     (void)dim;
-    return n + (n * (uint64_t)lmax + 1) / 2 + 1;
+    return n + (n * (uint64_t)(2 * lmax) + 1) / 2 + 1;
 }
 
 extern "C" uint64_t pg_sim_bytes(uint64_t n, int32_t kind) {
@@ -353,6 +357,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_tb = take((scan_tiles + 2) * 4);
     const uint64_t o_ll = take((scan_tiles + 2) * kTopSlots * 4);
     const uint64_t o_lf = take((scan_tiles + 2) * kTopSlots * 4);
+    const uint64_t o_k2 = take(bodies * 8);
     const uint64_t o_hot = take(nodes * sizeof(NodeHot)), o_cold = take(nodes * sizeof(NodeCold));
     const uint64_t o_acc = take(bodies * 24);
     const uint64_t o_d0 = take(segs * sizeof(SegDesc)), o_d1 = take(segs * sizeof(SegDesc));
@@ -392,6 +397,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->tile_base = (uint32_t*)(b + o_tb);
     e->lvl_last = (uint32_t*)(b + o_ll);
     e->lvl_first = (uint32_t*)(b + o_lf);
+    e->k2 = (uint64_t*)(b + o_k2);
     e->hot = (NodeHot*)(b + o_hot);
     e->cold = (NodeCold*)(b + o_cold);
     e->acc_out = (double*)(b + o_acc);
@@ -518,12 +524,16 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         });
     }
     const uint64_t* skeys = e->keys[bp.passes & 1];
-    const uint32_t* svals = e->vals[bp.passes & 1];
+    uint32_t* svals = e->vals[bp.passes & 1];
+    launch(e, cap, "ties", [&] {
+        k_ties<DIM><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(desc, total, skeys, svals, e->k2,
+                                                                   bp.nseg, bp.segbits, bp.lmax);
+    });
     const unsigned scan_tiles = (unsigned)((total + kScanTile - 1) / kScanTile);
     launch(e, cap, "scan_gather", [&] {
         k_scan_gather<DIM><<<scan_tiles, kBlock, 0, st>>>(
             desc, total, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_tot,
-            e->lo_level, e->lvl_last, e->lvl_first, e->err_global);
+            e->lo_level, e->lvl_last, e->lvl_first, e->k2, e->err_global);
     });
     launch(e, cap, "tree_scan", [&] {
         k_tree_scan<<<1 + kTopSlots, kScanThreads, 0, st>>>(e->tile_tot, e->tile_base, scan_tiles,
@@ -534,7 +544,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         k_nodes<DIM><<<scan_tiles, kNodeThreads, 0, st>>>(
             desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_base,
             e->lvl_last, e->lvl_first, e->first_node, e->lo_level, e->hot, e->cold, e->cap_nodes,
-            e->top_slot, e->top_kids, e->sc);
+            e->top_slot, e->top_kids, e->k2, e->sc);
     });
     const unsigned com_tiles = (unsigned)((total + kComTile - 1) / kComTile);
     launch(e, cap, "com_top", [&] {
@@ -1053,7 +1063,7 @@ extern "C" int pg_sim_stats(pg_sim* s, pg_stats* out) {
     out->interactions = st.interactions;
     out->max_level = st.max_level;
     out->error = map_err(st.err);
-    out->key_levels = (uint32_t)key_lmax(key_dim(s->p.kind), 0);
+    out->key_levels = 2u * (uint32_t)key_lmax(key_dim(s->p.kind), 0);
     if (e->last_sim == s) {
         uint32_t nn = 0;
         HIP_TRY(hipMemcpy(&nn, e->first_node + e->last_total, 4, hipMemcpyDeviceToHost));
@@ -1098,20 +1108,16 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
     if (!rec && !fl) return PG_OK;
     std::vector<NodeHot> hot(nn);
     std::vector<NodeCold> cold(nn);
-    std::vector<uint64_t> keys(n);
     std::vector<uint32_t> id(n);
     HIP_TRY(hipMemcpy(hot.data(), e->hot, nn * sizeof(NodeHot), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cold.data(), e->cold, nn * sizeof(NodeCold), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(keys.data(), e->keys[e->last_passes & 1], n * 8, hipMemcpyDeviceToHost));
     // The permuted ids live in the buffer the build wrote: the current one.
     HIP_TRY(hipMemcpy(id.data(), s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
-    const int dim = e->last_dim, lmax = e->last_lmax;
     for (uint64_t k = 0; k < nn && k < cap; ++k) {
         const NodeHot& h = hot[k];
         const NodeCold& c = cold[k];
         const int L = (int)h.level;
-        int oct = -1;
-        if (L > 0) oct = (int)((keys[c.first] >> (dim * (lmax - L))) & ((1u << dim) - 1u));
+        const int oct = L > 0 ? (int)h.octant : -1;
         const bool leaf = h.skip == k + 1;
         if (rec) {
             rec[4 * k + 0] = L;

@@ -69,6 +69,33 @@ __device__ __forceinline__ int key_lcp(uint64_t a, uint64_t b, int lmax) {
     return lz / DIM;
 }
 
+// Second key word. Bodies whose keys tie (they share every one of the lmax
+// levels a key holds) carry k2: the next lmax levels of the same descent,
+// level lmax + t in bits [DIM*(lmax-1-t), DIM*(lmax-t)). The tree then holds
+// up to 2*lmax levels; k2 is defined for tied bodies only.
+template <int DIM>
+__device__ __forceinline__ uint64_t k2_prefix(uint64_t k2, int lmax, int level) {
+    const int sh = DIM * (2 * lmax - level);
+    return sh >= 64 ? 0ull : k2 >> sh;
+}
+
+// Octant digit taken at tree depth d (from level d to level d + 1).
+template <int DIM>
+__device__ __forceinline__ uint32_t key_digit(uint64_t key, uint64_t k2, int lmax, int d) {
+    const uint64_t dmask = (1u << DIM) - 1u;
+    return d < lmax ? (uint32_t)((key >> (DIM * (lmax - 1 - d))) & dmask)
+                    : (uint32_t)((k2 >> (DIM * (2 * lmax - 1 - d))) & dmask);
+}
+
+// Common prefix length in levels of bodies ia and ib (keys a, b), reading
+// the second words only when the first ones are equal.
+template <int DIM>
+__device__ __forceinline__ int key_lcp2(uint64_t a, uint64_t b, const uint64_t* __restrict__ k2,
+                                        uint64_t ia, uint64_t ib, int lmax) {
+    const int l = key_lcp<DIM>(a, b, lmax);
+    return l < lmax ? l : lmax + key_lcp<DIM>(k2[ia], k2[ib], lmax);
+}
+
 // Exclusive scan of one value per thread over a block of NT threads
 // (s_wave holds NT / 64 values).
 template <int NT = kBlock>
@@ -435,6 +462,67 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
     }
 }
 
+// The reference's tree keeps splitting bodies that share every level a key
+// holds (octree.rs:92-125, down to depth 64). After the sort, each run of
+// equal keys (in practice two or three bodies of a dense core) gets the
+// next lmax levels of the same descent as second key words, and is put in
+// their order (insertion sort by the run's first thread). Equal second
+// words, or a run longer than kMaxTieRun, are a depth error.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ desc, uint64_t total,
+                                                 const uint64_t* __restrict__ skeys,
+                                                 uint32_t* __restrict__ svals,
+                                                 uint64_t* __restrict__ k2, uint32_t nseg,
+                                                 int segbits, int lmax) {
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g + 1 >= total) return;
+    const uint64_t key = skeys[g];
+    if (skeys[g + 1] != key || (g > 0 && skeys[g - 1] == key)) return;  // run heads only
+    const uint32_t s = seg_of_key(key, segbits);
+    if (s >= nseg) return;
+    const SegDesc& d = desc[s];
+    uint64_t r = g + 1;
+    while (r + 1 < total && skeys[r + 1] == key && r + 1 - g < (uint64_t)kMaxTieRun) ++r;
+    if (r + 1 < total && skeys[r + 1] == key) {
+        atomicOr(&d.st->err, kErrDepth);
+        return;
+    }
+    const double E = bits_f64(d.st->extent_bits);
+    for (uint64_t j = g; j <= r; ++j) {
+        const uint64_t src = (uint64_t)svals[j] - d.off;
+        if (src >= d.n) {
+            atomicOr(&d.st->err, kErrCapacity);
+            return;
+        }
+        const double px = d.sx[src], py = d.sy[src], pz = d.sz[src];
+        double ext = E, cx = 0.0, cy = 0.0, cz = 0.0;
+        uint64_t kk = 0;
+        for (int l = 0; l < 2 * lmax; ++l) {
+            uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
+            if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
+            const double h2 = half(ext);
+            cx = (o & 1u) ? cx + h2 : cx - h2;
+            cy = (o & 2u) ? cy + h2 : cy - h2;
+            if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+            ext = h2;
+            if (l >= lmax) kk = (kk << DIM) | o;
+        }
+        k2[j] = kk;
+    }
+    for (uint64_t j = g + 1; j <= r; ++j) {
+        const uint64_t kv = k2[j];
+        const uint32_t vv = svals[j];
+        uint64_t m = j;
+        while (m > g && k2[m - 1] > kv) {
+            k2[m] = k2[m - 1];
+            svals[m] = svals[m - 1];
+            --m;
+        }
+        k2[m] = kv;
+        svals[m] = vv;
+    }
+}
+
 // Per sorted body: its common-prefix lengths with its neighbours give the
 // tree levels it owns (lo..hi, hi = its leaf; lo = 1 + common prefix with the
 // previous body, 0 at a segment start); the exclusive scan of the per-body
@@ -456,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
     uint32_t* __restrict__ node_local, uint32_t* __restrict__ tile_tot,
     uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_last,
-    uint32_t* __restrict__ lvl_first, uint32_t* err) {
+    uint32_t* __restrict__ lvl_first, const uint64_t* __restrict__ k2, uint32_t* err) {
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_cnt[kScanTile];
     __shared__ uint32_t s_lastlo[kTopSlots], s_firstlo[kTopSlots];
@@ -485,13 +573,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
             } else {
                 const SegDesc& d = desc[s];
                 const uint64_t end = d.off + d.n;
-                int lcpL = i > d.off ? key_lcp<DIM>(skeys[i - 1], key, lmax) : -1;
-                int lcpR = i + 1 < end ? key_lcp<DIM>(key, skeys[i + 1], lmax) : -1;
-                if (lcpR >= lmax) {
+                const int depth = 2 * lmax;
+                int lcpL = i > d.off ? key_lcp2<DIM>(skeys[i - 1], key, k2, i - 1, i, lmax) : -1;
+                int lcpR = i + 1 < end ? key_lcp2<DIM>(key, skeys[i + 1], k2, i, i + 1, lmax) : -1;
+                if (lcpR >= depth) {
                     atomicOr(&d.st->err, kErrDepth);
-                    lcpR = lmax - 1;
+                    lcpR = depth - 1;
                 }
-                if (lcpL >= lmax) lcpL = lmax - 1;
+                if (lcpL >= depth) lcpL = depth - 1;
                 c = 1u + (uint32_t)max(0, lcpR - lcpL);
                 lo = lcpL + 1;
                 const uint64_t j = i - d.off;
@@ -707,7 +796,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     const uint32_t* __restrict__ lvl_first, uint32_t* __restrict__ first_node,
     const uint8_t* __restrict__ lo_level, NodeHot* __restrict__ hot, NodeCold* __restrict__ cold,
     uint64_t cap, uint64_t* __restrict__ top_slot, uint64_t* __restrict__ top_kids,
-    StepCounters* __restrict__ sc) {
+    const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc) {
     __shared__ uint64_t s_keys[kNodeTile];
     __shared__ uint32_t s_first[kNodeTile + 1];
     __shared__ uint8_t s_lo[kNodeTile];
@@ -741,7 +830,6 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     if (in_lds)
         for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) s_tag[k] = kTagNone;
     __syncthreads();
-    const uint64_t dmask = (1u << DIM) - 1u;
     for (uint64_t i = b0 + threadIdx.x; i < b1; i += kNodeThreads) {
         const uint64_t key = s_keys[i - b0];
         const uint32_t s = seg_of_key(key, segbits);
@@ -753,10 +841,18 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
         const int cnt = (int)(s_first[i - b0 + 1] - nbase);
         const int hi = lo + cnt - 1;
         // centre of the first owned node: the reference's descent, lo levels down
+        // levels past lmax exist only for bodies whose keys tie; they read
+        // the second key word
+        const uint64_t kx = hi > lmax ? k2[i] : 0ull;
+        // Do body j (key kj) and body i share `level` levels?
+        auto same_at = [&](uint64_t kj, uint64_t j, int level) -> bool {
+            if (level <= lmax) return key_prefix<DIM>(kj, lmax, level) == key_prefix<DIM>(key, lmax, level);
+            return kj == key && k2_prefix<DIM>(k2[j], lmax, level) == k2_prefix<DIM>(kx, lmax, level);
+        };
         double ext = bits_f64(d.st->extent_bits);
         double cx = 0.0, cy = 0.0, cz = 0.0;
         for (int L = 0; L < lo; ++L) {
-            const uint32_t o = (uint32_t)((key >> (DIM * (lmax - 1 - L))) & dmask);
+            const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
             const double h2 = half(ext);
             cx = (o & 1u) ? cx + h2 : cx - h2;
             cy = (o & 2u) ? cy + h2 : cy - h2;
@@ -776,10 +872,14 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             bool spans = false;
             if (!leaf) {
                 const uint64_t bound = min(end, b1);
-                last = b0 + range_end_lds<DIM>(s_keys, (uint32_t)(i - b0), (uint32_t)(bound - b0),
-                                               lmax, L);
-                if (last + 1 == b1 && b1 < end &&
-                    key_prefix<DIM>(s_next, lmax, L) == key_prefix<DIM>(key, lmax, L)) {
+                if (L <= lmax) {
+                    last = b0 + range_end_lds<DIM>(s_keys, (uint32_t)(i - b0),
+                                                   (uint32_t)(bound - b0), lmax, L);
+                } else {
+                    // inside a run of equal keys (a few bodies): linear
+                    while (last + 1 < bound && same_at(s_keys[last + 1 - b0], last + 1, L)) ++last;
+                }
+                if (last + 1 == b1 && b1 < end && same_at(s_next, b1, L)) {
                     spans = true;
                     const uint32_t nx = lvl_first[(uint64_t)(tile + 1) * kTopSlots + L];
                     last = (nx == ~0u ? total : (uint64_t)nx) - 1;
@@ -808,7 +908,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             else
                 h.skip = node_local[nxt] + tile_base[nxt / kNodeTile];
             h.level = (uint16_t)L;
-            h.octant = L > 0 ? (uint8_t)((key >> (DIM * (lmax - L))) & dmask) : 0;
+            h.octant = L > 0 ? (uint8_t)key_digit<DIM>(key, kx, lmax, L - 1) : 0;
             h.pad = 0;
             hot[nd] = h;
             cold[nd] = NodeCold{(uint32_t)i, (uint32_t)last};
@@ -833,9 +933,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 // the tile shares its prefix: then it spans for sure, and it
                 // starts at the last earlier body that starts a node at
                 // level L-1 or above.
-                const uint64_t P = key_prefix<DIM>(key, lmax, L - 1);
-                const bool p_before =
-                    k == 0 && b0 > d.off && key_prefix<DIM>(s_prev, lmax, L - 1) == P;
+                const bool p_before = k == 0 && b0 > d.off && same_at(s_prev, b0 - 1, L - 1);
                 uint64_t pt = tile;
                 bool pspans;
                 if (p_before) {
@@ -843,14 +941,14 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                     pt = a1 ? (uint64_t)(a1 - 1u) / kNodeTile : (uint64_t)tile;
                     pspans = true;
                 } else {
-                    pspans = b1 < end && key_prefix<DIM>(s_next, lmax, L - 1) == P;
+                    pspans = b1 < end && same_at(s_next, b1, L - 1);
                 }
                 if (pspans)
                     top_kids[(pt * kTopSlots + (uint64_t)(L - 1)) * 8 + h.octant] =
                         gen | (uint64_t)nd;
             }
             if (k + 1 < cnt) {
-                const uint32_t o = (uint32_t)((key >> (DIM * (lmax - 1 - L))) & dmask);
+                const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
                 const double h2 = half(ext);
                 cx = (o & 1u) ? cx + h2 : cx - h2;
                 cy = (o & 2u) ? cy + h2 : cy - h2;

@@ -94,6 +94,54 @@ def test_tree_topology_bit_exact(engine, kind, dim):
         np.testing.assert_array_equal(fl[leaf, 1:4], ofl[leaf, 1:4])  # leaves hold exact bodies
 
 
+def close_pairs_state(n=3000, seed=4):
+    """A cube with close pairs and a close triple inserted: separations from
+    3e-9 to 2e-7 in a unit-size tree, so their keys tie over all 21 levels
+    a 63-bit octree key holds (the reference's tree keeps splitting them,
+    octree.rs:92-125)."""
+    s = small_cube(n, seed=seed)
+    extra = []
+    offsets = [(3e-9, 0.0, 0.0), (0.0, 2e-8, -1e-8), (1e-7, 1e-7, 1e-7), (-2e-7, 5e-8, 0.0)]
+    for k, (dx, dy, dz) in enumerate(offsets):
+        b = s[[17 * k + 3]].copy()
+        b["x"] += dx
+        b["y"] += dy
+        b["z"] += dz
+        extra.append(b)
+    t = s[[101]].copy()  # a triple: two more bodies around body 101
+    t2 = s[[101]].copy()
+    t["x"] += 4e-9
+    t2["y"] -= 6e-9
+    return P.concat([s] + extra + [t, t2])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pairs_deeper_than_one_key_word(engine, precision):
+    """Bodies that share every level of the first key word: the tree goes on
+    to the levels the reference reaches, topology and centres bit-exact,
+    accelerations as for any other state."""
+    s = close_pairs_state()
+    sim = P.Sim(engine, s, kind="astro2", precision=precision, **CFG1)
+    a = sim.accel()
+    rec, fl = sim.tree()
+    st = sim.stats()
+    t = O.Tree.from_state(s, 3)
+    orec, ofl = t.serialize()
+    assert t.merges == 0 and t.fault == 0
+    assert st["max_level"] > 21 or int(rec[:, 0].max()) > 21
+    np.testing.assert_array_equal(rec[:, :3], orec[:, :3])
+    np.testing.assert_array_equal(fl[:, 4:8], ofl[:, 4:8])
+    ref, _, _ = O.bh_transform(s, CFG1["theta"], CFG1["e"])
+    r = rel_err(a, ref)
+    if precision == "fp64":
+        assert fp64_ok(r)
+    else:
+        assert np.quantile(r, 0.999) <= 1e-4
+    # and it steps like any other state
+    sim.step(3)
+    assert np.all(np.isfinite(sim.download()["x"]))
+
+
 def octant_path_key(p, extent, dim, levels):
     c = [0.0, 0.0, 0.0]
     h = extent
