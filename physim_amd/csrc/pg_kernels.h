@@ -1072,6 +1072,25 @@ struct Kahan3 {
     }
 };
 
+// The reference's opening test, `half_width / r < theta` with
+// r = sqrt(dx*dx + dy*dy + dz*dz) in FP64 (octree.rs:137-140), decided
+// exactly. An FP32 estimate hw * rsq(s) is within 3.3e-7 (relative) of the
+// exact quotient (conversions 6e-8 each, rsq 1.2e-7, product 6e-8), and the
+// FP64 quotient of the reference within 2.3e-16 of it, so outside a band of
+// 1e-6 * theta the estimate decides as the reference does. Inside the band,
+// or with operands outside FP32's normal range, the FP64 square root and
+// division decide.
+__device__ __forceinline__ bool accept_cell(double hw, double s, double theta) {
+    if (s > 1e-30 && s < 1e30 && hw > 1e-30 && hw < 1e30) {
+        const float q = (float)hw * __builtin_amdgcn_rsqf((float)s);
+        const float t = (float)theta;
+        const float band = 1e-6f * fabsf(t);
+        if (q < t - band) return true;
+        if (q > t + band) return false;
+    }
+    return hw / sqrt(s) < theta;
+}
+
 // Pair acceleration on body a from (b, mb), mixed precision: the separation
 // b - a is taken in FP64 and rounded once; |F|/m_a = mb / (s + e) along the
 // unsoftened direction (astro/src/lib.rs:84-113, with m_a cancelled).
@@ -1156,8 +1175,8 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
                 bool take = leaf;
                 if (!leaf) {
                     const double d0x = x - h.cx, d0y = y - h.cy, d0z = z - h.cz;
-                    const double r = sqrt(d0x * d0x + d0y * d0y + d0z * d0z);
-                    take = ldexp(E, -(int)h.level) / r < theta;
+                    take = accept_cell(ldexp(E, -(int)h.level),
+                                       d0x * d0x + d0y * d0y + d0z * d0z, theta);
                 }
                 if (take) {
                     ++count;
