@@ -79,7 +79,6 @@ struct pg_engine {
     uint32_t* node_local = nullptr;  // per-body node offset within its tile
     uint32_t* tile_tot = nullptr;    // nodes per tile
     uint32_t* tile_base = nullptr;   // first node of each tile (+ total)
-    uint32_t* lvl_last = nullptr;    // per tile and level: last node-starting body (+1), scanned
     uint32_t* lvl_first = nullptr;   // per tile and level: first node-starting body, scanned
     uint64_t* k2 = nullptr;          // second key words of bodies whose keys tie
     NodeHot* hot = nullptr;
@@ -93,7 +92,9 @@ struct pg_engine {
     uint32_t* ext_chunk = nullptr;
     uint64_t* ext_partial = nullptr;
     uint64_t* top_slot = nullptr;  // generation-tagged tile-spanning node per (tile, level)
-    uint64_t* top_kids = nullptr;  // its children by octant, generation-tagged
+    Sum4* out_part = nullptr;      // per tile and level: suffix sums of spanning nodes
+    Sum4* in_part = nullptr;       // per tile and level: prefix sums of spanning nodes
+    Sum4* tot_part = nullptr;      // per tile: mass sums of all its bodies
     StepCounters* sc = nullptr;
     uint32_t* err_global = nullptr;
     uint32_t cap_chunks = 0;
@@ -355,7 +356,9 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
     const uint64_t o_nl = take((bodies + 1) * 4), o_tt = take((scan_tiles + 2) * 4);
     const uint64_t o_tb = take((scan_tiles + 2) * 4);
-    const uint64_t o_ll = take((scan_tiles + 2) * kTopSlots * 4);
+    const uint64_t o_op = take((scan_tiles + 2) * kTopSlots * sizeof(Sum4));
+    const uint64_t o_ip = take((scan_tiles + 2) * kTopSlots * sizeof(Sum4));
+    const uint64_t o_tp = take((scan_tiles + 2) * sizeof(Sum4));
     const uint64_t o_lf = take((scan_tiles + 2) * kTopSlots * 4);
     const uint64_t o_k2 = take(bodies * 8);
     const uint64_t o_hot = take(nodes * sizeof(NodeHot)), o_cold = take(nodes * sizeof(NodeCold));
@@ -369,7 +372,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_es = take(chunks * 4), o_ec = take(chunks * 4), o_ep = take(chunks * 8);
     // a tile boundary is crossed by at most one node per level (levels 0..32)
     const uint64_t top = (bodies / kComTile + 2) * kTopSlots;
-    const uint64_t o_tl = take(top * 8), o_tk = take(top * 8 * 8);
+    const uint64_t o_tl = take(top * 8);
     const uint64_t total = off;
     if (e->sims_bytes + total > e->budget)
         return pg_set_error(PG_ERR_BUDGET, "scratch of %llu bytes exceeds the HBM budget",
@@ -395,7 +398,6 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->node_local = (uint32_t*)(b + o_nl);
     e->tile_tot = (uint32_t*)(b + o_tt);
     e->tile_base = (uint32_t*)(b + o_tb);
-    e->lvl_last = (uint32_t*)(b + o_ll);
     e->lvl_first = (uint32_t*)(b + o_lf);
     e->k2 = (uint64_t*)(b + o_k2);
     e->hot = (NodeHot*)(b + o_hot);
@@ -412,14 +414,15 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->ext_chunk = (uint32_t*)(b + o_ec);
     e->ext_partial = (uint64_t*)(b + o_ep);
     e->top_slot = (uint64_t*)(b + o_tl);
-    e->top_kids = (uint64_t*)(b + o_tk);
+    e->out_part = (Sum4*)(b + o_op);
+    e->in_part = (Sum4*)(b + o_ip);
+    e->tot_part = (Sum4*)(b + o_tp);
     e->cap_chunks = (uint32_t)chunks;
     e->cap_top = top;
     // Status words are versioned; zero them once so no stale tag can match.
     HIP_TRY(hipMemsetAsync(e->sc, 0, sizeof(StepCounters), e->stream));
     HIP_TRY(hipMemsetAsync(e->err_global, 0, 256, e->stream));
     HIP_TRY(hipMemsetAsync(e->top_slot, 0, top * 8, e->stream));
-    HIP_TRY(hipMemsetAsync(e->top_kids, 0, top * 64, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->cap_bodies = bodies;
     e->cap_nodes = nodes;
@@ -533,22 +536,21 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "scan_gather", [&] {
         k_scan_gather<DIM><<<scan_tiles, kBlock, 0, st>>>(
             desc, total, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_tot,
-            e->lo_level, e->lvl_last, e->lvl_first, e->k2, e->err_global);
+            e->lo_level, e->lvl_first, e->k2, e->err_global);
     });
     launch(e, cap, "tree_scan", [&] {
         k_tree_scan<<<1 + kTopSlots, kScanThreads, 0, st>>>(e->tile_tot, e->tile_base, scan_tiles,
-                                                             e->lvl_last, e->lvl_first,
-                                                             e->first_node, total);
+                                                             e->lvl_first, e->first_node, total);
     });
     launch(e, cap, "nodes", [&] {
         k_nodes<DIM><<<scan_tiles, kNodeThreads, 0, st>>>(
             desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_base,
-            e->lvl_last, e->lvl_first, e->first_node, e->lo_level, e->hot, e->cold, e->cap_nodes,
-            e->top_slot, e->top_kids, e->k2, e->sc);
+            e->lvl_first, e->first_node, e->lo_level, e->hot, e->cold, e->cap_nodes, e->top_slot,
+            e->out_part, e->in_part, e->tot_part, e->k2, e->sc);
     });
-    const unsigned com_tiles = (unsigned)((total + kComTile - 1) / kComTile);
-    launch(e, cap, "com_top", [&] {
-        k_com_top<<<1, 1024, 0, st>>>(e->hot, e->top_slot, e->top_kids, e->sc, com_tiles);
+    launch(e, cap, "com_span", [&] {
+        k_com_span<<<dim3(scan_tiles, kComSpanGroups), kBlock, 0, st>>>(
+            e->hot, e->cold, e->top_slot, e->out_part, e->in_part, e->tot_part, e->sc, total);
     });
     launch(e, cap, "walk", [&] {
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(

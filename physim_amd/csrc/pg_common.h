@@ -23,6 +23,7 @@ constexpr int kNodeThreads = 1024; // threads per node-building block
 constexpr int kNodeLds = 3584;     // nodes per tile whose centre of mass is reduced in LDS
 constexpr int kTopSlots = 65;      // tree levels 0..64: one tile-spanning node each
 constexpr int kMaxTieRun = 256;    // longest run of equal keys ordered by the second key word
+constexpr int kComSpanGroups = 5;   // level groups (of 4 waves) per tile in k_com_span
 static_assert(kScanTile == kNodeTile, "node offsets and node building share their tiles");
 static_assert(kTopSlots <= kBlock, "per-level tables are filled by one thread per level");
 

@@ -534,27 +534,23 @@ __global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ des
 //
 // Node ranges: the node at level L containing body i starts at the last body
 // j <= i with lo[j] <= L and ends before the first body j > i with lo[j] <= L.
-// Per tile and level this kernel records the last such body (+1, 0 = none)
-// and the first one (~0 = none); k_tree_scan turns them into a prefix maximum
-// and a suffix minimum over the tiles, so the node builder answers both
-// queries across tiles in one load.
+// Per tile and level this kernel records the first such body (~0 = none);
+// k_tree_scan turns that into a suffix minimum over the tiles, so the node
+// builder finds the end of a node that leaves its tile in one load.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys,
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
     uint32_t* __restrict__ node_local, uint32_t* __restrict__ tile_tot,
-    uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_last,
-    uint32_t* __restrict__ lvl_first, const uint64_t* __restrict__ k2, uint32_t* err) {
+    uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_first,
+    const uint64_t* __restrict__ k2, uint32_t* err) {
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_cnt[kScanTile];
-    __shared__ uint32_t s_lastlo[kTopSlots], s_firstlo[kTopSlots];
+    __shared__ uint32_t s_firstlo[kTopSlots];
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
     const uint64_t tbase = (uint64_t)tile * kScanTile;
-    if (tid < kTopSlots) {
-        s_lastlo[tid] = 0u;
-        s_firstlo[tid] = ~0u;
-    }
+    if (tid < kTopSlots) s_firstlo[tid] = ~0u;
     __syncthreads();
 
 #pragma unroll 2
@@ -594,7 +590,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 d.did[j] = d.sid[src];
             }
             lo_level[i] = (uint8_t)lo;
-            atomicMax(&s_lastlo[lo], (uint32_t)i + 1u);
             atomicMin(&s_firstlo[lo], (uint32_t)i);
         }
         s_cnt[li] = c;
@@ -613,12 +608,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
         run += c;
     }
     if (tid < kTopSlots) {
-        uint32_t mx = 0u, mn = ~0u;
-        for (int v = 0; v <= tid; ++v) {
-            mx = max(mx, s_lastlo[v]);
-            mn = min(mn, s_firstlo[v]);
-        }
-        lvl_last[(uint64_t)tile * kTopSlots + tid] = mx;
+        uint32_t mn = ~0u;
+        for (int v = 0; v <= tid; ++v) mn = min(mn, s_firstlo[v]);
         lvl_first[(uint64_t)tile * kTopSlots + tid] = mn;
     }
     if (tid == 0) tile_tot[tile] = tile_total;
@@ -648,12 +639,10 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s
 
 // Block 0: exclusive scan of the tile node totals into tile_base[0..tiles]
 // (tile_base[tiles] = all nodes) and first_node[total] = all nodes. Block
-// 1 + L: over the tiles, prefix maximum of lvl_last[., L] and suffix minimum
-// of lvl_first[., L], in place.
+// 1 + L: suffix minimum over the tiles of lvl_first[., L], in place.
 __global__ __launch_bounds__(kScanThreads) void k_tree_scan(
     const uint32_t* __restrict__ tile_tot, uint32_t* __restrict__ tile_base, uint32_t tiles,
-    uint32_t* __restrict__ lvl_last, uint32_t* __restrict__ lvl_first,
-    uint32_t* __restrict__ first_node, uint64_t total) {
+    uint32_t* __restrict__ lvl_first, uint32_t* __restrict__ first_node, uint64_t total) {
     __shared__ uint32_t s_wave[kScanThreads / 64];
     const uint32_t per = (tiles + kScanThreads - 1) / kScanThreads;
     const uint32_t b = threadIdx.x * per;
@@ -676,28 +665,7 @@ __global__ __launch_bounds__(kScanThreads) void k_tree_scan(
         return;
     }
     const uint32_t L = blockIdx.x - 1;
-    auto opmax = [](uint32_t a, uint32_t c) { return a > c ? a : c; };
     auto opmin = [](uint32_t a, uint32_t c) { return a < c ? a : c; };
-    // prefix maximum in tile order
-    uint32_t m = 0u;
-    for (uint32_t k = 0; k < per; ++k)
-        if (b + k < tiles) m = max(m, lvl_last[(uint64_t)(b + k) * kTopSlots + L]);
-    uint32_t run = block_inclusive_scan<kScanThreads>(m, s_wave, opmax);
-    run = __shfl_up(run, 1);
-    {
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) run = 0u;
-        for (int w = 0; w < wave; ++w) run = max(run, s_wave[w]);
-    }
-    // (s_wave still holds the per-wave maxima of the inclusive scan)
-    for (uint32_t k = 0; k < per; ++k) {
-        if (b + k < tiles) {
-            uint32_t* p = lvl_last + (uint64_t)(b + k) * kTopSlots + L;
-            run = max(run, *p);
-            *p = run;
-        }
-    }
-    __syncthreads();
     // suffix minimum: thread r walks the tiles from the end
     const uint32_t rb = threadIdx.x * per;
     uint32_t mn = ~0u;
@@ -738,6 +706,78 @@ __device__ __forceinline__ uint32_t range_end_lds(const uint64_t* s, uint32_t i,
         if (key_prefix<DIM>(s[mid], lmax, level) == P) last = mid; else hi = mid;
     }
     return last;
+}
+
+// Mass sums (m, m x, m y, m z) of body ranges, FP64.
+struct Sum4 {
+    double m, x, y, z;
+    __device__ __forceinline__ void add(const Sum4& o) {
+        m += o.m;
+        x += o.x;
+        y += o.y;
+        z += o.z;
+    }
+};
+
+__device__ __forceinline__ Sum4 shfl_dir4(const Sum4& v, int o, bool down) {
+    Sum4 r;
+    r.m = down ? __shfl_down(v.m, o) : __shfl_up(v.m, o);
+    r.x = down ? __shfl_down(v.x, o) : __shfl_up(v.x, o);
+    r.y = down ? __shfl_down(v.y, o) : __shfl_up(v.y, o);
+    r.z = down ? __shfl_down(v.z, o) : __shfl_up(v.z, o);
+    return r;
+}
+
+// Exclusive scan of one Sum4 per thread over a block of NT threads, in
+// thread order (REV: from the last thread down), and the block total. Fixed
+// association, so the result is deterministic.
+template <int NT, bool REV>
+__device__ __forceinline__ Sum4 block_excl_scan4(Sum4 v, Sum4* s_wave, Sum4* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Sum4 inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const Sum4 t = shfl_dir4(inc, o, REV);
+        if (REV ? lane + o < 64 : lane >= o) inc.add(t);
+    }
+    Sum4 ex = shfl_dir4(inc, 1, REV);
+    if (REV ? lane == 63 : lane == 0) ex = Sum4{0.0, 0.0, 0.0, 0.0};
+    if (REV ? lane == 0 : lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    Sum4 base{0.0, 0.0, 0.0, 0.0}, tot{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int w = 0; w < NT / 64; ++w) {
+        const int ww = REV ? NT / 64 - 1 - w : w;
+        const Sum4 c = s_wave[ww];
+        if (REV ? ww > wave : ww < wave) base.add(c);
+        tot.add(c);
+    }
+    __syncthreads();
+    *total = tot;
+    base.add(ex);
+    return base;
+}
+
+// Exclusive minimum of one u32 per thread over a block of NT threads, in
+// thread order or (REV) from the last thread down; ~0u for the first.
+template <int NT, bool REV>
+__device__ __forceinline__ uint32_t block_excl_min(uint32_t v, uint32_t* s_wave) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = REV ? __shfl_down(inc, o) : __shfl_up(inc, o);
+        if (REV ? lane + o < 64 : lane >= o) inc = min(inc, t);
+    }
+    uint32_t ex = REV ? __shfl_down(inc, 1) : __shfl_up(inc, 1);
+    if (REV ? lane == 63 : lane == 0) ex = ~0u;
+    if (REV ? lane == 0 : lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+#pragma unroll 1
+    for (int w = 0; w < NT / 64; ++w)
+        if (REV ? w > wave : w < wave) ex = min(ex, s_wave[w]);
+    __syncthreads();
+    return ex;
 }
 
 // Mass and centre of mass of an internal node from its children, in octant
@@ -784,27 +824,29 @@ constexpr uint8_t kTagNone = 0xFF;      // no node written (never matches a leve
 // com_from_children). A tile with more than kNodeLds nodes does the same level
 // loop on the global records.
 //
-// An internal node whose range leaves the tile is left to k_com_top: it is
-// recorded (generation-tagged) at slot tile * kTopSlots + level, a node whose
-// parent is such a node registers itself in the parent's kid slot (octant
-// order), and sc->top_levels bounds the levels k_com_top visits.
+// An internal node whose range leaves the tile is left to k_com_span: it is
+// recorded (generation-tagged) at slot tile * kTopSlots + level, and
+// sc->top_levels bounds the levels k_com_span visits. For those nodes the
+// block also writes the tile's partial mass sums (below).
 template <int DIM>
 __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys_g,
     uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ node_local,
-    const uint32_t* __restrict__ tile_base, const uint32_t* __restrict__ lvl_last,
-    const uint32_t* __restrict__ lvl_first, uint32_t* __restrict__ first_node,
-    const uint8_t* __restrict__ lo_level, NodeHot* __restrict__ hot, NodeCold* __restrict__ cold,
-    uint64_t cap, uint64_t* __restrict__ top_slot, uint64_t* __restrict__ top_kids,
-    const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc) {
+    const uint32_t* __restrict__ tile_base, const uint32_t* __restrict__ lvl_first,
+    uint32_t* __restrict__ first_node, const uint8_t* __restrict__ lo_level,
+    NodeHot* __restrict__ hot, NodeCold* __restrict__ cold, uint64_t cap,
+    uint64_t* __restrict__ top_slot, Sum4* __restrict__ out_part, Sum4* __restrict__ in_part,
+    Sum4* __restrict__ tot_part, const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc) {
     __shared__ uint64_t s_keys[kNodeTile];
     __shared__ uint32_t s_first[kNodeTile + 1];
     __shared__ uint8_t s_lo[kNodeTile];
     __shared__ double s_m[kNodeLds], s_x[kNodeLds], s_y[kNodeLds], s_z[kNodeLds];
     __shared__ uint16_t s_skip[kNodeLds];
     __shared__ uint8_t s_tag[kNodeLds];
-    __shared__ uint64_t s_prev, s_next;  // keys just before and just after the tile
+    __shared__ uint64_t s_next;  // key just after the tile
     __shared__ uint32_t s_maxl, s_topl;
+    __shared__ Sum4 s_w4[kNodeThreads / 64];
+    __shared__ uint32_t s_wmin[kNodeThreads / 64];
     const uint64_t gen = (uint64_t)sc->gen << 32;
     const uint32_t tile = blockIdx.x, tiles = gridDim.x;
     const uint64_t b0 = (uint64_t)tile * kNodeTile;
@@ -822,7 +864,6 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     }
     if (threadIdx.x == 0) {
         s_first[b1 - b0] = tile_base[tile + 1];
-        s_prev = b0 > 0 ? skeys_g[b0 - 1] : 0ull;  // read only when the tile does not start a segment
         s_next = b1 < total ? skeys_g[b1] : 0ull;  // read only when the segment continues past b1
         s_maxl = 0;
         s_topl = 0;
@@ -925,28 +966,6 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
                 atomicMax(&s_topl, (uint32_t)L + 1u);
             }
-            if (L > 0) {
-                // Kid registration when the parent (level L-1) leaves its
-                // tile. The parent is node nd-1 owned by body i when k > 0;
-                // for k == 0 it starts at an earlier body (body i-1 shares
-                // L-1 levels with i), in this tile unless the body before
-                // the tile shares its prefix: then it spans for sure, and it
-                // starts at the last earlier body that starts a node at
-                // level L-1 or above.
-                const bool p_before = k == 0 && b0 > d.off && same_at(s_prev, b0 - 1, L - 1);
-                uint64_t pt = tile;
-                bool pspans;
-                if (p_before) {
-                    const uint32_t a1 = lvl_last[(uint64_t)(tile - 1) * kTopSlots + (L - 1)];
-                    pt = a1 ? (uint64_t)(a1 - 1u) / kNodeTile : (uint64_t)tile;
-                    pspans = true;
-                } else {
-                    pspans = b1 < end && same_at(s_next, b1, L - 1);
-                }
-                if (pspans)
-                    top_kids[(pt * kTopSlots + (uint64_t)(L - 1)) * 8 + h.octant] =
-                        gen | (uint64_t)nd;
-            }
             if (k + 1 < cnt) {
                 const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
                 const double h2 = half(ext);
@@ -1011,47 +1030,136 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             __threadfence();
             __syncthreads();
         }
+        __threadfence();
+        __syncthreads();
+    }
+
+    // Partial mass sums for the tile-spanning nodes (k_com_span): the tile
+    // total; per level L, the suffix from the first body of the level-L
+    // ancestor of the tile's last body, where that ancestor starts in the
+    // tile and runs into the next one (out_part); and the prefix up to the
+    // end of the level-L ancestor of the tile's first body, where that one
+    // started before the tile and ends in it (in_part). Bodies come in
+    // contiguous pairs per thread; every sum has a fixed association.
+    const uint32_t nb = (uint32_t)(b1 - b0);
+    // mass sums and level of local body li (zero / ~0 when absent)
+    auto body_sum = [&](uint32_t li) -> Sum4 {
+        if (li >= nb) return Sum4{0.0, 0.0, 0.0, 0.0};
+        const uint32_t leaf = s_first[li + 1] - 1u;
+        if (leaf >= n1) return Sum4{0.0, 0.0, 0.0, 0.0};
+        double m, x, y, z;
+        if (in_lds) {
+            if (s_tag[leaf - n0] == kTagNone) return Sum4{0.0, 0.0, 0.0, 0.0};
+            m = s_m[leaf - n0]; x = s_x[leaf - n0]; y = s_y[leaf - n0]; z = s_z[leaf - n0];
+        } else {
+            m = hot[leaf].mass; x = hot[leaf].mx; y = hot[leaf].my; z = hot[leaf].mz;
+        }
+        return Sum4{m, m * x, m * y, m * z};
+    };
+    auto body_lo = [&](uint32_t li) -> uint32_t { return li < nb ? (uint32_t)s_lo[li] : ~0u; };
+    const uint32_t l0 = 2u * threadIdx.x, l1 = l0 + 1u;
+    const uint32_t pair_lo = min(body_lo(l0), body_lo(l1));
+    {
+        // prefix side: tile total, and in_part at the end of each ancestor
+        // of the first body (levels <= lam0) that ends in the tile
+        Sum4 pair = body_sum(l0);
+        pair.add(body_sum(l1));
+        Sum4 total4;
+        const Sum4 pre = block_excl_scan4<kNodeThreads, false>(pair, s_w4, &total4);
+        // (levels are at most 64: clamp "none" before comparing as int)
+        const int mn = (int)min(block_excl_min<kNodeThreads, false>(pair_lo, s_wmin), 255u);
+        if (threadIdx.x == 0) tot_part[tile] = total4;
+        const int lam0 = nb ? (int)s_lo[0] - 1 : -1;
+        // body li ends the level-L ancestor iff lo[li] <= L < every lo
+        // between the first body and li
+        if (l0 > 0 && l0 < nb)
+            for (int L = (int)body_lo(l0); L <= lam0 && L < mn; ++L)
+                in_part[(uint64_t)tile * kTopSlots + L] = pre;
+        if (l1 < nb) {
+            const int mn1 = min(mn, (int)body_lo(l0));
+            if ((int)body_lo(l1) <= lam0 && (int)body_lo(l1) < mn1) {
+                Sum4 p = pre;
+                p.add(body_sum(l0));
+                for (int L = (int)body_lo(l1); L <= lam0 && L < mn1; ++L)
+                    in_part[(uint64_t)tile * kTopSlots + L] = p;
+            }
+        }
+    }
+    {
+        // suffix side: out_part from the first body of each ancestor of the
+        // last body (levels <= lam1) that starts in the tile and runs on
+        Sum4 pair = body_sum(l0);
+        pair.add(body_sum(l1));
+        Sum4 total4;
+        const Sum4 suf = block_excl_scan4<kNodeThreads, true>(pair, s_w4, &total4);
+        const int mn = (int)min(block_excl_min<kNodeThreads, true>(pair_lo, s_wmin), 255u);
+        int lam1 = -1;
+        if (nb && b1 < total && seg_of_key(s_next, segbits) == seg_of_key(s_keys[nb - 1], segbits))
+            lam1 = min(key_lcp2<DIM>(s_keys[nb - 1], s_next, k2, b1 - 1, b1, lmax), kTopSlots - 1);
+        // body li starts the level-L ancestor iff lo[li] <= L < every lo
+        // after it
+        if (l1 < nb && (int)body_lo(l1) <= lam1 && (int)body_lo(l1) < mn) {
+            Sum4 sf = suf;
+            sf.add(body_sum(l1));
+            for (int L = (int)body_lo(l1); L <= lam1 && L < mn; ++L)
+                out_part[(uint64_t)tile * kTopSlots + L] = sf;
+        }
+        if (l0 < nb) {
+            const int mn0 = min(mn, (int)min(body_lo(l1), 255u));
+            if ((int)body_lo(l0) <= lam1 && (int)body_lo(l0) < mn0) {
+                Sum4 sf = suf;
+                sf.add(body_sum(l1));
+                sf.add(body_sum(l0));
+                for (int L = (int)body_lo(l0); L <= lam1 && L < mn0; ++L)
+                    out_part[(uint64_t)tile * kTopSlots + L] = sf;
+            }
+        }
     }
 }
 
-// Bottom-up centre of mass, top pass (one block): the tile-spanning nodes,
-// level by level, deepest first. Each one's kids are read from its kid slots
-// (octant order; a kid is a local node or leaf finished earlier, or a
-// spanning node of the previous level).
-__global__ __launch_bounds__(1024) void k_com_top(NodeHot* hot,
-                                                  const uint64_t* __restrict__ top_slot,
-                                                  const uint64_t* __restrict__ top_kids,
-                                                  const StepCounters* __restrict__ sc,
-                                                  uint32_t ntiles) {
+// Mass and centre of mass of every tile-spanning node (blocks: its owner
+// tile x groups of levels, one wave per level): its owner tile's suffix from
+// its first body, the totals of the tiles in between, and the prefix of the
+// tile where it ends (or that tile's total when it ends with the tile).
+__global__ __launch_bounds__(kBlock) void k_com_span(
+    NodeHot* __restrict__ hot, const NodeCold* __restrict__ cold,
+    const uint64_t* __restrict__ top_slot, const Sum4* __restrict__ out_part,
+    const Sum4* __restrict__ in_part, const Sum4* __restrict__ tot_part,
+    const StepCounters* __restrict__ sc, uint64_t total) {
+    const uint32_t t = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t gen = sc->gen;
     const int levels = (int)min(sc->top_levels, (uint32_t)kTopSlots);
-    for (int L = levels - 1; L >= 0; --L) {
-        for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
-            const uint64_t slot = (uint64_t)t * kTopSlots + (uint64_t)L;
-            const uint64_t v = top_slot[slot];
-            if ((uint32_t)(v >> 32) != gen) continue;
-            const uint32_t nd = (uint32_t)v;
-            uint64_t kid[8];
+    for (int L = (int)blockIdx.y * (kBlock / 64) + wave; L < levels;
+         L += (int)gridDim.y * (kBlock / 64)) {
+        const uint64_t sv = top_slot[(uint64_t)t * kTopSlots + L];
+        if ((uint32_t)(sv >> 32) != gen) continue;
+        const uint32_t nd = (uint32_t)sv;
+        const uint64_t last = cold[nd].last;
+        const uint32_t t1 = (uint32_t)(last / kNodeTile);
+        Sum4 a{0.0, 0.0, 0.0, 0.0};
+        for (uint32_t tt = t + 1u + (uint32_t)lane; tt < t1; tt += 64u) a.add(tot_part[tt]);
 #pragma unroll
-            for (int o = 0; o < 8; ++o) kid[o] = top_kids[slot * 8 + o];
-            double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
-#pragma unroll
-            for (int o = 0; o < 8; ++o) {
-                if ((uint32_t)(kid[o] >> 32) != gen) continue;
-                const NodeHot& c = hot[(uint32_t)kid[o]];
-                const double m = c.mass;
-                M += m;
-                X += m * c.mx;
-                Y += m * c.my;
-                Z += m * c.mz;
-            }
-            const double inv = 1.0 / M;
-            hot[nd].mx = X * inv;
-            hot[nd].my = Y * inv;
-            hot[nd].mz = Z * inv;
-            hot[nd].mass = M;
+        for (int o = 32; o > 0; o >>= 1) {
+            Sum4 b;
+            b.m = __shfl_xor(a.m, o);
+            b.x = __shfl_xor(a.x, o);
+            b.y = __shfl_xor(a.y, o);
+            b.z = __shfl_xor(a.z, o);
+            a = lane & o ? Sum4{b.m + a.m, b.x + a.x, b.y + a.y, b.z + a.z}
+                         : Sum4{a.m + b.m, a.x + b.x, a.y + b.y, a.z + b.z};
         }
-        __syncthreads();
+        if (lane == 0) {
+            Sum4 r = out_part[(uint64_t)t * kTopSlots + L];
+            r.add(a);
+            const uint64_t t1_end = min((uint64_t)(t1 + 1) * kNodeTile, total);
+            r.add(last + 1 == t1_end ? tot_part[t1] : in_part[(uint64_t)t1 * kTopSlots + L]);
+            const double inv = 1.0 / r.m;
+            hot[nd].mx = r.x * inv;
+            hot[nd].my = r.y * inv;
+            hot[nd].mz = r.z * inv;
+            hot[nd].mass = r.m;
+        }
     }
 }
 
