@@ -37,7 +37,7 @@ def algorithmic_bytes(kernel, n, nodes, tiles):
         # (+ velocities on the first two steps only), node offsets, levels
         "scan_gather": (8 + 4 + 65 + 65 + 4 + 1) * n,
         # keys, node offsets, levels, leaf x y z m; node records written once
-        "nodes": 45.0 * n + (64.0 + 8.0) * nodes,
+        "nodes": 45.0 * n + 64.0 * nodes,
         # own x y z m fixed + Stormer x_prev in, x and x_prev out; tree read once
         "walk": (33.0 + 24.0 + 48.0) * n + 64.0 * nodes,
         "direct": 32.0 * n,

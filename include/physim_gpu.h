@@ -47,7 +47,7 @@ typedef enum pg_status {
     PG_ERR_COINCIDENT = 6, /* bodies within 1e-9 on every axis (reference merge rule) */
     PG_ERR_NONFINITE = 7,  /* NaN or infinite coordinate */
     PG_ERR_STATE = 8,      /* call out of order */
-    PG_ERR_INTERNAL = 9,   /* a device-side bounded wait expired or a capacity was hit */
+    PG_ERR_INTERNAL = 9,   /* a device-side capacity was exceeded */
 } pg_status;
 
 typedef enum pg_kind {

@@ -82,7 +82,6 @@ struct pg_engine {
     uint32_t* lvl_first = nullptr;   // per tile and level: first node-starting body, scanned
     uint64_t* k2 = nullptr;          // second key words of bodies whose keys tie
     NodeHot* hot = nullptr;
-    NodeCold* cold = nullptr;
     double* acc_out = nullptr;
     SegDesc* desc[2] = {nullptr, nullptr};
     uint64_t* seg_off = nullptr;
@@ -92,6 +91,7 @@ struct pg_engine {
     uint32_t* ext_chunk = nullptr;
     uint64_t* ext_partial = nullptr;
     uint64_t* top_slot = nullptr;  // generation-tagged tile-spanning node per (tile, level)
+    uint32_t* top_last = nullptr;  // its last body
     Sum4* out_part = nullptr;      // per tile and level: suffix sums of spanning nodes
     Sum4* in_part = nullptr;       // per tile and level: prefix sums of spanning nodes
     Sum4* tot_part = nullptr;      // per tile: mass sums of all its bodies
@@ -182,8 +182,8 @@ extern "C" uint64_t pg_sim_bytes(uint64_t n, int32_t kind) {
     const uint64_t scratch =
         kind == PG_KIND_DIRECT
             ? n * 24
-            : n * (8 * 2 + 4 * 2 + 4 + 1 + 24) + nodes_bound(n, dim, key_lmax(dim, 0)) *
-                                                      (sizeof(NodeHot) + sizeof(NodeCold));
+            : n * (8 * 2 + 4 * 2 + 4 + 4 + 1 + 8 + 24) + nodes_bound(n, dim, key_lmax(dim, 0)) *
+                                                      sizeof(NodeHot);
     return state_bytes(n) + scratch;
 }
 
@@ -361,7 +361,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_tp = take((scan_tiles + 2) * sizeof(Sum4));
     const uint64_t o_lf = take((scan_tiles + 2) * kTopSlots * 4);
     const uint64_t o_k2 = take(bodies * 8);
-    const uint64_t o_hot = take(nodes * sizeof(NodeHot)), o_cold = take(nodes * sizeof(NodeCold));
+    const uint64_t o_hot = take(nodes * sizeof(NodeHot));
     const uint64_t o_acc = take(bodies * 24);
     const uint64_t o_d0 = take(segs * sizeof(SegDesc)), o_d1 = take(segs * sizeof(SegDesc));
     const uint64_t o_so = take((segs + 1) * 8);
@@ -372,7 +372,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_es = take(chunks * 4), o_ec = take(chunks * 4), o_ep = take(chunks * 8);
     // a tile boundary is crossed by at most one node per level (levels 0..32)
     const uint64_t top = (bodies / kComTile + 2) * kTopSlots;
-    const uint64_t o_tl = take(top * 8);
+    const uint64_t o_tl = take(top * 8), o_tla = take(top * 4);
     const uint64_t total = off;
     if (e->sims_bytes + total > e->budget)
         return pg_set_error(PG_ERR_BUDGET, "scratch of %llu bytes exceeds the HBM budget",
@@ -401,7 +401,6 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->lvl_first = (uint32_t*)(b + o_lf);
     e->k2 = (uint64_t*)(b + o_k2);
     e->hot = (NodeHot*)(b + o_hot);
-    e->cold = (NodeCold*)(b + o_cold);
     e->acc_out = (double*)(b + o_acc);
     e->desc[0] = (SegDesc*)(b + o_d0);
     e->desc[1] = (SegDesc*)(b + o_d1);
@@ -414,6 +413,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->ext_chunk = (uint32_t*)(b + o_ec);
     e->ext_partial = (uint64_t*)(b + o_ep);
     e->top_slot = (uint64_t*)(b + o_tl);
+    e->top_last = (uint32_t*)(b + o_tla);
     e->out_part = (Sum4*)(b + o_op);
     e->in_part = (Sum4*)(b + o_ip);
     e->tot_part = (Sum4*)(b + o_tp);
@@ -494,15 +494,12 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     hipStream_t st = e->stream;
     SegDesc* desc = e->desc[q];
     const uint64_t total = bp.total;
-    launch(e, cap, "prep", [&] {
-        k_prep<<<1, kBlock, 0, st>>>(desc, bp.nseg, e->sc, INTEG ? 1 : 0);
-    });
     launch(e, cap, "extent", [&] {
         k_extent_partial<<<bp.nchunks, kBlock, 0, st>>>(desc, e->ext_seg, e->ext_chunk,
                                                         e->ext_partial);
     });
     launch(e, cap, "extent_final", [&] {
-        k_extent_final<<<bp.nseg, kBlock, 0, st>>>(desc, e->ext_partial);
+        k_extent_final<<<bp.nseg, kBlock, 0, st>>>(desc, e->ext_partial, e->sc, INTEG ? 1 : 0);
     });
     const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
     launch(e, cap, "keys", [&] {
@@ -545,12 +542,12 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "nodes", [&] {
         k_nodes<DIM><<<scan_tiles, kNodeThreads, 0, st>>>(
             desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_base,
-            e->lvl_first, e->first_node, e->lo_level, e->hot, e->cold, e->cap_nodes, e->top_slot,
+            e->lvl_first, e->first_node, e->lo_level, e->hot, e->cap_nodes, e->top_slot, e->top_last,
             e->out_part, e->in_part, e->tot_part, e->k2, e->sc);
     });
     launch(e, cap, "com_span", [&] {
         k_com_span<<<dim3(scan_tiles, kComSpanGroups), kBlock, 0, st>>>(
-            e->hot, e->cold, e->top_slot, e->out_part, e->in_part, e->tot_part, e->sc, total);
+            e->hot, e->top_last, e->top_slot, e->out_part, e->in_part, e->tot_part, e->sc, total);
     });
     launch(e, cap, "walk", [&] {
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
@@ -607,7 +604,7 @@ static uint32_t map_err(uint32_t bits) {
     if (bits & kErrNonFinite) return PG_ERR_NONFINITE;
     if (bits & kErrCoincident) return PG_ERR_COINCIDENT;
     if (bits & kErrDepth) return PG_ERR_DEPTH;
-    if (bits & (kErrCapacity | kErrSpin)) return PG_ERR_INTERNAL;
+    if (bits & kErrCapacity) return PG_ERR_INTERNAL;
     return PG_OK;
 }
 
@@ -625,7 +622,7 @@ static int err_text(uint32_t bits) {
                                 "two bodies share every level the Morton key holds");
         case PG_ERR_INTERNAL:
             return pg_set_error(PG_ERR_INTERNAL,
-                                "device-side capacity or bounded wait exceeded (bits 0x%x)", bits);
+                                "device-side capacity exceeded (bits 0x%x)", bits);
         default:
             return PG_OK;
     }
@@ -1109,23 +1106,22 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
     if (count) *count = nn;
     if (!rec && !fl) return PG_OK;
     std::vector<NodeHot> hot(nn);
-    std::vector<NodeCold> cold(nn);
     std::vector<uint32_t> id(n);
     HIP_TRY(hipMemcpy(hot.data(), e->hot, nn * sizeof(NodeHot), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(cold.data(), e->cold, nn * sizeof(NodeCold), hipMemcpyDeviceToHost));
     // The permuted ids live in the buffer the build wrote: the current one.
     HIP_TRY(hipMemcpy(id.data(), s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
+    uint64_t first = 0;  // the body owning node k: first_node[first] <= k < first_node[first + 1]
     for (uint64_t k = 0; k < nn && k < cap; ++k) {
         const NodeHot& h = hot[k];
-        const NodeCold& c = cold[k];
+        while (first + 1 < n && fn[first + 1] <= k) ++first;
         const int L = (int)h.level;
         const int oct = L > 0 ? (int)h.octant : -1;
         const bool leaf = h.skip == k + 1;
         if (rec) {
             rec[4 * k + 0] = L;
             rec[4 * k + 1] = oct;
-            rec[4 * k + 2] = leaf ? (int64_t)id[c.first] : -1;
-            rec[4 * k + 3] = c.first;
+            rec[4 * k + 2] = leaf ? (int64_t)id[first] : -1;
+            rec[4 * k + 3] = (int64_t)first;
         }
         if (fl) {
             fl[8 * k + 0] = h.mass;

@@ -13,8 +13,6 @@ constexpr int kSortTile = kBlock * kSortItems;
 constexpr int kScanItems = 8;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
-constexpr uint32_t kSpinLimit = 1u << 22;  // bound of every device-side wait
-constexpr int kScanLookWindow = 8;  // look-back status loads in flight (node-offset scan)
 constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort tiles
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
 constexpr int kNodeTile = 2048;    // sorted bodies per node-building block
@@ -33,7 +31,6 @@ enum : uint32_t {
     kErrCoincident = 1u << 1,
     kErrNonFinite = 1u << 2,
     kErrCapacity = 1u << 3,
-    kErrSpin = 1u << 4,
 };
 
 struct SegState;
@@ -72,8 +69,7 @@ struct SegState {
     uint32_t vperm;         // velocities must travel with the state (not yet derivable)
 };
 
-// Tree node, split into the part the walk reads (64 bytes, one line) and the
-// part only the build reads.
+// Tree node record, one 64-byte line (what the walk reads per visited node).
 struct NodeHot {
     double cx, cy, cz;      // geometric centre (reference rounding)
     double mx, my, mz;      // centre of mass (leaf: the body's exact position)
@@ -82,9 +78,6 @@ struct NodeHot {
     uint16_t level;
     uint8_t octant;         // child index within the parent (0 for a root)
     uint8_t pad;
-};
-struct NodeCold {
-    uint32_t first, last;   // sorted-body range
 };
 
 // Batch-wide counters reset at the start of every step.

@@ -23,18 +23,6 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-// Look-back status word: [63:34] version tag, [33:32] flag, [31:0] value.
-constexpr uint64_t kFlagAgg = 1, kFlagPrefix = 2;
-__device__ __forceinline__ uint64_t status_pack(uint32_t tag, uint64_t flag, uint32_t v) {
-    return ((uint64_t)(tag & 0x3FFFFFFFu) << 34) | (flag << 32) | v;
-}
-__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Segment of global index g: largest s with seg_off[s] <= g (empty segments
 // share an offset with their successor and are skipped).
 __device__ __forceinline__ uint32_t seg_of(const uint64_t* __restrict__ seg_off, uint32_t nseg,
@@ -122,58 +110,6 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return wbase + inc - v;
 }
 
-// Decoupled look-back: exclusive prefix of tile `tile` for one value lane.
-// Status words travel between XCDs through memory, so each load costs a
-// round trip of the order of a microsecond and the walk is latency-bound: the
-// words of W predecessors are loaded together (their latencies overlap), then
-// consumed nearest first: aggregates add up until an inclusive prefix ends the
-// walk; a predecessor that has not published yet restarts the window there.
-// With every tile walking at W tiles per round trip, the last of T tiles
-// resolves after about sqrt(2T / W) round trips.
-//
-// Tiles are numbered by blockIdx: blocks are dispatched in index order, so
-// the lowest unfinished tile is always resident and never waits. Every wait
-// is bounded by kSpinLimit regardless (kErrSpin).
-template <int W>
-__device__ __forceinline__ uint32_t lookback(const uint64_t* status, uint32_t tile,
-                                             uint32_t stride, uint32_t col, uint32_t tag,
-                                             uint32_t* err) {
-    uint32_t excl = 0;
-    int64_t t = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    const uint32_t want = tag & 0x3FFFFFFFu;
-    while (t >= 0) {
-        uint64_t w[W];
-#pragma unroll
-        for (int k = 0; k < W; ++k)
-            w[k] = t - k >= 0 ? status_load(status + (uint64_t)(t - k) * stride + col) : 0ull;
-        bool done = false, stalled = false;
-        int used = 0;
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            if (done || stalled || t - k < 0) continue;
-            const uint64_t flag = (w[k] >> 32) & 3ull;
-            if ((uint32_t)(w[k] >> 34) != want || flag == 0) {
-                stalled = true;
-                continue;
-            }
-            excl += (uint32_t)w[k];
-            ++used;
-            if (flag == kFlagPrefix) done = true;
-        }
-        if (done) break;
-        t -= used;
-        if (stalled) {
-            if (++spins > kSpinLimit) {
-                atomicOr(err, kErrSpin);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    return excl;
-}
-
 // ------------------------------------------------------------ step prologue
 __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCounters* sc,
                        int integrate) {
@@ -238,9 +174,12 @@ __global__ __launch_bounds__(kBlock) void k_extent_partial(const SegDesc* __rest
     if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
-// One block per segment: the segment's extent from its chunk partials.
+// One block per segment: the segment's extent from its chunk partials, and
+// the tree step's prologue (what k_prep does for the direct step): the batch
+// counters (block 0) and the segment's step state.
 __global__ __launch_bounds__(kBlock) void k_extent_final(const SegDesc* __restrict__ desc,
-                                                         const uint64_t* __restrict__ partial) {
+                                                         const uint64_t* __restrict__ partial,
+                                                         StepCounters* sc, int integrate) {
     __shared__ uint64_t s_wave[kBlock / 64];
     const SegDesc& d = desc[blockIdx.x];
     uint64_t m = 0;
@@ -249,7 +188,19 @@ __global__ __launch_bounds__(kBlock) void k_extent_final(const SegDesc* __restri
         m = b > m ? b : m;
     }
     m = block_max_u64(m, s_wave);
-    if (threadIdx.x == 0) d.st->extent_bits = m;
+    if (threadIdx.x == 0) {
+        if (blockIdx.x == 0) {
+            sc->gen += 1;
+            sc->top_levels = 0;
+        }
+        SegState* st = d.st;
+        st->extent_bits = m;
+        st->interactions = 0;
+        st->max_level = 0;
+        st->history = st->steps_done > 0 ? 1u : 0u;
+        st->vperm = st->steps_done <= 1 ? 1u : 0u;
+        if (integrate) st->steps_done += 1;
+    }
 }
 
 // Adds one digit per lane (256 = none) to the block's LDS digit counters:
@@ -581,7 +532,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 lo = lcpL + 1;
                 const uint64_t j = i - d.off;
                 d.dx[j] = d.sx[src]; d.dy[j] = d.sy[src]; d.dz[j] = d.sz[src];
-                d.dpx[j] = d.spx[src]; d.dpy[j] = d.spy[src]; d.dpz[j] = d.spz[src];
+                // the previous positions only where the Stormer history is
+                // valid (the Taylor step writes them; force-only calls never
+                // read them)
+                if (d.st->history) {
+                    d.dpx[j] = d.spx[src]; d.dpy[j] = d.spy[src]; d.dpz[j] = d.spz[src];
+                }
                 if (d.st->vperm) {
                     d.dvx[j] = d.svx[src]; d.dvy[j] = d.svy[src]; d.dvz[j] = d.svz[src];
                 }
@@ -834,8 +790,8 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ node_local,
     const uint32_t* __restrict__ tile_base, const uint32_t* __restrict__ lvl_first,
     uint32_t* __restrict__ first_node, const uint8_t* __restrict__ lo_level,
-    NodeHot* __restrict__ hot, NodeCold* __restrict__ cold, uint64_t cap,
-    uint64_t* __restrict__ top_slot, Sum4* __restrict__ out_part, Sum4* __restrict__ in_part,
+    NodeHot* __restrict__ hot, uint64_t cap, uint64_t* __restrict__ top_slot,
+    uint32_t* __restrict__ top_last, Sum4* __restrict__ out_part, Sum4* __restrict__ in_part,
     Sum4* __restrict__ tot_part, const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc) {
     __shared__ uint64_t s_keys[kNodeTile];
     __shared__ uint32_t s_first[kNodeTile + 1];
@@ -952,7 +908,6 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             h.octant = L > 0 ? (uint8_t)key_digit<DIM>(key, kx, lmax, L - 1) : 0;
             h.pad = 0;
             hot[nd] = h;
-            cold[nd] = NodeCold{(uint32_t)i, (uint32_t)last};
             if (in_lds) {
                 const uint32_t loc = (uint32_t)nd - n0;
                 s_tag[loc] = (uint8_t)L | (leaf || spans ? 0 : kTagInternal);
@@ -964,6 +919,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             }
             if (spans && L < kTopSlots) {
                 top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
+                top_last[(uint64_t)tile * kTopSlots + L] = (uint32_t)last;
                 atomicMax(&s_topl, (uint32_t)L + 1u);
             }
             if (k + 1 < cnt) {
@@ -1024,7 +980,8 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
         // records, made visible across the block by device-scope fences
         for (int L = maxl - 1; L >= 0; --L) {
             for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kNodeThreads) {
-                if ((int)hot[nd].level == L && hot[nd].skip != nd + 1u && cold[nd].last < b1)
+                // internal and inside the tile: its subtree ends by the tile's last node
+                if ((int)hot[nd].level == L && hot[nd].skip != nd + 1u && hot[nd].skip <= n1)
                     com_from_children(hot, nd);
             }
             __threadfence();
@@ -1122,7 +1079,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
 // its first body, the totals of the tiles in between, and the prefix of the
 // tile where it ends (or that tile's total when it ends with the tile).
 __global__ __launch_bounds__(kBlock) void k_com_span(
-    NodeHot* __restrict__ hot, const NodeCold* __restrict__ cold,
+    NodeHot* __restrict__ hot, const uint32_t* __restrict__ top_last,
     const uint64_t* __restrict__ top_slot, const Sum4* __restrict__ out_part,
     const Sum4* __restrict__ in_part, const Sum4* __restrict__ tot_part,
     const StepCounters* __restrict__ sc, uint64_t total) {
@@ -1135,7 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_com_span(
         const uint64_t sv = top_slot[(uint64_t)t * kTopSlots + L];
         if ((uint32_t)(sv >> 32) != gen) continue;
         const uint32_t nd = (uint32_t)sv;
-        const uint64_t last = cold[nd].last;
+        const uint64_t last = top_last[(uint64_t)t * kTopSlots + L];
         const uint32_t t1 = (uint32_t)(last / kNodeTile);
         Sum4 a{0.0, 0.0, 0.0, 0.0};
         for (uint32_t tt = t + 1u + (uint32_t)lane; tt < t1; tt += 64u) a.add(tot_part[tt]);
@@ -1275,8 +1232,15 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
         const bool walker = mine && !fixed;
         uint32_t skip_until = 0;
         uint32_t nd = root;
+        // The next node is nd + 1 (some lane opened this one) or its skip:
+        // both records are loaded while this one is tested, so the walk's
+        // dependent chain does not wait a load per node (indices clamped
+        // into the node array; a record past `end` is never used).
+        const uint32_t clamp = (uint32_t)(cap - 1);
+        NodeHot h = hot[min(nd, clamp)];
         while (nd < end) {
-            const NodeHot h = hot[nd];
+            const NodeHot h_open = hot[min(nd + 1u, clamp)];
+            const NodeHot h_skip = hot[min(h.skip, clamp)];
             const bool leaf = h.skip == nd + 1u;
             bool open = false;
             if (walker && nd >= skip_until) {
@@ -1299,7 +1263,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
                     open = true;
                 }
             }
-            nd = uniform_u32(__ballot(open) || h.skip <= nd ? nd + 1u : h.skip);
+            const bool step = __ballot(open) || h.skip <= nd;
+            nd = uniform_u32(step ? nd + 1u : h.skip);
+            h = step ? h_open : h_skip;
         }
         pending = pending && !mine;
     }
