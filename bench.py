@@ -24,9 +24,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def algorithmic_bytes(kernel, n, nodes, tiles):
+SORT_TILE = 2048   # keys per radix-sort tile
+
+
+def algorithmic_bytes(kernel, n, nodes):
     """Compulsory bytes one launch of `kernel` moves for n bodies and the
     given tree size: each input read once, each output written once."""
+    tiles = (n + SORT_TILE - 1) // SORT_TILE
     table = {
         "extent": 24.0 * n,                          # read x, y, z
         "keys": 36.0 * n + 1024.0 * tiles,           # read x, y, z; write key, index; digit counts
@@ -367,11 +371,10 @@ def main():
     sim.step(1)
     sim.set_counting(False)
     st = sim.stats()
-    tiles = (n + 2047) // 2048
     per_step_ms = {k: v[0] / args.profile_steps for k, v in prof.items()}
     dom = max(per_step_ms, key=per_step_ms.get)
     avg_ms = prof[dom][0] / prof[dom][1]
-    abytes = algorithmic_bytes(dom, n, st["nodes"], tiles)
+    abytes = algorithmic_bytes(dom, n, st["nodes"])
     achieved = abytes / (avg_ms * 1e-3) / 1e9
 
     # HBM bytes per launch of that kernel, measured with rocprofv3 FETCH_SIZE /

@@ -79,6 +79,7 @@ struct pg_engine {
     uint32_t* node_local = nullptr;  // per-body node offset within its tile
     uint32_t* tile_tot = nullptr;    // nodes per tile
     uint32_t* tile_base = nullptr;   // first node of each tile (+ total)
+    uint32_t* tile_start = nullptr;  // first body of each node tile (+ total)
     uint32_t* lvl_first = nullptr;   // per tile and level: first node-starting body, scanned
     uint64_t* k2 = nullptr;          // second key words of bodies whose keys tie
     NodeHot* hot = nullptr;
@@ -98,12 +99,13 @@ struct pg_engine {
     StepCounters* sc = nullptr;
     uint32_t* err_global = nullptr;
     uint32_t cap_chunks = 0;
+    uint32_t cap_tiles = 0;       // node tiles the scratch holds
     uint64_t cap_top = 0;
 
     // host mirrors
     std::vector<SegDesc> h_desc[2];
     std::vector<uint64_t> h_seg_off;
-    std::vector<uint32_t> h_blk_seg, h_blk_tile, h_ext_seg, h_ext_chunk;
+    std::vector<uint32_t> h_blk_seg, h_blk_tile, h_ext_seg, h_ext_chunk, h_tile_start;
 
     // graph cache (two entries: the two parities of an odd step count)
     struct GraphEntry {
@@ -343,7 +345,8 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     blocks = std::max<uint32_t>(blocks, 16);
     nodes = std::max<uint64_t>(nodes, 1024);
     const uint64_t sort_tiles = (bodies + kSortTile - 1) / kSortTile;
-    const uint64_t scan_tiles = (bodies + kScanTile - 1) / kScanTile;
+    // node tiles start at every segment's first body: at most bodies/T + segs
+    const uint64_t node_tiles = bodies / kNodeTile + segs + 1;
     uint64_t off = 0;
     auto take = [&](uint64_t bytes) {
         uint64_t o = off;
@@ -354,12 +357,12 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_v0 = take(bodies * 4), o_v1 = take(bodies * 4);
     const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies);
     const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
-    const uint64_t o_nl = take((bodies + 1) * 4), o_tt = take((scan_tiles + 2) * 4);
-    const uint64_t o_tb = take((scan_tiles + 2) * 4);
-    const uint64_t o_op = take((scan_tiles + 2) * kTopSlots * sizeof(Sum4));
-    const uint64_t o_ip = take((scan_tiles + 2) * kTopSlots * sizeof(Sum4));
-    const uint64_t o_tp = take((scan_tiles + 2) * sizeof(Sum4));
-    const uint64_t o_lf = take((scan_tiles + 2) * kTopSlots * 4);
+    const uint64_t o_nl = take((bodies + 1) * 4), o_tt = take((node_tiles + 2) * 4);
+    const uint64_t o_tb = take((node_tiles + 2) * 4), o_ts = take((node_tiles + 2) * 4);
+    const uint64_t o_op = take((node_tiles + 2) * kTopSlots * sizeof(Sum4));
+    const uint64_t o_ip = take((node_tiles + 2) * kTopSlots * sizeof(Sum4));
+    const uint64_t o_tp = take((node_tiles + 2) * sizeof(Sum4));
+    const uint64_t o_lf = take((node_tiles + 2) * kTopSlots * 4);
     const uint64_t o_k2 = take(bodies * 8);
     const uint64_t o_hot = take(nodes * sizeof(NodeHot));
     const uint64_t o_acc = take(bodies * 24);
@@ -371,7 +374,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t chunks = bodies / kExtentChunk + segs + 1;
     const uint64_t o_es = take(chunks * 4), o_ec = take(chunks * 4), o_ep = take(chunks * 8);
     // a tile boundary is crossed by at most one node per level (levels 0..32)
-    const uint64_t top = (bodies / kComTile + 2) * kTopSlots;
+    const uint64_t top = (node_tiles + 2) * kTopSlots;
     const uint64_t o_tl = take(top * 8), o_tla = take(top * 4);
     const uint64_t total = off;
     if (e->sims_bytes + total > e->budget)
@@ -398,6 +401,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->node_local = (uint32_t*)(b + o_nl);
     e->tile_tot = (uint32_t*)(b + o_tt);
     e->tile_base = (uint32_t*)(b + o_tb);
+    e->tile_start = (uint32_t*)(b + o_ts);
     e->lvl_first = (uint32_t*)(b + o_lf);
     e->k2 = (uint64_t*)(b + o_k2);
     e->hot = (NodeHot*)(b + o_hot);
@@ -418,6 +422,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->in_part = (Sum4*)(b + o_ip);
     e->tot_part = (Sum4*)(b + o_tp);
     e->cap_chunks = (uint32_t)chunks;
+    e->cap_tiles = (uint32_t)node_tiles;
     e->cap_top = top;
     // Status words are versioned; zero them once so no stale tag can match.
     HIP_TRY(hipMemsetAsync(e->sc, 0, sizeof(StepCounters), e->stream));
@@ -480,6 +485,7 @@ struct BatchPlan {
     uint64_t total = 0;
     uint32_t nblocks = 0;
     uint32_t nchunks = 0;
+    uint32_t ntiles = 0;  // node tiles
     bool integrate = true;
 };
 
@@ -498,14 +504,11 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         k_extent_partial<<<bp.nchunks, kBlock, 0, st>>>(desc, e->ext_seg, e->ext_chunk,
                                                         e->ext_partial);
     });
-    launch(e, cap, "extent_final", [&] {
-        k_extent_final<<<bp.nseg, kBlock, 0, st>>>(desc, e->ext_partial, e->sc, INTEG ? 1 : 0);
-    });
     const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
     launch(e, cap, "keys", [&] {
-        k_keys<DIM><<<sort_tiles, kBlock, 0, st>>>(desc, e->seg_off, bp.nseg, total, bp.segbits,
-                                                   bp.lmax, e->keys[0], e->vals[0], e->sort_cnt,
-                                                   sort_tiles);
+        k_keys<DIM><<<sort_tiles, kKeyThreads, 0, st>>>(
+            desc, e->seg_off, bp.nseg, total, bp.segbits, bp.lmax, e->ext_partial, e->sc,
+            INTEG ? 1 : 0, e->keys[0], e->vals[0], e->sort_cnt, sort_tiles);
     });
     for (int p = 0; p < bp.passes; ++p) {
         const int a = p & 1, b = a ^ 1;
@@ -529,25 +532,27 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         k_ties<DIM><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(desc, total, skeys, svals, e->k2,
                                                                    bp.nseg, bp.segbits, bp.lmax);
     });
-    const unsigned scan_tiles = (unsigned)((total + kScanTile - 1) / kScanTile);
+    const unsigned tiles = bp.ntiles;
     launch(e, cap, "scan_gather", [&] {
-        k_scan_gather<DIM><<<scan_tiles, kBlock, 0, st>>>(
-            desc, total, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_tot,
-            e->lo_level, e->lvl_first, e->k2, e->err_global);
+        k_scan_gather<DIM><<<tiles, kBlock, 0, st>>>(
+            desc, e->tile_start, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local,
+            e->tile_tot, e->lo_level, e->lvl_first, e->k2, e->err_global);
     });
     launch(e, cap, "tree_scan", [&] {
-        k_tree_scan<<<1 + kTopSlots, kScanThreads, 0, st>>>(e->tile_tot, e->tile_base, scan_tiles,
+        k_tree_scan<<<1 + kTopSlots, kScanThreads, 0, st>>>(e->tile_tot, e->tile_base, tiles,
                                                              e->lvl_first, e->first_node, total);
     });
     launch(e, cap, "nodes", [&] {
-        k_nodes<DIM><<<scan_tiles, kNodeThreads, 0, st>>>(
-            desc, total, skeys, bp.nseg, bp.segbits, bp.lmax, e->node_local, e->tile_base,
-            e->lvl_first, e->first_node, e->lo_level, e->hot, e->cap_nodes, e->top_slot, e->top_last,
-            e->out_part, e->in_part, e->tot_part, e->k2, e->sc);
+        k_nodes<DIM><<<tiles, kNodeThreads, 0, st>>>(
+            desc, e->seg_off, e->tile_start, total, skeys, bp.nseg, bp.segbits, bp.lmax,
+            e->node_local, e->tile_base, e->lvl_first, e->first_node, e->lo_level, e->hot,
+            e->cap_nodes, e->top_slot, e->top_last, e->out_part, e->in_part, e->tot_part, e->k2,
+            e->sc);
     });
     launch(e, cap, "com_span", [&] {
-        k_com_span<<<dim3(scan_tiles, kComSpanGroups), kBlock, 0, st>>>(
-            e->hot, e->top_last, e->top_slot, e->out_part, e->in_part, e->tot_part, e->sc, total);
+        k_com_span<<<dim3(tiles, kComSpanGroups), kBlock, 0, st>>>(
+            desc, e->seg_off, bp.nseg, e->tile_start, e->hot, e->top_last, e->top_slot,
+            e->out_part, e->in_part, e->tot_part, e->sc);
     });
     launch(e, cap, "walk", [&] {
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
@@ -683,9 +688,26 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         }
     }
     bp.nchunks = (uint32_t)e->h_ext_seg.size();
+    // node tiles of kNodeTile bodies from each segment's first body, so that a
+    // segment's tree arithmetic does not depend on where the batch puts it
+    e->h_tile_start.clear();
+    std::vector<uint32_t> tile0(bp.nseg);
+    {
+        uint64_t o = 0;
+        for (uint32_t s = 0; s < bp.nseg; ++s) {
+            tile0[s] = (uint32_t)e->h_tile_start.size();
+            for (uint64_t t = 0; t < sims[s]->n; t += kNodeTile)
+                e->h_tile_start.push_back((uint32_t)(o + t));
+            o += sims[s]->n;
+        }
+        bp.ntiles = (uint32_t)e->h_tile_start.size();
+        e->h_tile_start.push_back((uint32_t)o);
+    }
     PG_TRY(ensure_scratch(e, bp.total, nodes, bp.nseg, bp.nblocks));
     if (bp.nchunks > e->cap_chunks)
         return pg_set_error(PG_ERR_INTERNAL, "extent chunk table overflow");
+    if (bp.ntiles > e->cap_tiles)
+        return pg_set_error(PG_ERR_INTERNAL, "node tile table overflow");
 
     // descriptors for both parities
     e->h_seg_off.assign(bp.nseg + 1, 0);
@@ -717,6 +739,7 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
             d.st = sm->st;
             d.chunk0 = chunk0[s];
             d.nchunks = nchunk[s];
+            d.tile0 = tile0[s];
         }
         off += sm->n;
     }
@@ -736,6 +759,8 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     HIP_TRY(hipMemcpyAsync(e->ext_seg, e->h_ext_seg.data(), bp.nchunks * 4,
                            hipMemcpyHostToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(e->ext_chunk, e->h_ext_chunk.data(), bp.nchunks * 4,
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->tile_start, e->h_tile_start.data(), (bp.ntiles + 1) * 4,
                            hipMemcpyHostToDevice, e->stream));
 
     const bool flips = bp.kind != PG_KIND_DIRECT || integrate;

@@ -10,20 +10,22 @@ namespace pg {
 constexpr int kBlock = 256;        // threads per block (4 wave64)
 constexpr int kSortItems = 8;      // keys per thread in a radix-sort tile
 constexpr int kSortTile = kBlock * kSortItems;
-constexpr int kScanItems = 8;      // counts per thread in the node-offset scan
+constexpr int kScanItems = 2;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
+constexpr int kKeyThreads = 1024;  // threads of a key-building block (one sort tile)
 constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort tiles
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
-constexpr int kNodeTile = 2048;    // sorted bodies per node-building block
-constexpr int kComTile = kNodeTile; // the centre-of-mass tiles are the node tiles
-constexpr int kNodeThreads = 1024; // threads per node-building block
-constexpr int kNodeLds = 3584;     // nodes per tile whose centre of mass is reduced in LDS
+constexpr int kNodeTile = 512;     // sorted bodies per node-building block
+constexpr int kNodeThreads = 256;  // threads per node-building block (two bodies each)
+constexpr int kNodeLds = 896;      // nodes per tile built in LDS (larger tiles: in HBM)
 constexpr int kTopSlots = 65;      // tree levels 0..64: one tile-spanning node each
 constexpr int kMaxTieRun = 256;    // longest run of equal keys ordered by the second key word
 constexpr int kComSpanGroups = 5;   // level groups (of 4 waves) per tile in k_com_span
 static_assert(kScanTile == kNodeTile, "node offsets and node building share their tiles");
+static_assert(kNodeTile == 2 * kNodeThreads, "the partial sums take two bodies per thread");
 static_assert(kTopSlots <= kBlock, "per-level tables are filled by one thread per level");
+static_assert(kSortTile % kKeyThreads == 0 && kKeyThreads >= 256, "whole key tiles, a digit per thread");
 
 // Device-side error bits (sticky per segment), mapped to pg_status on the host.
 enum : uint32_t {
@@ -56,6 +58,7 @@ struct SegDesc {
     struct SegState* st;
     uint32_t chunk0;  // first extent chunk of this segment
     uint32_t nchunks;
+    uint32_t tile0;   // first node tile of this segment (node tiles start at its first body)
 };
 
 // Per-segment device scalars.

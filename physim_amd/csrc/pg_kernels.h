@@ -39,6 +39,15 @@ __device__ __forceinline__ uint32_t seg_of_key(uint64_t key, int segbits) {
     return segbits ? (uint32_t)(key >> (64 - segbits)) : 0u;
 }
 
+// Node tile of global index g < total: node tiles of kNodeTile bodies start
+// at every segment's first body.
+__device__ __forceinline__ uint32_t tile_of(const SegDesc* __restrict__ desc,
+                                            const uint64_t* __restrict__ seg_off, uint32_t nseg,
+                                            uint64_t g) {
+    const SegDesc& d = desc[seg_of(seg_off, nseg, g)];
+    return d.tile0 + (uint32_t)((g - d.off) / kNodeTile);
+}
+
 // Key prefix holding the segment bits and the top `level` tree digits.
 template <int DIM>
 __device__ __forceinline__ uint64_t key_prefix(uint64_t key, int lmax, int level) {
@@ -111,6 +120,21 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 }
 
 // ------------------------------------------------------------ step prologue
+// Opens a segment's step: its extent (bit pattern of max |coord|; 0 for the
+// direct sum, which has no tree) and step state.
+__device__ __forceinline__ void open_segment(SegState* st, uint64_t extent_bits, int integrate) {
+    st->extent_bits = extent_bits;
+    st->interactions = 0;
+    st->max_level = 0;
+    st->history = st->steps_done > 0 ? 1u : 0u;
+    // v is (x - x_prev) / dt after any step but the Taylor first one
+    // (verlet.rs:68-70), so it only has to be carried before and right
+    // after that step.
+    st->vperm = st->steps_done <= 1 ? 1u : 0u;
+    if (integrate) st->steps_done += 1;
+}
+
+// The direct step's prologue: batch counters and every segment's step.
 __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCounters* sc,
                        int integrate) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,21 +143,11 @@ __global__ void k_prep(const SegDesc* __restrict__ desc, uint32_t nseg, StepCoun
         sc->gen += 1;
         sc->top_levels = 0;
     }
-    for (uint32_t s = t; s < nseg; s += stride) {
-        SegState* st = desc[s].st;
-        st->extent_bits = 0;
-        st->interactions = 0;
-        st->max_level = 0;
-        st->history = st->steps_done > 0 ? 1u : 0u;
-        // v is (x - x_prev) / dt after any step but the Taylor first one
-        // (verlet.rs:68-70), so it only has to be carried before and right
-        // after that step.
-        st->vperm = st->steps_done <= 1 ? 1u : 0u;
-        if (integrate) st->steps_done += 1;
-    }
+    for (uint32_t s = t; s < nseg; s += stride) open_segment(desc[s].st, 0, integrate);
 }
 
-// Block-wide max of one u64 per thread (256 threads).
+// Block-wide max of one u64 per thread over NT threads.
+template <int NT = kBlock>
 __device__ __forceinline__ uint64_t block_max_u64(uint64_t v, uint64_t* s_wave) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -145,7 +159,7 @@ __device__ __forceinline__ uint64_t block_max_u64(uint64_t v, uint64_t* s_wave) 
     __syncthreads();
     uint64_t m = s_wave[0];
 #pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w) m = s_wave[w] > m ? s_wave[w] : m;
+    for (int w = 1; w < NT / 64; ++w) m = s_wave[w] > m ? s_wave[w] : m;
     __syncthreads();
     return m;
 }
@@ -174,35 +188,6 @@ __global__ __launch_bounds__(kBlock) void k_extent_partial(const SegDesc* __rest
     if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
-// One block per segment: the segment's extent from its chunk partials, and
-// the tree step's prologue (what k_prep does for the direct step): the batch
-// counters (block 0) and the segment's step state.
-__global__ __launch_bounds__(kBlock) void k_extent_final(const SegDesc* __restrict__ desc,
-                                                         const uint64_t* __restrict__ partial,
-                                                         StepCounters* sc, int integrate) {
-    __shared__ uint64_t s_wave[kBlock / 64];
-    const SegDesc& d = desc[blockIdx.x];
-    uint64_t m = 0;
-    for (uint32_t c = threadIdx.x; c < d.nchunks; c += kBlock) {
-        const uint64_t b = partial[d.chunk0 + c];
-        m = b > m ? b : m;
-    }
-    m = block_max_u64(m, s_wave);
-    if (threadIdx.x == 0) {
-        if (blockIdx.x == 0) {
-            sc->gen += 1;
-            sc->top_levels = 0;
-        }
-        SegState* st = d.st;
-        st->extent_bits = m;
-        st->interactions = 0;
-        st->max_level = 0;
-        st->history = st->steps_done > 0 ? 1u : 0u;
-        st->vperm = st->steps_done <= 1 ? 1u : 0u;
-        if (integrate) st->steps_done += 1;
-    }
-}
-
 // Adds one digit per lane (256 = none) to the block's LDS digit counters:
 // lanes of a wave that hold the same digit are found with match-any ballots
 // and their leader adds the group's size. Every lane of the wave must call.
@@ -223,28 +208,58 @@ __device__ __forceinline__ void wave_count_digit(uint32_t* s_h, uint32_t d) {
 // at each level the octant is (x > cx) | (y > cy) << 1 [| (z > cz) << 2] and
 // the child centre is c -/+ extent/2, rounded as the reference rounds it.
 // One block per radix-sort tile of kSortTile bodies (the batch's segments are
-// concatenated); also writes the tile's digit counts of the first radix pass
-// (digit-major: cnt[digit * tiles + tile]).
+// concatenated), which also writes the tile's digit counts of the first radix
+// pass (digit-major: cnt[digit * tiles + tile]). The extent
+// (transformers.rs:126-131) is the maximum of the segment's chunk partials,
+// reduced by the whole block when the tile lies in one segment; the thread of
+// a segment's first body stores it and opens the segment's step, and block 0
+// resets the batch counters.
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_keys(const SegDesc* __restrict__ desc,
-                                                 const uint64_t* __restrict__ seg_off,
-                                                 uint32_t nseg, uint64_t total, int segbits,
-                                                 int lmax, uint64_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals,
-                                                 uint32_t* __restrict__ cnt, uint32_t tiles) {
+__global__ __launch_bounds__(kKeyThreads) void k_keys(
+    const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off, uint32_t nseg,
+    uint64_t total, int segbits, int lmax, const uint64_t* __restrict__ partial,
+    StepCounters* __restrict__ sc, int integrate, uint64_t* __restrict__ keys,
+    uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt, uint32_t tiles) {
+    __shared__ uint64_t s_wave[kKeyThreads / 64];
     __shared__ uint32_t s_h[256];
-    s_h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-    for (int k = 0; k < kSortItems; ++k) {
-        const uint64_t g = base + (uint64_t)k * kBlock + threadIdx.x;
+    if (threadIdx.x < 256) s_h[threadIdx.x] = 0;
+    const uint64_t g0 = (uint64_t)blockIdx.x * kSortTile;
+    const uint32_t sa = seg_of(seg_off, nseg, g0);
+    const uint32_t sb = seg_of(seg_off, nseg, min(g0 + (uint64_t)kSortTile, total) - 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->gen += 1;
+        sc->top_levels = 0;
+    }
+    // the tile's extent when it lies in one segment (block-uniform branch)
+    uint64_t mblock = 0;
+    if (sa == sb) {
+        const SegDesc& d = desc[sa];
+        for (uint32_t c = threadIdx.x; c < d.nchunks; c += kKeyThreads) {
+            const uint64_t b = partial[d.chunk0 + c];
+            mblock = b > mblock ? b : mblock;
+        }
+        mblock = block_max_u64<kKeyThreads>(mblock, s_wave);
+    } else {
+        __syncthreads();
+    }
+    for (int k = 0; k < kSortTile / kKeyThreads; ++k) {
+        const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
         uint32_t dig = 256u;
         if (g < total) {
             const uint32_t s = seg_of(seg_off, nseg, g);
             const SegDesc& d = desc[s];
+            uint64_t m = mblock;
+            if (sa != sb) {
+                m = 0;
+                for (uint32_t c = 0; c < d.nchunks; ++c) {
+                    const uint64_t b = partial[d.chunk0 + c];
+                    m = b > m ? b : m;
+                }
+            }
+            if (g == d.off) open_segment(d.st, m, integrate);
             const uint64_t j = g - d.off;
             const double px = d.sx[j], py = d.sy[j], pz = d.sz[j];
-            double ext = bits_f64(d.st->extent_bits);
+            double ext = bits_f64(m);
             double cx = 0.0, cy = 0.0, cz = 0.0;
             uint64_t key = 0;
             for (int l = 0; l < lmax; ++l) {
@@ -265,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(const SegDesc* __restrict__ des
         wave_count_digit(s_h, dig);
     }
     __syncthreads();
-    cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
+    if (threadIdx.x < 256) cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
 // Stable LSD radix sort of (key, value) pairs, 8-bit digits, three kernels
@@ -490,7 +505,8 @@ __global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ des
 // builder finds the end of a node that leaves its tile in one load.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_scan_gather(
-    const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys,
+    const SegDesc* __restrict__ desc, const uint32_t* __restrict__ tile_start,
+    const uint64_t* __restrict__ skeys,
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
     uint32_t* __restrict__ node_local, uint32_t* __restrict__ tile_tot,
     uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_first,
@@ -500,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     __shared__ uint32_t s_firstlo[kTopSlots];
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
-    const uint64_t tbase = (uint64_t)tile * kScanTile;
+    const uint64_t tbase = tile_start[tile], tend = tile_start[tile + 1];
     if (tid < kTopSlots) s_firstlo[tid] = ~0u;
     __syncthreads();
 
@@ -509,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
         const uint32_t li = (uint32_t)k * kBlock + tid;
         const uint64_t i = tbase + li;
         uint32_t c = 0;
-        if (i < total) {
+        if (i < tend) {
             const uint64_t key = skeys[i];
             const uint32_t s = seg_of_key(key, segbits);
             const uint64_t src = s < nseg ? (uint64_t)svals[i] - desc[s].off : ~0ull;
@@ -573,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t li = (uint32_t)k * kBlock + tid;
-        if (tbase + li < total) node_local[tbase + li] = s_cnt[li];
+        if (tbase + li < tend) node_local[tbase + li] = s_cnt[li];
     }
 }
 
@@ -757,28 +773,56 @@ __device__ __forceinline__ void com_from_children(NodeHot* hot, uint32_t p) {
     hot[p].mass = M;
 }
 
-// Local-node tags of the LDS reduction (low 6 bits: level).
+// Local-node tags of the LDS reduction (low 7 bits: level, at most 64).
 constexpr uint8_t kTagInternal = 0x80;  // internal node whose subtree lies in the tile
-constexpr uint8_t kTagNone = 0xFF;      // no node written (never matches a level)
+constexpr uint8_t kTagLevel = 0x7F;
 
-// Builds the tree and its masses and centres of mass, one block per tile of
-// kNodeTile sorted bodies (the tiles of k_scan_gather).
-//
-// The tile's node offsets (tile base + offsets within the tile) are staged in
-// LDS with its keys and levels, and written out as first_node. Every node a
-// sorted body owns gets its geometric centre by the reference's descent,
-// half-width level, octant, sorted range and skip (= first node of the body
-// after the range); a leaf takes its body's exact position and mass. A node's
-// range end is found by bisection over the staged keys; a node that runs past
-// the tile ends before the first later body starting a node at its level or
-// above (the scanned per-level table), so no search leaves the tile.
+// A node record as four 16-byte chunks: (cx, cy), (cz, mx), (my, mz),
+// (mass, skip, level | octant << 16).
+static_assert(sizeof(NodeHot) == 64, "one node record per 64-byte line");
+static_assert(offsetof(NodeHot, mass) == 48 && offsetof(NodeHot, skip) == 56 &&
+                  offsetof(NodeHot, level) == 60 && offsetof(NodeHot, octant) == 62,
+              "chunk layout of a node record");
+__device__ __forceinline__ uint4 pack2(double a, double b) {
+    const uint64_t x = f64_bits(a), y = f64_bits(b);
+    return make_uint4((uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32));
+}
+__device__ __forceinline__ uint4 pack_tail(double mass, uint32_t skip, uint32_t level,
+                                           uint32_t octant) {
+    const uint64_t x = f64_bits(mass);
+    return make_uint4((uint32_t)x, (uint32_t)(x >> 32), skip, level | (octant << 16));
+}
+__device__ __forceinline__ void store_node(NodeHot* p, const NodeHot& h) {
+    uint4* o = reinterpret_cast<uint4*>(p);
+    o[0] = pack2(h.cx, h.cy);
+    o[1] = pack2(h.cz, h.mx);
+    o[2] = pack2(h.my, h.mz);
+    o[3] = pack_tail(h.mass, h.skip, h.level, h.octant);
+}
+
+// Builds the tree and its masses and centres of mass, one block per node tile
+// (up to kNodeTile sorted bodies of one segment; the tiles of k_scan_gather).
 //
 // The tile's bodies own the preorder node range [tile_base[t],
-// tile_base[t+1]); an internal node whose bodies all lie in the tile has its
-// whole subtree there, so its mass and centre of mass are reduced in LDS,
-// deepest level first, from its children in octant order (the arithmetic of
-// com_from_children). A tile with more than kNodeLds nodes does the same level
-// loop on the global records.
+// tile_base[t+1]) (tile base + offsets within the tile, also written out as
+// first_node). Every node gets its geometric centre by the reference's
+// descent, its level, octant, sorted range and skip (= first node of the body
+// after the range); a leaf takes its body's exact position and mass. A node's
+// range end is found by a galloping search over the staged keys (keys carry
+// their segment, so no range crosses one); a node that runs past the tile
+// ends before the first later body starting a node at its level or above (the
+// scanned per-level table), so no search leaves the tile.
+//
+// A tile of at most kNodeLds nodes is built in LDS, one thread per node: the
+// bodies fill the node -> owner table and their leaves; each node then finds
+// its range, skip and tag; an internal node whose bodies all lie in the tile
+// has its whole subtree there, so its mass and centre of mass are reduced in
+// LDS, deepest level first, from its children in octant order (the
+// arithmetic of com_from_children); last, every record is written once, a
+// 16-byte chunk per thread, so that a wave stores 1 KiB of consecutive
+// records per instruction. A tile with more nodes (clustered bodies) builds
+// them body by body into the global records and runs the same level loop
+// there.
 //
 // An internal node whose range leaves the tile is left to k_com_span: it is
 // recorded (generation-tagged) at slot tile * kTopSlots + level, and
@@ -786,18 +830,22 @@ constexpr uint8_t kTagNone = 0xFF;      // no node written (never matches a leve
 // block also writes the tile's partial mass sums (below).
 template <int DIM>
 __global__ __launch_bounds__(kNodeThreads) void k_nodes(
-    const SegDesc* __restrict__ desc, uint64_t total, const uint64_t* __restrict__ skeys_g,
+    const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off,
+    const uint32_t* __restrict__ tile_start, uint64_t total, const uint64_t* __restrict__ skeys_g,
     uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ node_local,
     const uint32_t* __restrict__ tile_base, const uint32_t* __restrict__ lvl_first,
     uint32_t* __restrict__ first_node, const uint8_t* __restrict__ lo_level,
     NodeHot* __restrict__ hot, uint64_t cap, uint64_t* __restrict__ top_slot,
     uint32_t* __restrict__ top_last, Sum4* __restrict__ out_part, Sum4* __restrict__ in_part,
     Sum4* __restrict__ tot_part, const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc) {
+    static_assert(kNodeTile <= 65536, "owner indices are 16-bit");
     __shared__ uint64_t s_keys[kNodeTile];
     __shared__ uint32_t s_first[kNodeTile + 1];
     __shared__ uint8_t s_lo[kNodeTile];
     __shared__ double s_m[kNodeLds], s_x[kNodeLds], s_y[kNodeLds], s_z[kNodeLds];
-    __shared__ uint16_t s_skip[kNodeLds];
+    __shared__ uint16_t s_skip[kNodeLds];  // skip - first node, at most the tile's node count
+    __shared__ uint16_t s_own[kNodeLds];
+    __shared__ uint32_t s_spanskip[kTopSlots];  // skip of the tile-spanning node per level
     __shared__ uint8_t s_tag[kNodeLds];
     __shared__ uint64_t s_next;  // key just after the tile
     __shared__ uint32_t s_maxl, s_topl;
@@ -805,144 +853,126 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     __shared__ uint32_t s_wmin[kNodeThreads / 64];
     const uint64_t gen = (uint64_t)sc->gen << 32;
     const uint32_t tile = blockIdx.x, tiles = gridDim.x;
-    const uint64_t b0 = (uint64_t)tile * kNodeTile;
-    const uint64_t b1 = min(b0 + (uint64_t)kNodeTile, total);
+    const uint64_t b0 = tile_start[tile], b1 = tile_start[tile + 1];
+    const uint32_t nb = (uint32_t)(b1 - b0);
     const uint32_t n0 = tile_base[tile];
-    const uint32_t n1 = (uint32_t)min((uint64_t)tile_base[tile + 1], cap);
+    const uint32_t nend = tile_base[tile + 1];
+    const uint32_t n1 = (uint32_t)min((uint64_t)nend, cap);
     const uint32_t nloc = n1 > n0 ? n1 - n0 : 0u;
     const bool in_lds = nloc <= (uint32_t)kNodeLds;
-    for (uint64_t j = b0 + threadIdx.x; j < b1; j += kNodeThreads) {
-        s_keys[j - b0] = skeys_g[j];
-        s_lo[j - b0] = lo_level[j];
-        const uint32_t f = node_local[j] + n0;
-        s_first[j - b0] = f;
-        first_node[j] = f;
-    }
+    // a node tile lies in one segment
+    const SegDesc& d = desc[seg_of(seg_off, nseg, b0)];
+    const double ext0 = bits_f64(d.st->extent_bits);
     if (threadIdx.x == 0) {
-        s_first[b1 - b0] = tile_base[tile + 1];
+        s_first[nb] = nend;
         s_next = b1 < total ? skeys_g[b1] : 0ull;  // read only when the segment continues past b1
         s_maxl = 0;
         s_topl = 0;
     }
-    if (in_lds)
-        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) s_tag[k] = kTagNone;
-    __syncthreads();
-    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kNodeThreads) {
-        const uint64_t key = s_keys[i - b0];
-        const uint32_t s = seg_of_key(key, segbits);
-        if (s >= nseg) continue;
-        const SegDesc& d = desc[s];
-        const uint64_t end = d.off + d.n;
-        const int lo = s_lo[i - b0];
-        const uint32_t nbase = s_first[i - b0];
-        const int cnt = (int)(s_first[i - b0 + 1] - nbase);
-        const int hi = lo + cnt - 1;
-        // centre of the first owned node: the reference's descent, lo levels down
-        // levels past lmax exist only for bodies whose keys tie; they read
-        // the second key word
-        const uint64_t kx = hi > lmax ? k2[i] : 0ull;
-        // Do body j (key kj) and body i share `level` levels?
-        auto same_at = [&](uint64_t kj, uint64_t j, int level) -> bool {
-            if (level <= lmax) return key_prefix<DIM>(kj, lmax, level) == key_prefix<DIM>(key, lmax, level);
-            return kj == key && k2_prefix<DIM>(k2[j], lmax, level) == k2_prefix<DIM>(kx, lmax, level);
-        };
-        double ext = bits_f64(d.st->extent_bits);
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        for (int L = 0; L < lo; ++L) {
-            const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
-            const double h2 = half(ext);
-            cx = (o & 1u) ? cx + h2 : cx - h2;
-            cy = (o & 2u) ? cy + h2 : cy - h2;
-            if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-            ext = h2;
-        }
-        // the owned nodes, all lanes of a wave in step
-        for (int k = 0; k < cnt; ++k) {
-            const int L = lo + k;
-            const uint64_t nd = (uint64_t)nbase + (uint64_t)k;
-            if (nd >= cap) {
-                atomicOr(&d.st->err, kErrCapacity);
-                break;
-            }
-            const bool leaf = L == hi;
-            uint64_t last = i;
-            bool spans = false;
-            if (!leaf) {
-                const uint64_t bound = min(end, b1);
-                if (L <= lmax) {
-                    last = b0 + range_end_lds<DIM>(s_keys, (uint32_t)(i - b0),
-                                                   (uint32_t)(bound - b0), lmax, L);
-                } else {
-                    // inside a run of equal keys (a few bodies): linear
-                    while (last + 1 < bound && same_at(s_keys[last + 1 - b0], last + 1, L)) ++last;
-                }
-                if (last + 1 == b1 && b1 < end && same_at(s_next, b1, L)) {
-                    spans = true;
-                    const uint32_t nx = lvl_first[(uint64_t)(tile + 1) * kTopSlots + L];
-                    last = (nx == ~0u ? total : (uint64_t)nx) - 1;
-                }
-            }
-            NodeHot h;
-            h.cx = cx; h.cy = cy; h.cz = cz;
-            if (leaf) {
-                const uint64_t j = i - d.off;
-                h.mx = d.dx[j]; h.my = d.dy[j]; h.mz = d.dz[j];
-                h.mass = d.dm[j];
-                // the reference merges bodies within 1e-9 on every axis
-                // (octree.rs:69-80); the device reports them instead
-                if (j + 1 < d.n && fabs(h.mx - d.dx[j + 1]) < 1e-9 &&
-                    fabs(h.my - d.dy[j + 1]) < 1e-9 && fabs(h.mz - d.dz[j + 1]) < 1e-9)
-                    atomicOr(&d.st->err, kErrCoincident);
-            } else {
-                h.mx = h.my = h.mz = 0.0;
-                h.mass = 0.0;
-            }
-            const uint64_t nxt = last + 1;
-            if (!spans)
-                h.skip = s_first[nxt - b0];
-            else if (nxt >= total)
-                h.skip = tile_base[tiles];
-            else
-                h.skip = node_local[nxt] + tile_base[nxt / kNodeTile];
-            h.level = (uint16_t)L;
-            h.octant = L > 0 ? (uint8_t)key_digit<DIM>(key, kx, lmax, L - 1) : 0;
-            h.pad = 0;
-            hot[nd] = h;
-            if (in_lds) {
-                const uint32_t loc = (uint32_t)nd - n0;
-                s_tag[loc] = (uint8_t)L | (leaf || spans ? 0 : kTagInternal);
-                s_skip[loc] = (uint16_t)min(h.skip - n0, nloc);
-                if (leaf) {
-                    s_m[loc] = h.mass;
-                    s_x[loc] = h.mx; s_y[loc] = h.my; s_z[loc] = h.mz;
-                }
-            }
-            if (spans && L < kTopSlots) {
-                top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
-                top_last[(uint64_t)tile * kTopSlots + L] = (uint32_t)last;
-                atomicMax(&s_topl, (uint32_t)L + 1u);
-            }
-            if (k + 1 < cnt) {
-                const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
-                const double h2 = half(ext);
-                cx = (o & 1u) ? cx + h2 : cx - h2;
-                cy = (o & 2u) ? cy + h2 : cy - h2;
-                if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-                ext = h2;
-            }
-        }
-        atomicMax(&s_maxl, (uint32_t)hi);
+    // Bodies: staging, the node -> owner table (LDS tiles), checks, leaves.
+    for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
+        const uint64_t i = b0 + li;
+        const uint64_t key = skeys_g[i];
+        const uint32_t lo = lo_level[i];
+        const uint32_t f = node_local[i] + n0;
+        const uint32_t fn = li + 1u < nb ? node_local[i + 1] + n0 : nend;
+        s_keys[li] = key;
+        s_lo[li] = (uint8_t)lo;
+        s_first[li] = f;
+        first_node[i] = f;
+        if (in_lds)
+            for (uint32_t nd = f; nd < fn && nd < n1; ++nd) s_own[nd - n0] = (uint16_t)li;
+        const uint64_t j = i - d.off;
+        const double x = d.dx[j], y = d.dy[j], z = d.dz[j], m = d.dm[j];
+        const bool more = j + 1 < d.n;
+        const double x1 = more ? d.dx[j + 1] : 0.0, y1 = more ? d.dy[j + 1] : 0.0,
+                     z1 = more ? d.dz[j + 1] : 0.0;
+        const uint32_t hi = lo + (fn - f) - 1u;
+        if (fn > cap) atomicOr(&d.st->err, kErrCapacity);
+        // the reference merges bodies within 1e-9 on every axis
+        // (octree.rs:69-80); the device reports them instead
+        if (more && fabs(x - x1) < 1e-9 && fabs(y - y1) < 1e-9 && fabs(z - z1) < 1e-9)
+            atomicOr(&d.st->err, kErrCoincident);
+        atomicMax(&s_maxl, hi);
         // read first: once the running maximum is reached almost no lane
         // needs the atomic (a stale read only costs an extra atomic)
-        if (d.count && (uint32_t)hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(&d.st->max_level, (uint32_t)hi);
+        if (d.count && hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&d.st->max_level, hi);
+        if (in_lds) {
+            if (fn - 1u < n1) {
+                const uint32_t leaf = fn - 1u - n0;
+                s_m[leaf] = m;
+                s_x[leaf] = x;
+                s_y[leaf] = y;
+                s_z[leaf] = z;
+            }
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0 && s_topl) atomicMax(&sc->top_levels, s_topl);
-    // internal nodes sit above the deepest leaf
-    const int maxl = (int)min(s_maxl, (uint32_t)kTopSlots);
+
+    // Do keys kj (body j) and key (with second word kx) share `level` levels?
+    auto same_at = [&](uint64_t kj, uint64_t j, uint64_t key, uint64_t kx, int level) -> bool {
+        if (level <= lmax) return key_prefix<DIM>(kj, lmax, level) == key_prefix<DIM>(key, lmax, level);
+        return kj == key && k2_prefix<DIM>(k2[j], lmax, level) == k2_prefix<DIM>(kx, lmax, level);
+    };
+    // Skip of the internal node at level L owned by body i (key): the first
+    // node after its range; its last body, and whether it leaves the tile.
+    auto internal_skip = [&](uint64_t i, uint64_t key, int L, uint64_t& last, bool& spans) -> uint32_t {
+        // levels past lmax exist only for bodies whose keys tie; they read
+        // the second key word (a run of equal keys is a few bodies: linear)
+        const uint64_t kx = L > lmax ? k2[i] : 0ull;
+        last = i;
+        spans = false;
+        if (L <= lmax)
+            last = b0 + range_end_lds<DIM>(s_keys, (uint32_t)(i - b0), nb, lmax, L);
+        else
+            while (last + 1 < b1 && same_at(s_keys[last + 1 - b0], last + 1, key, kx, L)) ++last;
+        if (last + 1 == b1 && b1 < total && same_at(s_next, b1, key, kx, L)) {
+            spans = true;
+            const uint32_t nx = lvl_first[(uint64_t)(tile + 1) * kTopSlots + L];
+            last = (nx == ~0u ? total : (uint64_t)nx) - 1;
+            const uint64_t nxt = last + 1;
+            return nxt >= total ? tile_base[tiles]
+                                : node_local[nxt] + tile_base[tile_of(desc, seg_off, nseg, nxt)];
+        }
+        return s_first[last + 1 - b0];
+    };
+    // a tile-spanning node is the only one of its level that starts in the
+    // tile and leaves it
+    auto record_span = [&](uint32_t nd, int L, uint64_t last, uint32_t skip) {
+        if (L >= kTopSlots) return;
+        top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
+        top_last[(uint64_t)tile * kTopSlots + L] = (uint32_t)last;
+        s_spanskip[L] = skip;
+        atomicMax(&s_topl, (uint32_t)L + 1u);
+    };
+
     if (in_lds) {
+        // Nodes: level, range, skip and tag; spanning nodes get zero mass
+        // here and theirs from k_com_span.
+        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
+            const uint32_t li = s_own[k];
+            const uint32_t nbase = s_first[li], nnext = s_first[li + 1];
+            const int L = (int)s_lo[li] + (int)(n0 + k - nbase);
+            const bool leaf = n0 + k + 1u == nnext;
+            bool spans = false;
+            uint32_t skip = nnext;
+            if (!leaf) {
+                uint64_t last;
+                skip = internal_skip(b0 + li, s_keys[li], L, last, spans);
+                s_m[k] = 0.0;
+                s_x[k] = 0.0;
+                s_y[k] = 0.0;
+                s_z[k] = 0.0;
+                if (spans) record_span(n0 + k, L, last, skip);
+            }
+            s_skip[k] = (uint16_t)min(skip - n0, nloc);
+            s_tag[k] = (uint8_t)L | (leaf || spans ? 0 : kTagInternal);
+        }
+        __syncthreads();
+        // internal nodes sit above the deepest leaf
+        const int maxl = (int)min(s_maxl, (uint32_t)kTopSlots);
         for (int L = maxl - 1; L >= 0; --L) {
             const uint8_t want = (uint8_t)L | kTagInternal;
             for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
@@ -966,18 +996,97 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             }
             __syncthreads();
         }
-        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
-            if ((s_tag[k] & kTagInternal) && s_tag[k] != kTagNone) {
-                NodeHot& h = hot[n0 + k];
-                h.mx = s_x[k];
-                h.my = s_y[k];
-                h.mz = s_z[k];
-                h.mass = s_m[k];
+        // Records, chunk q of node k per thread; the centre by the
+        // reference's descent from the segment's root.
+        for (uint32_t c = threadIdx.x; c < 4u * nloc; c += kNodeThreads) {
+            const uint32_t k = c >> 2, q = c & 3u;
+            const uint32_t li = s_own[k];
+            const int L = s_tag[k] & kTagLevel;
+            const uint64_t key = s_keys[li];
+            const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
+            uint4 v;
+            if (q < 2u) {
+                double ext = ext0, cx = 0.0, cy = 0.0, cz = 0.0;
+                for (int l = 0; l < L; ++l) {
+                    const uint32_t o = key_digit<DIM>(key, kx, lmax, l);
+                    const double h2 = half(ext);
+                    cx = (o & 1u) ? cx + h2 : cx - h2;
+                    cy = (o & 2u) ? cy + h2 : cy - h2;
+                    if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+                    ext = h2;
+                }
+                v = q == 0u ? pack2(cx, cy) : pack2(cz, s_x[k]);
+            } else if (q == 2u) {
+                v = pack2(s_y[k], s_z[k]);
+            } else {
+                const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
+                // a skip out of the tile: the next tile's first node, or the
+                // recorded one of a spanning node (neither leaf nor internal)
+                const uint32_t sk = s_skip[k];
+                const bool spanning = !(s_tag[k] & kTagInternal) && n0 + k + 1u != s_first[li + 1];
+                const uint32_t skip = sk < nloc ? n0 + sk : spanning ? s_spanskip[L] : nend;
+                v = pack_tail(s_m[k], skip, (uint32_t)L, oct);
             }
+            reinterpret_cast<uint4*>(hot + n0 + k)[q] = v;
         }
     } else {
-        // more nodes than LDS holds: the same level loop on the global
-        // records, made visible across the block by device-scope fences
+        // More nodes than LDS holds: body by body into the global records.
+        for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
+            const uint64_t i = b0 + li;
+            const uint64_t key = s_keys[li];
+            const int lo = s_lo[li];
+            const uint32_t nbase = s_first[li];
+            const int cnt = (int)(s_first[li + 1] - nbase);
+            const int hi = lo + cnt - 1;
+            const uint64_t kx = hi > lmax ? k2[i] : 0ull;
+            double ext = ext0;
+            double cx = 0.0, cy = 0.0, cz = 0.0;
+            for (int L = 0; L < lo; ++L) {
+                const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
+                const double h2 = half(ext);
+                cx = (o & 1u) ? cx + h2 : cx - h2;
+                cy = (o & 2u) ? cy + h2 : cy - h2;
+                if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+                ext = h2;
+            }
+            for (int k = 0; k < cnt; ++k) {
+                const int L = lo + k;
+                const uint64_t nd = (uint64_t)nbase + (uint64_t)k;
+                if (nd >= cap) break;  // reported above
+                NodeHot h;
+                h.cx = cx; h.cy = cy; h.cz = cz;
+                if (L == hi) {
+                    const uint64_t j = i - d.off;
+                    h.mx = d.dx[j]; h.my = d.dy[j]; h.mz = d.dz[j];
+                    h.mass = d.dm[j];
+                    h.skip = s_first[li + 1];
+                } else {
+                    h.mx = h.my = h.mz = 0.0;
+                    h.mass = 0.0;
+                    uint64_t last;
+                    bool spans;
+                    h.skip = internal_skip(i, key, L, last, spans);
+                    if (spans) record_span((uint32_t)nd, L, last, h.skip);
+                }
+                h.level = (uint16_t)L;
+                h.octant = L > 0 ? (uint8_t)key_digit<DIM>(key, kx, lmax, L - 1) : 0;
+                h.pad = 0;
+                store_node(hot + nd, h);
+                if (k + 1 < cnt) {
+                    const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
+                    const double h2 = half(ext);
+                    cx = (o & 1u) ? cx + h2 : cx - h2;
+                    cy = (o & 2u) ? cy + h2 : cy - h2;
+                    if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+                    ext = h2;
+                }
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        // the same level loop on the global records, made visible across the
+        // block by device-scope fences
+        const int maxl = (int)min(s_maxl, (uint32_t)kTopSlots);
         for (int L = maxl - 1; L >= 0; --L) {
             for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kNodeThreads) {
                 // internal and inside the tile: its subtree ends by the tile's last node
@@ -987,9 +1096,8 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             __threadfence();
             __syncthreads();
         }
-        __threadfence();
-        __syncthreads();
     }
+    if (threadIdx.x == 0 && s_topl) atomicMax(&sc->top_levels, s_topl);
 
     // Partial mass sums for the tile-spanning nodes (k_com_span): the tile
     // total; per level L, the suffix from the first body of the level-L
@@ -998,7 +1106,6 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     // end of the level-L ancestor of the tile's first body, where that one
     // started before the tile and ends in it (in_part). Bodies come in
     // contiguous pairs per thread; every sum has a fixed association.
-    const uint32_t nb = (uint32_t)(b1 - b0);
     // mass sums and level of local body li (zero / ~0 when absent)
     auto body_sum = [&](uint32_t li) -> Sum4 {
         if (li >= nb) return Sum4{0.0, 0.0, 0.0, 0.0};
@@ -1006,7 +1113,6 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
         if (leaf >= n1) return Sum4{0.0, 0.0, 0.0, 0.0};
         double m, x, y, z;
         if (in_lds) {
-            if (s_tag[leaf - n0] == kTagNone) return Sum4{0.0, 0.0, 0.0, 0.0};
             m = s_m[leaf - n0]; x = s_x[leaf - n0]; y = s_y[leaf - n0]; z = s_z[leaf - n0];
         } else {
             m = hot[leaf].mass; x = hot[leaf].mx; y = hot[leaf].my; z = hot[leaf].mz;
@@ -1079,10 +1185,11 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
 // its first body, the totals of the tiles in between, and the prefix of the
 // tile where it ends (or that tile's total when it ends with the tile).
 __global__ __launch_bounds__(kBlock) void k_com_span(
-    NodeHot* __restrict__ hot, const uint32_t* __restrict__ top_last,
-    const uint64_t* __restrict__ top_slot, const Sum4* __restrict__ out_part,
-    const Sum4* __restrict__ in_part, const Sum4* __restrict__ tot_part,
-    const StepCounters* __restrict__ sc, uint64_t total) {
+    const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off, uint32_t nseg,
+    const uint32_t* __restrict__ tile_start, NodeHot* __restrict__ hot,
+    const uint32_t* __restrict__ top_last, const uint64_t* __restrict__ top_slot,
+    const Sum4* __restrict__ out_part, const Sum4* __restrict__ in_part,
+    const Sum4* __restrict__ tot_part, const StepCounters* __restrict__ sc) {
     const uint32_t t = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t gen = sc->gen;
@@ -1093,7 +1200,7 @@ __global__ __launch_bounds__(kBlock) void k_com_span(
         if ((uint32_t)(sv >> 32) != gen) continue;
         const uint32_t nd = (uint32_t)sv;
         const uint64_t last = top_last[(uint64_t)t * kTopSlots + L];
-        const uint32_t t1 = (uint32_t)(last / kNodeTile);
+        const uint32_t t1 = tile_of(desc, seg_off, nseg, last);
         Sum4 a{0.0, 0.0, 0.0, 0.0};
         for (uint32_t tt = t + 1u + (uint32_t)lane; tt < t1; tt += 64u) a.add(tot_part[tt]);
 #pragma unroll
@@ -1109,8 +1216,8 @@ __global__ __launch_bounds__(kBlock) void k_com_span(
         if (lane == 0) {
             Sum4 r = out_part[(uint64_t)t * kTopSlots + L];
             r.add(a);
-            const uint64_t t1_end = min((uint64_t)(t1 + 1) * kNodeTile, total);
-            r.add(last + 1 == t1_end ? tot_part[t1] : in_part[(uint64_t)t1 * kTopSlots + L]);
+            r.add(last + 1 == tile_start[t1 + 1] ? tot_part[t1]
+                                                 : in_part[(uint64_t)t1 * kTopSlots + L]);
             const double inv = 1.0 / r.m;
             hot[nd].mx = r.x * inv;
             hot[nd].my = r.y * inv;
