@@ -514,14 +514,14 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         const int a = p & 1, b = a ^ 1;
         if (p > 0)
             launch(e, cap, "sort_count", [&] {
-                k_sort_count<<<sort_tiles, kBlock, 0, st>>>(e->keys[a], total, 8 * p, e->sort_cnt,
-                                                            sort_tiles);
+                k_sort_count<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[a], total, 8 * p,
+                                                                  e->sort_cnt, sort_tiles);
             });
         launch(e, cap, "sort_scan", [&] {
             k_sort_scan<<<256, kScanThreads, 0, st>>>(e->sort_cnt, sort_tiles, e->sort_tot);
         });
         launch(e, cap, "sort_scatter", [&] {
-            k_sort_scatter<<<sort_tiles, kBlock, 0, st>>>(e->keys[a], e->vals[a], e->keys[b],
+            k_sort_scatter<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[a], e->vals[a], e->keys[b],
                                                           e->vals[b], total, 8 * p, e->sort_cnt,
                                                           sort_tiles, e->sort_tot, e->err_global);
         });

@@ -8,8 +8,9 @@
 namespace pg {
 
 constexpr int kBlock = 256;        // threads per block (4 wave64)
-constexpr int kSortItems = 8;      // keys per thread in a radix-sort tile
-constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kSortTile = 2048;    // keys per radix-sort tile
+constexpr int kSortThreads = 512;  // threads per radix-sort tile
+constexpr int kSortItems = kSortTile / kSortThreads;  // keys per thread
 constexpr int kScanItems = 2;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
@@ -26,6 +27,7 @@ static_assert(kScanTile == kNodeTile, "node offsets and node building share thei
 static_assert(kNodeTile == 2 * kNodeThreads, "the partial sums take two bodies per thread");
 static_assert(kTopSlots <= kBlock, "per-level tables are filled by one thread per level");
 static_assert(kSortTile % kKeyThreads == 0 && kKeyThreads >= 256, "whole key tiles, a digit per thread");
+static_assert(kSortTile % kSortThreads == 0 && kSortThreads >= 256, "whole sort tiles, a digit per thread");
 
 // Device-side error bits (sticky per segment), mapped to pg_status on the host.
 enum : uint32_t {

@@ -291,26 +291,27 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
 // cross-XCD round trips when every tile of a pass is resident at once.)
 
 // (1) Digit counts of one tile: cnt[digit * tiles + tile].
-__global__ __launch_bounds__(kBlock) void k_sort_count(const uint64_t* __restrict__ kin, uint64_t n,
-                                                       int shift, uint32_t* __restrict__ cnt,
-                                                       uint32_t tiles) {
+__global__ __launch_bounds__(kSortThreads) void k_sort_count(const uint64_t* __restrict__ kin,
+                                                             uint64_t n, int shift,
+                                                             uint32_t* __restrict__ cnt,
+                                                             uint32_t tiles) {
     __shared__ uint32_t s_h[256];
-    s_h[threadIdx.x] = 0;
+    if (threadIdx.x < 256) s_h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
     uint64_t key[kSortItems];
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const uint64_t idx = base + (uint64_t)k * kBlock + threadIdx.x;
+        const uint64_t idx = base + (uint64_t)k * kSortThreads + threadIdx.x;
         key[k] = idx < n ? kin[idx] : 0ull;
     }
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const uint64_t idx = base + (uint64_t)k * kBlock + threadIdx.x;
+        const uint64_t idx = base + (uint64_t)k * kSortThreads + threadIdx.x;
         wave_count_digit(s_h, idx < n ? (uint32_t)((key[k] >> shift) & 255u) : 256u);
     }
     __syncthreads();
-    cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
+    if (threadIdx.x < 256) cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
 // (2) One block per digit: exclusive scan of the digit's per-tile counts (in
@@ -341,19 +342,20 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict
 // LDS counters; a key's output index is the digit's global base (scan of the
 // digit totals) + the tile's offset within the digit (pass 2) + its rank in
 // the tile. The tile is reordered in LDS and written out in runs per digit.
-__global__ __launch_bounds__(kBlock) void k_sort_scatter(
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
     uint32_t* __restrict__ vout, uint64_t n, int shift, const uint32_t* __restrict__ cnt,
     uint32_t tiles, const uint32_t* __restrict__ tot, uint32_t* err) {
-    __shared__ uint32_t s_wcnt[kBlock / 64][256];
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t s_wcnt[kWaves][256];
     __shared__ uint32_t s_lstart[256];
     __shared__ uint32_t s_gbase[256];
-    __shared__ uint32_t s_scan[kBlock / 64];
+    __shared__ uint32_t s_scan[kWaves];
     __shared__ uint64_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < (kBlock / 64) * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
+    for (int i = tid; i < kWaves * 256; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = blockIdx.x;
     const uint64_t base = (uint64_t)tile * kSortTile;
@@ -391,20 +393,25 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
         rank[k] = old + (uint32_t)__popcll(peers & lt);
     }
     __syncthreads();
-    // Per digit (one thread each): exclusive offsets over waves, tile count.
+    // Per digit (the first 256 threads): exclusive offsets over waves, tile count.
     const uint32_t d = tid;
-    uint32_t c = 0;
+    uint32_t c = 0, t = 0;
+    if (tid < 256) {
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
-        const uint32_t cw = s_wcnt[w][d];
-        s_wcnt[w][d] = c;
-        c += cw;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t cw = s_wcnt[w][d];
+            s_wcnt[w][d] = c;
+            c += cw;
+        }
+        t = tot[d];
     }
     uint32_t dummy;
-    const uint32_t lstart = block_exclusive_scan(c, s_scan, &dummy);
-    const uint32_t gbase = block_exclusive_scan(tot[d], s_scan, &dummy);
-    s_lstart[d] = lstart;
-    s_gbase[d] = gbase + cnt[(uint64_t)d * tiles + tile] - lstart;
+    const uint32_t lstart = block_exclusive_scan<kSortThreads>(c, s_scan, &dummy);
+    const uint32_t gbase = block_exclusive_scan<kSortThreads>(t, s_scan, &dummy);
+    if (tid < 256) {
+        s_lstart[d] = lstart;
+        s_gbase[d] = gbase + cnt[(uint64_t)d * tiles + tile] - lstart;
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
@@ -416,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
     }
     __syncthreads();
     const uint32_t valid_n = (uint32_t)min((uint64_t)kSortTile, n - base);
-    for (uint32_t p = tid; p < valid_n; p += kBlock) {
+    for (uint32_t p = tid; p < valid_n; p += kSortThreads) {
         const uint64_t k = s_keys[p];
         const uint32_t o = s_gbase[(k >> shift) & 255u] + p;
         if (o < n) {
