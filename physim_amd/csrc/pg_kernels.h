@@ -825,9 +825,9 @@ __device__ __forceinline__ void store_node(NodeHot* p, const NodeHot& h) {
 // its range, skip and tag; an internal node whose bodies all lie in the tile
 // has its whole subtree there, so its mass and centre of mass are reduced in
 // LDS, deepest level first, from its children in octant order (the
-// arithmetic of com_from_children); last, every record is written once, a
-// 16-byte chunk per thread, so that a wave stores 1 KiB of consecutive
-// records per instruction. A tile with more nodes (clustered bodies) builds
+// arithmetic of com_from_children); last, every record is written once, by
+// the thread of its node (one descent per node). A tile with more nodes
+// (clustered bodies) builds
 // them body by body into the global records and runs the same level loop
 // there.
 //
@@ -978,12 +978,22 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             s_tag[k] = (uint8_t)L | (leaf || spans ? 0 : kTagInternal);
         }
         __syncthreads();
-        // internal nodes sit above the deepest leaf
+        // internal nodes sit above the deepest leaf; a thread's (at most
+        // kNodeLds / kNodeThreads) node tags stay in registers
         const int maxl = (int)min(s_maxl, (uint32_t)kTopSlots);
+        constexpr int kPerThread = kNodeLds / kNodeThreads + (kNodeLds % kNodeThreads ? 1 : 0);
+        uint8_t mytag[kPerThread];
+#pragma unroll
+        for (int r = 0; r < kPerThread; ++r) {
+            const uint32_t k = threadIdx.x + (uint32_t)r * kNodeThreads;
+            mytag[r] = k < nloc ? s_tag[k] : (uint8_t)0;
+        }
         for (int L = maxl - 1; L >= 0; --L) {
             const uint8_t want = (uint8_t)L | kTagInternal;
-            for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
-                if (s_tag[k] != want) continue;
+#pragma unroll
+            for (int r = 0; r < kPerThread; ++r) {
+                if (mytag[r] != want) continue;
+                const uint32_t k = threadIdx.x + (uint32_t)r * kNodeThreads;
                 double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
                 const uint32_t send = s_skip[k];
                 for (uint32_t c = k + 1u; c < send;) {
@@ -1003,38 +1013,34 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
             }
             __syncthreads();
         }
-        // Records, chunk q of node k per thread; the centre by the
-        // reference's descent from the segment's root.
-        for (uint32_t c = threadIdx.x; c < 4u * nloc; c += kNodeThreads) {
-            const uint32_t k = c >> 2, q = c & 3u;
+        // Records, one per thread: the centre by the reference's descent
+        // from the segment's root, the rest from LDS.
+        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
             const uint32_t li = s_own[k];
-            const int L = s_tag[k] & kTagLevel;
+            const uint8_t tag = s_tag[k];
+            const int L = tag & kTagLevel;
             const uint64_t key = s_keys[li];
             const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
-            uint4 v;
-            if (q < 2u) {
-                double ext = ext0, cx = 0.0, cy = 0.0, cz = 0.0;
-                for (int l = 0; l < L; ++l) {
-                    const uint32_t o = key_digit<DIM>(key, kx, lmax, l);
-                    const double h2 = half(ext);
-                    cx = (o & 1u) ? cx + h2 : cx - h2;
-                    cy = (o & 2u) ? cy + h2 : cy - h2;
-                    if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-                    ext = h2;
-                }
-                v = q == 0u ? pack2(cx, cy) : pack2(cz, s_x[k]);
-            } else if (q == 2u) {
-                v = pack2(s_y[k], s_z[k]);
-            } else {
-                const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
-                // a skip out of the tile: the next tile's first node, or the
-                // recorded one of a spanning node (neither leaf nor internal)
-                const uint32_t sk = s_skip[k];
-                const bool spanning = !(s_tag[k] & kTagInternal) && n0 + k + 1u != s_first[li + 1];
-                const uint32_t skip = sk < nloc ? n0 + sk : spanning ? s_spanskip[L] : nend;
-                v = pack_tail(s_m[k], skip, (uint32_t)L, oct);
+            double ext = ext0, cx = 0.0, cy = 0.0, cz = 0.0;
+            for (int l = 0; l < L; ++l) {
+                const uint32_t o = key_digit<DIM>(key, kx, lmax, l);
+                const double h2 = half(ext);
+                cx = (o & 1u) ? cx + h2 : cx - h2;
+                cy = (o & 2u) ? cy + h2 : cy - h2;
+                if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+                ext = h2;
             }
-            reinterpret_cast<uint4*>(hot + n0 + k)[q] = v;
+            const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
+            // a skip out of the tile: the next tile's first node, or the
+            // recorded one of a spanning node (neither leaf nor internal)
+            const uint32_t sk = s_skip[k];
+            const bool spanning = !(tag & kTagInternal) && n0 + k + 1u != s_first[li + 1];
+            const uint32_t skip = sk < nloc ? n0 + sk : spanning ? s_spanskip[L] : nend;
+            uint4* rec = reinterpret_cast<uint4*>(hot + n0 + k);
+            rec[0] = pack2(cx, cy);
+            rec[1] = pack2(cz, s_x[k]);
+            rec[2] = pack2(s_y[k], s_z[k]);
+            rec[3] = pack_tail(s_m[k], skip, (uint32_t)L, oct);
         }
     } else {
         // More nodes than LDS holds: body by body into the global records.
