@@ -88,6 +88,12 @@ int pg_engine_memory(pg_engine* eng, uint64_t* used, uint64_t* budget);
 uint64_t pg_sim_bytes(uint64_t n, int32_t kind);
 /* Capture repeated steps in hipGraphs (default on). */
 int pg_engine_set_graphs(pg_engine* eng, int enable);
+/* Key sorting of the tree build: a sample sort into at most max_buckets
+ * buckets (<= 1024; a larger batch, or max_buckets = 0, uses the 8-bit LSD
+ * radix sort), each sorted in LDS when it holds at most bucket_cap keys
+ * (<= 4096) and in HBM otherwise. Both give the same order. Defaults 1024,
+ * 4096. A tuning and testing control; no reference counterpart. */
+int pg_engine_set_sort(pg_engine* eng, uint32_t max_buckets, uint32_t bucket_cap);
 /* Per-kernel HIP-event timing of every launch (eager mode, no graphs). */
 int pg_engine_set_profiling(pg_engine* eng, int enable);
 /* Accumulated per-kernel times. names: '\n'-separated kernel names

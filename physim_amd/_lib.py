@@ -112,6 +112,7 @@ def lib():
         "pg_sim_bytes": (U64, [U64, ctypes.c_int32]),
         "pg_engine_set_graphs": (I, [P, I]),
         "pg_engine_set_profiling": (I, [P, I]),
+        "pg_engine_set_sort": (I, [P, U32, U32]),
         "pg_engine_profile_read": (I, [P, ctypes.c_char_p, SZ, P, P, U32, ctypes.POINTER(U32)]),
         "pg_engine_profile_reset": (I, [P]),
         "pg_engine_synchronize": (I, [P]),
@@ -257,6 +258,12 @@ class Engine:
 
     def set_profiling(self, on):
         check(lib().pg_engine_set_profiling(self.h, int(on)))
+
+    def set_sort(self, max_buckets=1024, bucket_cap=4096):
+        """Key sort of the tree build: sample sort into at most max_buckets
+        buckets (0: the LSD radix sort), buckets above bucket_cap keys
+        sorted in HBM instead of LDS. Both orders are the same."""
+        check(lib().pg_engine_set_sort(self.h, int(max_buckets), int(bucket_cap)))
 
     def profile_reset(self):
         check(lib().pg_engine_profile_reset(self.h))

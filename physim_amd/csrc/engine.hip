@@ -76,6 +76,13 @@ struct pg_engine {
     uint8_t* lo_level = nullptr;
     uint32_t* sort_cnt = nullptr;  // per-tile digit counts, digit-major
     uint32_t* sort_tot = nullptr;  // per-digit totals of one pass
+    uint16_t* bkt = nullptr;       // sample sort: bucket of each key
+    uint64_t* spl = nullptr;       // sample sort: splitters
+    uint32_t* bcount = nullptr;    // sample sort: keys per bucket
+    uint32_t* bfill = nullptr;     // sample sort: reserved keys per bucket
+    uint32_t* bbase = nullptr;     // sample sort: first key of each bucket
+    uint32_t max_buckets = kMaxBuckets;  // larger batches use the radix sort
+    uint32_t bucket_cap = kBucketCap;    // larger buckets are sorted in HBM
     uint32_t* node_local = nullptr;  // per-body node offset within its tile
     uint32_t* tile_tot = nullptr;    // nodes per tile
     uint32_t* tile_base = nullptr;   // first node of each tile (+ total)
@@ -122,7 +129,7 @@ struct pg_engine {
     // last build (for tree / Morton export)
     pg_sim* last_sim = nullptr;
     uint64_t last_total = 0;
-    int last_passes = 0;
+    int last_sorted = 0;   // key buffer holding the sorted keys
     int last_dim = 3;
     int last_lmax = 21;
 
@@ -254,6 +261,23 @@ extern "C" int pg_engine_set_graphs(pg_engine* e, int enable) {
     return PG_OK;
 }
 
+extern "C" int pg_engine_set_sort(pg_engine* e, uint32_t max_buckets, uint32_t bucket_cap) {
+    if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    if (max_buckets > kMaxBuckets || bucket_cap > kBucketCap)
+        return pg_set_error(PG_ERR_ARG, "sample sort limits are %d buckets of %d keys in LDS",
+                            kMaxBuckets, kBucketCap);
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    // captured steps hold the previous choice
+    for (auto& g : e->graph_cache)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    e->graph_cache.clear();
+    e->max_buckets = max_buckets;
+    e->bucket_cap = bucket_cap;
+    return PG_OK;
+}
+
 extern "C" int pg_engine_set_profiling(pg_engine* e, int enable) {
     if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
     e->profiling = enable != 0;
@@ -357,6 +381,9 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_v0 = take(bodies * 4), o_v1 = take(bodies * 4);
     const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies);
     const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
+    const uint64_t o_bk = take(bodies * 2), o_sp = take(kMaxBuckets * 8);
+    const uint64_t o_bc = take(kMaxBuckets * 4), o_bf = take(kMaxBuckets * 4);
+    const uint64_t o_bb = take(kMaxBuckets * 4);
     const uint64_t o_nl = take((bodies + 1) * 4), o_tt = take((node_tiles + 2) * 4);
     const uint64_t o_tb = take((node_tiles + 2) * 4), o_ts = take((node_tiles + 2) * 4);
     const uint64_t o_op = take((node_tiles + 2) * kTopSlots * sizeof(Sum4));
@@ -398,6 +425,11 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->lo_level = (uint8_t*)(b + o_lo);
     e->sort_cnt = (uint32_t*)(b + o_ss);
     e->sort_tot = (uint32_t*)(b + o_st);
+    e->bkt = (uint16_t*)(b + o_bk);
+    e->spl = (uint64_t*)(b + o_sp);
+    e->bcount = (uint32_t*)(b + o_bc);
+    e->bfill = (uint32_t*)(b + o_bf);
+    e->bbase = (uint32_t*)(b + o_bb);
     e->node_local = (uint32_t*)(b + o_nl);
     e->tile_tot = (uint32_t*)(b + o_tt);
     e->tile_base = (uint32_t*)(b + o_tb);
@@ -481,6 +513,8 @@ struct BatchPlan {
     int kind = PG_KIND_OCTREE;
     int precision = PG_PREC_MIXED;
     int dim = 3, segbits = 0, lmax = 21, passes = 8;
+    uint32_t buckets = 0;  // sample sort with this many buckets (0: radix sort)
+    uint64_t sort_tag = 0; // identifies the batch (sims, sizes) for the sample sort's splitters
     uint32_t nseg = 0;
     uint64_t total = 0;
     uint32_t nblocks = 0;
@@ -508,26 +542,50 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "keys", [&] {
         k_keys<DIM><<<sort_tiles, kKeyThreads, 0, st>>>(
             desc, e->seg_off, bp.nseg, total, bp.segbits, bp.lmax, e->ext_partial, e->sc,
-            INTEG ? 1 : 0, e->keys[0], e->vals[0], e->sort_cnt, sort_tiles);
+            INTEG ? 1 : 0, bp.buckets ? bp.sort_tag : 0ull, e->keys[0], e->vals[0], e->sort_cnt,
+            sort_tiles);
     });
-    for (int p = 0; p < bp.passes; ++p) {
-        const int a = p & 1, b = a ^ 1;
-        if (p > 0)
-            launch(e, cap, "sort_count", [&] {
-                k_sort_count<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[a], total, 8 * p,
-                                                                  e->sort_cnt, sort_tiles);
+    if (bp.buckets) {
+        const uint32_t nb = bp.buckets;
+        launch(e, cap, "splitters", [&] {
+            k_splitters<<<1, kSplitThreads, 0, st>>>(e->keys[1], e->keys[0], total, nb, e->spl,
+                                                     e->bcount, e->bfill, e->sc);
+        });
+        launch(e, cap, "bucket_count", [&] {
+            k_bucket_count<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[0], total, e->spl, nb,
+                                                                e->bkt, e->bcount);
+        });
+        launch(e, cap, "bucket_scatter", [&] {
+            k_bucket_scatter<<<sort_tiles, kSortThreads, 0, st>>>(
+                e->keys[0], e->vals[0], e->bkt, total, e->bcount, e->bfill, e->bbase, nb,
+                e->keys[1], e->vals[1]);
+        });
+        launch(e, cap, "bucket_sort", [&] {
+            k_bucket_sort<<<nb, kBucketThreads, 0, st>>>(e->keys[1], e->vals[1], e->keys[0],
+                                                         e->vals[0], e->bcount, e->bbase,
+                                                         e->bucket_cap);
+        });
+    } else {
+        for (int p = 0; p < bp.passes; ++p) {
+            const int a = p & 1, b = a ^ 1;
+            if (p > 0)
+                launch(e, cap, "sort_count", [&] {
+                    k_sort_count<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[a], total, 8 * p,
+                                                                      e->sort_cnt, sort_tiles);
+                });
+            launch(e, cap, "sort_scan", [&] {
+                k_sort_scan<<<256, kScanThreads, 0, st>>>(e->sort_cnt, sort_tiles, e->sort_tot);
             });
-        launch(e, cap, "sort_scan", [&] {
-            k_sort_scan<<<256, kScanThreads, 0, st>>>(e->sort_cnt, sort_tiles, e->sort_tot);
-        });
-        launch(e, cap, "sort_scatter", [&] {
-            k_sort_scatter<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[a], e->vals[a], e->keys[b],
-                                                          e->vals[b], total, 8 * p, e->sort_cnt,
-                                                          sort_tiles, e->sort_tot, e->err_global);
-        });
+            launch(e, cap, "sort_scatter", [&] {
+                k_sort_scatter<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[a], e->vals[a], e->keys[b],
+                                                              e->vals[b], total, 8 * p, e->sort_cnt,
+                                                              sort_tiles, e->sort_tot, e->err_global);
+            });
+        }
     }
-    const uint64_t* skeys = e->keys[bp.passes & 1];
-    uint32_t* svals = e->vals[bp.passes & 1];
+    const int sorted = bp.buckets ? 1 : bp.passes & 1;
+    const uint64_t* skeys = e->keys[sorted];
+    uint32_t* svals = e->vals[sorted];
     launch(e, cap, "ties", [&] {
         k_ties<DIM><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(desc, total, skeys, svals, e->k2,
                                                                    bp.nseg, bp.segbits, bp.lmax);
@@ -661,6 +719,10 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     for (pg_sim* s : sims) bp.total += s->n;
     if (bp.total >= (1ull << 32))
         return pg_set_error(PG_ERR_ARG, "a batch holds at most 2^32-1 bodies");
+    {
+        const uint64_t nb = (bp.total + kBucketKeys - 1) / kBucketKeys;
+        bp.buckets = nb >= 1 && nb <= e->max_buckets ? (uint32_t)nb : 0u;
+    }
     uint64_t nodes = bp.kind == PG_KIND_DIRECT ? 0 : nodes_bound(bp.total, bp.dim, bp.lmax) + bp.nseg;
     if (nodes >= (1ull << 32)) return pg_set_error(PG_ERR_ARG, "tree too large for 32-bit nodes");
     e->h_blk_seg.clear();
@@ -763,6 +825,22 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     HIP_TRY(hipMemcpyAsync(e->tile_start, e->h_tile_start.data(), (bp.ntiles + 1) * 4,
                            hipMemcpyHostToDevice, e->stream));
 
+    // the batch's identity for the sample sort: its splitters may come from
+    // the keys its previous step sorted only when the same batch ran last
+    {
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&](uint64_t v) {
+            h ^= v;
+            h *= 1099511628211ull;
+        };
+        mix(bp.total);
+        mix(bp.nseg);
+        for (pg_sim* s : sims) {
+            mix((uint64_t)(uintptr_t)s);
+            mix(s->n);
+        }
+        bp.sort_tag = h ? h : 1;
+    }
     const bool flips = bp.kind != PG_KIND_DIRECT || integrate;
     uint32_t done = 0;
     if (integrate && e->graphs && !e->profiling && steps >= 2) {
@@ -808,7 +886,7 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
 
     e->last_sim = bp.nseg == 1 && bp.kind != PG_KIND_DIRECT ? sims[0] : nullptr;
     e->last_total = bp.total;
-    e->last_passes = bp.passes;
+    e->last_sorted = bp.buckets ? 1 : bp.passes & 1;
     e->last_dim = bp.dim;
     e->last_lmax = bp.lmax;
 
@@ -1113,7 +1191,7 @@ extern "C" int pg_sim_morton(pg_sim* s, uint64_t* keys, uint32_t* perm, uint64_t
     if (e->last_sim != s) return pg_set_error(PG_ERR_STATE, "the last build was not of this sim");
     if (n != s->n) return pg_set_error(PG_ERR_ARG, "n mismatch");
     HIP_TRY(hipSetDevice(e->device));
-    if (keys) HIP_TRY(hipMemcpy(keys, e->keys[e->last_passes & 1], n * 8, hipMemcpyDeviceToHost));
+    if (keys) HIP_TRY(hipMemcpy(keys, e->keys[e->last_sorted], n * 8, hipMemcpyDeviceToHost));
     if (perm) HIP_TRY(hipMemcpy(perm, s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
     return PG_OK;
 }

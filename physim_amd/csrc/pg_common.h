@@ -15,6 +15,12 @@ constexpr int kScanItems = 2;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr int kKeyThreads = 1024;  // threads of a key-building block (one sort tile)
+constexpr int kBucketKeys = 2048;  // keys per bucket of the sample sort (on average)
+constexpr int kBucketCap = 4096;   // largest bucket sorted in LDS (larger ones: in HBM)
+constexpr int kBucketThreads = 512;   // threads of a bucket-sorting block
+constexpr int kMaxBuckets = 1024;  // sample sort up to kMaxBuckets * kBucketKeys keys
+constexpr int kSampleOver = 4;     // splitter samples per bucket (sampling from unsorted keys)
+constexpr int kSplitThreads = 1024;  // threads of the splitter block
 constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort tiles
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
 constexpr int kNodeTile = 512;     // sorted bodies per node-building block
@@ -87,8 +93,11 @@ struct NodeHot {
 
 // Batch-wide counters reset at the start of every step.
 struct StepCounters {
-    uint32_t gen;           // bumped per step: versions look-back status words
+    uint32_t gen;           // bumped per step: versions tile-spanning node slots
     uint32_t top_levels;    // 1 + deepest level of a tile-spanning node (0: none)
+    uint64_t sort_tag;      // batch whose sorted keys the sample sort left (0: none)
+    uint32_t split_prev;    // this step's splitters may come from those keys
+    uint32_t pad;
 };
 
 }  // namespace pg

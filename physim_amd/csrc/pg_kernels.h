@@ -218,7 +218,7 @@ template <int DIM>
 __global__ __launch_bounds__(kKeyThreads) void k_keys(
     const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off, uint32_t nseg,
     uint64_t total, int segbits, int lmax, const uint64_t* __restrict__ partial,
-    StepCounters* __restrict__ sc, int integrate, uint64_t* __restrict__ keys,
+    StepCounters* __restrict__ sc, int integrate, uint64_t sort_tag, uint64_t* __restrict__ keys,
     uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt, uint32_t tiles) {
     __shared__ uint64_t s_wave[kKeyThreads / 64];
     __shared__ uint32_t s_h[256];
@@ -229,6 +229,10 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->gen += 1;
         sc->top_levels = 0;
+        // the sample sort may take its splitters from the sorted keys of this
+        // batch's previous step (sort_tag 0: the radix sort, which leaves none)
+        sc->split_prev = sort_tag != 0 && sc->sort_tag == sort_tag ? 1u : 0u;
+        sc->sort_tag = sort_tag;
     }
     // the tile's extent when it lies in one segment (block-uniform branch)
     uint64_t mblock = 0;
@@ -435,6 +439,395 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     }
 }
 
+// ------------------------------------------------------------ sample sort
+// The keys arrive in the previous step's Morton order, so they are nearly
+// sorted, and keys taken every kBucketKeys positions split them into buckets
+// of about kBucketKeys keys each. (1) k_splitters: one block takes the
+// quantiles of the batch's previous sorted keys (exact for the previous
+// step, so buckets stay near kBucketKeys as bodies move), or, on a batch's
+// first step, sorts kSampleOver evenly spaced keys per bucket and keeps
+// every kSampleOver-th; (2) k_bucket_count: each key's bucket (the
+// number of splitters <= key) and the bucket sizes; (3) k_bucket_scatter:
+// each tile reserves a range in every bucket it holds and writes its keys
+// there, in runs; (4) k_bucket_sort: each bucket is sorted by (key, index)
+// in LDS (a larger one in HBM, by one block). Runs of equal keys then take
+// (second key word, index) order in k_ties, the order a stable sort by key
+// gives.
+
+__global__ __launch_bounds__(kSplitThreads) void k_splitters(
+    const uint64_t* __restrict__ prev, const uint64_t* __restrict__ keys, uint64_t n, uint32_t nb,
+    uint64_t* __restrict__ spl, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bfill,
+    const StepCounters* __restrict__ sc) {
+    __shared__ uint64_t s[kMaxBuckets * kSampleOver];
+    __shared__ uint32_t s_unsorted;
+    for (uint32_t b = threadIdx.x; b < nb; b += kSplitThreads) {
+        bcount[b] = 0;
+        bfill[b] = 0;
+    }
+    if (nb <= 1) return;
+    if (sc->split_prev) {
+        // quantiles of the previous step's sorted keys, if they are in order
+        if (threadIdx.x == 0) s_unsorted = 0;
+        for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads)
+            s[b] = prev[(uint64_t)(b + 1) * n / nb];
+        __syncthreads();
+        bool bad = false;
+        for (uint32_t b = threadIdx.x; b + 2 < nb; b += kSplitThreads) bad |= s[b] > s[b + 1];
+        if (bad) s_unsorted = 1;
+        __syncthreads();
+        if (!s_unsorted) {
+            for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads) spl[b] = s[b];
+            return;
+        }
+        __syncthreads();
+    }
+    // kSampleOver evenly spaced keys per bucket, padded to a power of two,
+    // bitonic-sorted; every kSampleOver-th is a splitter
+    const uint32_t m = nb * kSampleOver;
+    uint32_t p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    for (uint32_t j = threadIdx.x; j < p2; j += kSplitThreads)
+        s[j] = j < m ? keys[(uint64_t)j * n / m] : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= p2; k <<= 1) {
+        for (uint32_t h = k >> 1; h > 0; h >>= 1) {
+            for (uint32_t i = threadIdx.x; i < p2 / 2; i += kSplitThreads) {
+                const uint32_t a = ((i & ~(h - 1)) << 1) | (i & (h - 1)), b = a + h;
+                const uint64_t x = s[a], y = s[b];
+                if ((x > y) == ((a & k) == 0)) {
+                    s[a] = y;
+                    s[b] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads) spl[b] = s[(b + 1) * kSampleOver];
+}
+
+// Bucket of a key: the number of splitters (sorted, nb - 1 of them) <= key.
+__device__ __forceinline__ uint32_t bucket_of(const uint64_t* s_spl, uint32_t nb, uint64_t key) {
+    uint32_t lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_spl[mid] <= key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_bucket_count(const uint64_t* __restrict__ keys,
+                                                               uint64_t n,
+                                                               const uint64_t* __restrict__ spl,
+                                                               uint32_t nb,
+                                                               uint16_t* __restrict__ bkt,
+                                                               uint32_t* __restrict__ bcount) {
+    __shared__ uint64_t s_spl[kMaxBuckets];
+    __shared__ uint32_t s_h[kMaxBuckets];
+    for (uint32_t b = threadIdx.x; b < nb; b += kSortThreads) {
+        s_h[b] = 0;
+        if (b + 1 < nb) s_spl[b] = spl[b];
+    }
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    uint64_t key[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t idx = base + (uint64_t)k * kSortThreads + threadIdx.x;
+        key[k] = idx < n ? keys[idx] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t idx = base + (uint64_t)k * kSortThreads + threadIdx.x;
+        if (idx < n) {
+            const uint32_t b = bucket_of(s_spl, nb, key[k]);
+            bkt[idx] = (uint16_t)b;
+            atomicAdd(&s_h[b], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += kSortThreads)
+        if (s_h[b]) atomicAdd(&bcount[b], s_h[b]);
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    const uint16_t* __restrict__ bkt, uint64_t n, const uint32_t* __restrict__ bcount,
+    uint32_t* __restrict__ bfill, uint32_t* __restrict__ bbase, uint32_t nb,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    constexpr int kWaves = kSortThreads / 64;
+    static_assert(kMaxBuckets == 2 * kSortThreads, "two buckets per thread in the scans");
+    __shared__ uint32_t s_wcnt[kWaves][kMaxBuckets];
+    __shared__ uint32_t s_lstart[kMaxBuckets];
+    __shared__ uint32_t s_dst[kMaxBuckets];
+    __shared__ uint32_t s_scan[kWaves];
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint16_t s_b[kSortTile];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < kWaves * kMaxBuckets; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    uint64_t key[kSortItems];
+    uint32_t val[kSortItems], bk[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t idx = base + (uint64_t)wave * 64 * kSortItems + k * 64 + lane;
+        const bool valid = idx < n;
+        key[k] = valid ? kin[idx] : 0ull;
+        val[k] = valid ? vin[idx] : 0u;
+        bk[k] = valid ? (uint32_t)bkt[idx] : kMaxBuckets;
+    }
+    // rank within the wave among equal buckets (match-any over 10 bits)
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t b = bk[k];
+        const bool valid = b < (uint32_t)kMaxBuckets;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 10; ++bit) {
+            const bool on = (b >> bit) & 1u;
+            const uint64_t bb = __ballot(on);
+            peers &= on ? bb : ~bb;
+        }
+        if (!valid) peers = 1ull << lane;
+        const int leader = __ffsll((long long)peers) - 1;
+        uint32_t old = 0;
+        if (valid && lane == leader) {
+            old = s_wcnt[wave][b];
+            s_wcnt[wave][b] = old + (uint32_t)__popcll(peers);
+        }
+        old = __shfl(old, leader);
+        rank[k] = old + (uint32_t)__popcll(peers & lt);
+    }
+    __syncthreads();
+    // buckets 2t and 2t + 1: counts over the waves, tile-local starts,
+    // global bucket bases, and the reserved output ranges
+    const uint32_t b0 = 2u * (uint32_t)tid, b1 = b0 + 1u;
+    uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint32_t x0 = s_wcnt[w][b0], x1 = s_wcnt[w][b1];
+        s_wcnt[w][b0] = c0;
+        s_wcnt[w][b1] = c1;
+        c0 += x0;
+        c1 += x1;
+    }
+    const uint32_t n0 = b0 < nb ? bcount[b0] : 0u, n1 = b1 < nb ? bcount[b1] : 0u;
+    uint32_t dummy;
+    const uint32_t l0 = block_exclusive_scan<kSortThreads>(c0 + c1, s_scan, &dummy);
+    const uint32_t g0 = block_exclusive_scan<kSortThreads>(n0 + n1, s_scan, &dummy);
+    s_lstart[b0] = l0;
+    s_lstart[b1] = l0 + c0;
+    s_dst[b0] = c0 ? g0 + atomicAdd(&bfill[b0], c0) : 0u;
+    s_dst[b1] = c1 ? g0 + n0 + atomicAdd(&bfill[b1], c1) : 0u;
+    if (blockIdx.x == 0) {
+        if (b0 < nb) bbase[b0] = g0;
+        if (b1 < nb) bbase[b1] = g0 + n0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t b = bk[k];
+        if (b < (uint32_t)kMaxBuckets) {
+            const uint32_t p = s_lstart[b] + s_wcnt[wave][b] + rank[k];
+            s_keys[p] = key[k];
+            s_vals[p] = val[k];
+            s_b[p] = (uint16_t)b;
+        }
+    }
+    __syncthreads();
+    const uint32_t valid_n = (uint32_t)min((uint64_t)kSortTile, n - base);
+    for (uint32_t p = tid; p < valid_n; p += kSortThreads) {
+        const uint32_t b = s_b[p];
+        const uint32_t o = s_dst[b] + (p - s_lstart[b]);
+        kout[o] = s_keys[p];
+        vout[o] = s_vals[p];
+    }
+}
+
+// Stable ranks of a chunk of up to kBucketChunk keys by an 8-bit digit (256
+// = no key), one block of kBucketThreads threads, item k of thread (wave,
+// lane) at chunk index wave * 256 + k * 64 + lane: pos[k] is the item's index
+// in the chunk ordered by digit; s_dstart[d] and s_dcnt[d] are the chunk's
+// start and count of digit d.
+constexpr int kBucketChunk = 4 * kBucketThreads;
+__device__ __forceinline__ void chunk_rank(const uint32_t (&dig)[4], uint32_t (&pos)[4],
+                                           uint32_t (*s_wcnt)[256], uint32_t* s_dstart,
+                                           uint32_t* s_dcnt, uint32_t* s_scan) {
+    constexpr int kWaves = kBucketThreads / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < kWaves * 256; i += kBucketThreads) (&s_wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t rank[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t d = dig[k];
+        const bool valid = d < 256u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        if (!valid) peers = 1ull << lane;
+        const int leader = __ffsll((long long)peers) - 1;
+        uint32_t old = 0;
+        if (valid && lane == leader) {
+            old = s_wcnt[wave][d];
+            s_wcnt[wave][d] = old + (uint32_t)__popcll(peers);
+        }
+        old = __shfl(old, leader);
+        rank[k] = old + (uint32_t)__popcll(peers & lt);
+    }
+    __syncthreads();
+    uint32_t c = 0;
+    if (tid < 256) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t x = s_wcnt[w][tid];
+            s_wcnt[w][tid] = c;
+            c += x;
+        }
+        s_dcnt[tid] = c;
+    }
+    uint32_t dummy;
+    const uint32_t st = block_exclusive_scan<kBucketThreads>(c, s_scan, &dummy);
+    if (tid < 256) s_dstart[tid] = st;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        pos[k] = dig[k] < 256u ? s_dstart[dig[k]] + s_wcnt[wave][dig[k]] + rank[k] : 0u;
+}
+
+// Highest bit in which any of the block's keys differs from `ref` (-1: none).
+__device__ __forceinline__ int block_top_diff_bit(uint64_t x, uint64_t* s_or) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+    if (lane == 0) s_or[wave] = x;
+    __syncthreads();
+    uint64_t a = 0;
+#pragma unroll
+    for (int w = 0; w < kBucketThreads / 64; ++w) a |= s_or[w];
+    __syncthreads();
+    return a ? 63 - __clzll((long long)a) : -1;
+}
+
+// (4) One block per bucket: sorts keys[base, base + count) (with vals) by
+// (key, value). Up to `cap` keys (at most kBucketCap) by a bitonic sort in
+// LDS; more by stable LSD passes in HBM, ping-ponging with tkeys / tvals over
+// the same range (then equal keys keep their arrival order; k_ties orders
+// them).
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t* __restrict__ tkeys,
+    uint32_t* __restrict__ tvals, const uint32_t* __restrict__ bcount,
+    const uint32_t* __restrict__ bbase, uint32_t cap) {
+    constexpr int kWaves = kBucketThreads / 64;
+    // LDS: the bitonic arrays, or (HBM path) the ranking tables
+    constexpr int kRankBytes = (kWaves * 256 + 3 * 256 + kWaves) * 4 + kWaves * 8;
+    constexpr int kSortBytes = kBucketCap * 12;
+    __shared__ __align__(16) uint8_t s_raw[kSortBytes > kRankBytes ? kSortBytes : kRankBytes];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t cnt = bcount[blockIdx.x];
+    const uint64_t base = bbase[blockIdx.x];
+    if (cnt <= 1) return;
+    if (cnt <= min(cap, (uint32_t)kBucketCap)) {
+        uint64_t* s_k = reinterpret_cast<uint64_t*>(s_raw);
+        uint32_t* s_v = reinterpret_cast<uint32_t*>(s_raw + kBucketCap * 8);
+        uint32_t p2 = 2;
+        while (p2 < cnt) p2 <<= 1;
+        for (uint32_t i = tid; i < p2; i += kBucketThreads) {
+            // padding sorts last: (~0, ~0) exceeds every (key, index)
+            s_k[i] = i < cnt ? keys[base + i] : ~0ull;
+            s_v[i] = i < cnt ? vals[base + i] : ~0u;
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= p2; k <<= 1) {
+            for (uint32_t h = k >> 1; h > 0; h >>= 1) {
+                for (uint32_t i = tid; i < p2 / 2; i += kBucketThreads) {
+                    const uint32_t a = ((i & ~(h - 1)) << 1) | (i & (h - 1)), b = a + h;
+                    const uint64_t ka = s_k[a], kb = s_k[b];
+                    const uint32_t va = s_v[a], vb = s_v[b];
+                    const bool gt = ka > kb || (ka == kb && va > vb);
+                    if (gt == ((a & k) == 0)) {
+                        s_k[a] = kb;
+                        s_k[b] = ka;
+                        s_v[a] = vb;
+                        s_v[b] = va;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = tid; i < cnt; i += kBucketThreads) {
+            keys[base + i] = s_k[i];
+            vals[base + i] = s_v[i];
+        }
+        return;
+    }
+    // HBM: stable LSD passes over the range, chunk by chunk in order
+    uint32_t(*s_wcnt)[256] = reinterpret_cast<uint32_t(*)[256]>(s_raw);
+    uint32_t* s_dstart = reinterpret_cast<uint32_t*>(s_raw) + kWaves * 256;
+    uint32_t* s_dcnt = s_dstart + 256;
+    uint32_t* s_run = s_dcnt + 256;
+    uint32_t* s_scan = s_run + 256;
+    uint64_t* s_or = reinterpret_cast<uint64_t*>(s_raw + (kWaves * 256 + 3 * 256 + kWaves) * 4);
+    const uint64_t ref = keys[base];
+    uint64_t diff = 0;
+    for (uint32_t i = tid; i < cnt; i += kBucketThreads) diff |= keys[base + i] ^ ref;
+    const int hb = block_top_diff_bit(diff, s_or);
+    uint64_t *ka = keys + base, *kb = tkeys + base;
+    uint32_t *va = vals + base, *vb = tvals + base;
+    for (int shift = 0; shift <= hb; shift += 8) {
+        // digit totals of the range, then running starts
+        if (tid < 256) s_run[tid] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < cnt; i += kBucketThreads)
+            atomicAdd(&s_run[(uint32_t)((ka[i] >> shift) & 255u)], 1u);
+        __syncthreads();
+        uint32_t total;
+        const uint32_t start = block_exclusive_scan<kBucketThreads>(tid < 256 ? s_run[tid] : 0u,
+                                                                    s_scan, &total);
+        if (tid < 256) s_run[tid] = start;
+        __syncthreads();
+        for (uint32_t c0 = 0; c0 < cnt; c0 += kBucketChunk) {
+            uint64_t key[4];
+            uint32_t val[4], dig[4], pos[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t i = c0 + (uint32_t)wave * 256u + (uint32_t)k * 64u + (uint32_t)lane;
+                const bool valid = i < cnt;
+                key[k] = valid ? ka[i] : 0ull;
+                val[k] = valid ? va[i] : 0u;
+                dig[k] = valid ? (uint32_t)((key[k] >> shift) & 255u) : 256u;
+            }
+            chunk_rank(dig, pos, s_wcnt, s_dstart, s_dcnt, s_scan);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (dig[k] < 256u) {
+                    const uint32_t o = s_run[dig[k]] + (pos[k] - s_dstart[dig[k]]);
+                    kb[o] = key[k];
+                    vb[o] = val[k];
+                }
+            }
+            __syncthreads();
+            if (tid < 256) s_run[tid] += s_dcnt[tid];
+            __syncthreads();
+        }
+        uint64_t* tk = ka; ka = kb; kb = tk;
+        uint32_t* tv = va; va = vb; vb = tv;
+    }
+    if (ka != keys + base) {
+        for (uint32_t i = tid; i < cnt; i += kBucketThreads) {
+            keys[base + i] = ka[i];
+            vals[base + i] = va[i];
+        }
+    }
+}
+
 // The reference's tree keeps splitting bodies that share every level a key
 // holds (octree.rs:92-125, down to depth 64). After the sort, each run of
 // equal keys (in practice two or three bodies of a dense core) gets the
@@ -486,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ des
         const uint64_t kv = k2[j];
         const uint32_t vv = svals[j];
         uint64_t m = j;
-        while (m > g && k2[m - 1] > kv) {
+        while (m > g && (k2[m - 1] > kv || (k2[m - 1] == kv && svals[m - 1] > vv))) {
             k2[m] = k2[m - 1];
             svals[m] = svals[m - 1];
             --m;

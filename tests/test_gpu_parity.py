@@ -272,6 +272,35 @@ def test_graph_and_eager_steps_identical(engine, kind):
     assert a.download().tobytes() == b.download().tobytes()
 
 
+# ------------------------------------------------------------ key sorting
+@pytest.mark.parametrize("make", [
+    lambda: P.gen_plummer_virial(60000, seed=12, mass=1.0, a=0.1),  # random order: uneven buckets
+    lambda: small_cube(100003, seed=3),
+    lambda: close_pairs_state(3000, seed=6),                         # equal keys (ties)
+], ids=["plummer60k", "cube100k", "ties"])
+def test_sample_sort_equals_radix_sort(engine, make):
+    """The sample sort (buckets in LDS, or in HBM past a small bucket cap)
+    and the LSD radix sort give the same Morton order, tree and steps."""
+    s = make()
+    out = {}
+    try:
+        for name, limits in (("radix", (0, 4096)), ("sample", (1024, 4096)),
+                             ("sample_hbm", (1024, 64)), ("all_hbm", (1024, 0))):
+            engine.set_sort(*limits)
+            sim = P.Sim(engine, s, kind="astro2", theta=0.9, e=0.05, dt=1e-5)
+            sim.step(3)
+            keys, perm = sim.morton()
+            rec, fl = sim.tree()
+            out[name] = (sim.download().tobytes(), keys.tobytes(), perm.tobytes(),
+                         rec.tobytes(), fl.tobytes())
+            sim.close()
+    finally:
+        engine.set_sort(1024, 4096)
+    for name in ("sample", "sample_hbm", "all_hbm"):
+        for a, b, what in zip(out["radix"], out[name], ("state", "keys", "perm", "tree", "nodes")):
+            assert a == b, (name, what)
+
+
 # ---------------------------------------------------------------- batching
 def test_batch_equals_alone(engine):
     states = [small_cube(n, seed=k) for k, n in enumerate((5000, 1, 2, 777, 12000))]
