@@ -37,6 +37,11 @@ def algorithmic_bytes(kernel, n, nodes):
         "sort_count": 8.0 * n + 1024.0 * tiles,      # read keys; write digit counts
         "sort_scan": 2048.0 * tiles,                 # read + write the count matrix
         "sort_scatter": 24.0 * n + 1024.0 * tiles,   # read + write (key, index); read offsets
+        # sample sort: splitters (one block, a few KiB); bucket of each key;
+        # (key, index, bucket) read and (key, index) written; buckets sorted
+        "bucket_count": (8 + 2) * n,
+        "bucket_scatter": (8 + 4 + 2 + 12) * n,
+        "bucket_sort": 24.0 * n,
         # keys, sorted index, state read (x y z px py pz m id fixed) and written
         # (+ velocities on the first two steps only), node offsets, levels
         "scan_gather": (8 + 4 + 65 + 65 + 4 + 1) * n,

@@ -15,9 +15,10 @@ constexpr int kScanItems = 2;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr int kKeyThreads = 1024;  // threads of a key-building block (one sort tile)
-constexpr int kBucketKeys = 2048;  // keys per bucket of the sample sort (on average)
+constexpr int kBucketKeys = 1856;  // keys per bucket of the sample sort (on average; steady-state
+                                   // buckets stay within 2048, the four-keys-per-thread sort)
 constexpr int kBucketCap = 4096;   // largest bucket sorted in LDS (larger ones: in HBM)
-constexpr int kBucketThreads = 512;   // threads of a bucket-sorting block
+constexpr int kBucketThreads = 512;   // threads of a bucket-sorting block (up to 8 keys each)
 constexpr int kMaxBuckets = 1024;  // sample sort up to kMaxBuckets * kBucketKeys keys
 constexpr int kSampleOver = 4;     // splitter samples per bucket (sampling from unsorted keys)
 constexpr int kSplitThreads = 1024;  // threads of the splitter block

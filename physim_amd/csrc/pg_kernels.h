@@ -449,10 +449,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
 // every kSampleOver-th; (2) k_bucket_count: each key's bucket (the
 // number of splitters <= key) and the bucket sizes; (3) k_bucket_scatter:
 // each tile reserves a range in every bucket it holds and writes its keys
-// there, in runs; (4) k_bucket_sort: each bucket is sorted by (key, index)
-// in LDS (a larger one in HBM, by one block). Runs of equal keys then take
-// (second key word, index) order in k_ties, the order a stable sort by key
-// gives.
+// there, in runs; (4) k_bucket_sort: each bucket is sorted by key (a
+// larger one in HBM, by one block). Runs of equal keys then take (second key
+// word, index) order in k_ties, the order a stable sort by key gives.
 
 __global__ __launch_bounds__(kSplitThreads) void k_splitters(
     const uint64_t* __restrict__ prev, const uint64_t* __restrict__ keys, uint64_t n, uint32_t nb,
@@ -716,17 +715,139 @@ __device__ __forceinline__ int block_top_diff_bit(uint64_t x, uint64_t* s_or) {
     return a ? 63 - __clzll((long long)a) : -1;
 }
 
+// Bitonic sort of P = E * kBucketThreads (key, value) pairs, E per thread in
+// registers (element i = thread * E + e): exchanges between a thread's own
+// elements in registers, between threads of a wave by shuffles, and the
+// widest ones through LDS (s_k / s_v, element i at (i % E) * kBucketThreads +
+// i / E, conflict-free). Ascending in (key, value).
+// FULL: order by (key, value); else by key alone (equal keys in no fixed
+// order), which needs no key to equal the padding ~0.
+template <bool FULL>
+__device__ __forceinline__ bool pair_gt(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
+    return FULL ? (ka > kb || (ka == kb && va > vb)) : ka > kb;
+}
+
+// Compare-exchange with a partner held elsewhere: keep the smaller (keep_min)
+// or the larger of the two, by selects.
+template <bool FULL>
+__device__ __forceinline__ void bitonic_keep(uint64_t& k, uint32_t& v, uint64_t ok, uint32_t ov,
+                                             bool keep_min) {
+    const bool take = keep_min ? pair_gt<FULL>(k, v, ok, ov) : pair_gt<FULL>(ok, ov, k, v);
+    k = take ? ok : k;
+    v = take ? ov : v;
+}
+
+template <bool FULL, int E, int H>
+__device__ __forceinline__ void bitonic_regs(uint64_t (&key)[E], uint32_t (&val)[E], uint32_t k) {
+    const uint32_t t0 = threadIdx.x * E;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (e & H) continue;
+        const bool up = ((t0 + (uint32_t)e) & k) == 0;
+        const uint64_t ka = key[e], kb = key[e | H];
+        const uint32_t va = val[e], vb = val[e | H];
+        const bool sw = pair_gt<FULL>(ka, va, kb, vb) == up;
+        key[e] = sw ? kb : ka;
+        key[e | H] = sw ? ka : kb;
+        val[e] = sw ? vb : va;
+        val[e | H] = sw ? va : vb;
+    }
+}
+
+template <bool FULL, int E>
+__device__ void bitonic_block(uint64_t (&key)[E], uint32_t (&val)[E], uint64_t* s_k,
+                              uint32_t* s_v) {
+    constexpr uint32_t P = (uint32_t)E * kBucketThreads;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t h = k >> 1; h > 0; h >>= 1) {
+            if (h >= (uint32_t)E) {
+                // k > h >= E: the direction is the same for a thread's elements
+                const bool up = ((t * E) & k) == 0;
+                const uint32_t m = h / E;
+                const bool keep_min = ((t & m) == 0) == up;
+                if (h >= 64u * E) {
+                    // partner in another wave: through LDS
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        s_k[(uint32_t)e * kBucketThreads + t] = key[e];
+                        s_v[(uint32_t)e * kBucketThreads + t] = val[e];
+                    }
+                    __syncthreads();
+                    const uint32_t pt = t ^ m;
+                    uint64_t ok[E];
+                    uint32_t ov[E];
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        ok[e] = s_k[(uint32_t)e * kBucketThreads + pt];
+                        ov[e] = s_v[(uint32_t)e * kBucketThreads + pt];
+                    }
+#pragma unroll
+                    for (int e = 0; e < E; ++e) bitonic_keep<FULL>(key[e], val[e], ok[e], ov[e], keep_min);
+                    __syncthreads();
+                } else {
+                    // partner in the same wave: shuffles
+                    uint64_t ok[E];
+                    uint32_t ov[E];
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        ok[e] = __shfl_xor(key[e], (int)m);
+                        ov[e] = __shfl_xor(val[e], (int)m);
+                    }
+#pragma unroll
+                    for (int e = 0; e < E; ++e) bitonic_keep<FULL>(key[e], val[e], ok[e], ov[e], keep_min);
+                }
+            } else {
+                // within the thread
+                if (E > 4 && h == 4) bitonic_regs<FULL, E, (E > 4 ? 4 : 1)>(key, val, k);
+                else if (E > 2 && h == 2) bitonic_regs<FULL, E, (E > 2 ? 2 : 1)>(key, val, k);
+                else bitonic_regs<FULL, E, 1>(key, val, k);
+            }
+        }
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void bucket_sort_regs(uint64_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals, uint64_t base,
+                                                 uint32_t cnt, uint64_t* s_k, uint32_t* s_v) {
+    uint64_t key[E];
+    uint32_t val[E];
+    const uint32_t t = threadIdx.x;
+    bool top = false;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = t * E + (uint32_t)e;
+        // padding sorts last: (~0, ~0) exceeds every (key, index)
+        key[e] = i < cnt ? keys[base + i] : ~0ull;
+        val[e] = i < cnt ? vals[base + i] : ~0u;
+        top |= i < cnt && key[e] == ~0ull;
+    }
+    // by key alone unless a key ties with the padding
+    if (__syncthreads_or(top)) bitonic_block<true, E>(key, val, s_k, s_v);
+    else bitonic_block<false, E>(key, val, s_k, s_v);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = t * E + (uint32_t)e;
+        if (i < cnt) {
+            keys[base + i] = key[e];
+            vals[base + i] = val[e];
+        }
+    }
+}
+
 // (4) One block per bucket: sorts keys[base, base + count) (with vals) by
-// (key, value). Up to `cap` keys (at most kBucketCap) by a bitonic sort in
-// LDS; more by stable LSD passes in HBM, ping-ponging with tkeys / tvals over
-// the same range (then equal keys keep their arrival order; k_ties orders
-// them).
+// key. Up to `cap` keys (at most kBucketCap) by a bitonic sort held in
+// registers; more by stable LSD passes in HBM, ping-ponging with tkeys /
+// tvals over the same range. Equal keys come out in no fixed order; k_ties
+// orders them.
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t* __restrict__ tkeys,
     uint32_t* __restrict__ tvals, const uint32_t* __restrict__ bcount,
     const uint32_t* __restrict__ bbase, uint32_t cap) {
     constexpr int kWaves = kBucketThreads / 64;
-    // LDS: the bitonic arrays, or (HBM path) the ranking tables
+    static_assert(kBucketCap == 8 * kBucketThreads, "up to eight keys per thread");
+    // LDS: the bitonic exchange arrays, or (HBM path) the ranking tables
     constexpr int kRankBytes = (kWaves * 256 + 3 * 256 + kWaves) * 4 + kWaves * 8;
     constexpr int kSortBytes = kBucketCap * 12;
     __shared__ __align__(16) uint8_t s_raw[kSortBytes > kRankBytes ? kSortBytes : kRankBytes];
@@ -734,48 +855,24 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
     const uint32_t cnt = bcount[blockIdx.x];
     const uint64_t base = bbase[blockIdx.x];
     if (cnt <= 1) return;
-    if (cnt <= min(cap, (uint32_t)kBucketCap)) {
-        uint64_t* s_k = reinterpret_cast<uint64_t*>(s_raw);
-        uint32_t* s_v = reinterpret_cast<uint32_t*>(s_raw + kBucketCap * 8);
-        uint32_t p2 = 2;
-        while (p2 < cnt) p2 <<= 1;
-        for (uint32_t i = tid; i < p2; i += kBucketThreads) {
-            // padding sorts last: (~0, ~0) exceeds every (key, index)
-            s_k[i] = i < cnt ? keys[base + i] : ~0ull;
-            s_v[i] = i < cnt ? vals[base + i] : ~0u;
-        }
-        __syncthreads();
-        for (uint32_t k = 2; k <= p2; k <<= 1) {
-            for (uint32_t h = k >> 1; h > 0; h >>= 1) {
-                for (uint32_t i = tid; i < p2 / 2; i += kBucketThreads) {
-                    const uint32_t a = ((i & ~(h - 1)) << 1) | (i & (h - 1)), b = a + h;
-                    const uint64_t ka = s_k[a], kb = s_k[b];
-                    const uint32_t va = s_v[a], vb = s_v[b];
-                    const bool gt = ka > kb || (ka == kb && va > vb);
-                    if (gt == ((a & k) == 0)) {
-                        s_k[a] = kb;
-                        s_k[b] = ka;
-                        s_v[a] = vb;
-                        s_v[b] = va;
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        for (uint32_t i = tid; i < cnt; i += kBucketThreads) {
-            keys[base + i] = s_k[i];
-            vals[base + i] = s_v[i];
-        }
-        return;
-    }
-    // HBM: stable LSD passes over the range, chunk by chunk in order
-    uint32_t(*s_wcnt)[256] = reinterpret_cast<uint32_t(*)[256]>(s_raw);
-    uint32_t* s_dstart = reinterpret_cast<uint32_t*>(s_raw) + kWaves * 256;
+    // ranking tables of the HBM passes
+    uint8_t* s_tab = s_raw;
+    uint32_t(*s_wcnt)[256] = reinterpret_cast<uint32_t(*)[256]>(s_tab);
+    uint32_t* s_dstart = reinterpret_cast<uint32_t*>(s_tab) + kWaves * 256;
     uint32_t* s_dcnt = s_dstart + 256;
     uint32_t* s_run = s_dcnt + 256;
     uint32_t* s_scan = s_run + 256;
-    uint64_t* s_or = reinterpret_cast<uint64_t*>(s_raw + (kWaves * 256 + 3 * 256 + kWaves) * 4);
+    uint64_t* s_or = reinterpret_cast<uint64_t*>(s_tab + (kWaves * 256 + 3 * 256 + kWaves) * 4);
     const uint64_t ref = keys[base];
+    if (cnt <= min(cap, (uint32_t)kBucketCap)) {
+        uint64_t* s_bk = reinterpret_cast<uint64_t*>(s_raw);
+        uint32_t* s_bv = reinterpret_cast<uint32_t*>(s_raw + kBucketCap * 8);
+        if (cnt <= 2u * kBucketThreads) bucket_sort_regs<2>(keys, vals, base, cnt, s_bk, s_bv);
+        else if (cnt <= 4u * kBucketThreads) bucket_sort_regs<4>(keys, vals, base, cnt, s_bk, s_bv);
+        else bucket_sort_regs<8>(keys, vals, base, cnt, s_bk, s_bv);
+        return;
+    }
+    // HBM: stable LSD passes over the range, chunk by chunk in order
     uint64_t diff = 0;
     for (uint32_t i = tid; i < cnt; i += kBucketThreads) diff |= keys[base + i] ^ ref;
     const int hb = block_top_diff_bit(diff, s_or);
