@@ -536,7 +536,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     const uint64_t total = bp.total;
     launch(e, cap, "extent", [&] {
         k_extent_partial<<<bp.nchunks, kBlock, 0, st>>>(desc, e->ext_seg, e->ext_chunk,
-                                                        e->ext_partial);
+                                                        e->ext_partial, e->sc, bp.sort_tag);
     });
     const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
     launch(e, cap, "keys", [&] {
@@ -561,9 +561,9 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
                 e->keys[1], e->vals[1]);
         });
         launch(e, cap, "bucket_sort", [&] {
-            k_bucket_sort<<<nb, kBucketThreads, 0, st>>>(e->keys[1], e->vals[1], e->keys[0],
-                                                         e->vals[0], e->bcount, e->bbase,
-                                                         e->bucket_cap);
+            k_bucket_sort<DIM><<<nb, kBucketThreads, 0, st>>>(
+                e->keys[1], e->vals[1], e->keys[0], e->vals[0], e->bcount, e->bbase,
+                e->bucket_cap, desc, total, e->k2, bp.nseg, bp.segbits, bp.lmax);
         });
     } else {
         for (int p = 0; p < bp.passes; ++p) {
@@ -586,14 +586,15 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     const int sorted = bp.buckets ? 1 : bp.passes & 1;
     const uint64_t* skeys = e->keys[sorted];
     uint32_t* svals = e->vals[sorted];
-    launch(e, cap, "ties", [&] {
-        k_ties<DIM><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(desc, total, skeys, svals, e->k2,
-                                                                   bp.nseg, bp.segbits, bp.lmax);
-    });
+    if (!bp.buckets)
+        launch(e, cap, "ties", [&] {
+            k_ties<DIM><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
+                desc, total, skeys, svals, e->k2, bp.nseg, bp.segbits, bp.lmax);
+        });
     const unsigned tiles = bp.ntiles;
     launch(e, cap, "scan_gather", [&] {
         k_scan_gather<DIM><<<tiles, kBlock, 0, st>>>(
-            desc, e->tile_start, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local,
+            desc, e->tile_start, e->ext_partial, bp.nchunks, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local,
             e->tile_tot, e->lo_level, e->lvl_first, e->k2, e->err_global);
     });
     launch(e, cap, "tree_scan", [&] {
@@ -615,7 +616,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "walk", [&] {
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
             desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
-            e->acc_out);
+            e->acc_out, e->ext_partial, e->sc, bp.sort_tag);
     });
 }
 
@@ -1018,6 +1019,8 @@ extern "C" int pg_sim_upload(pg_sim* s, const Entity* state, uint64_t n) {
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemsetAsync(s->st, 0, sizeof(SegState), e->stream));
+    // new positions: no extent partials are current
+    if (e->sc) HIP_TRY(hipMemsetAsync(&e->sc->ext_tag, 0, sizeof(uint64_t), e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     s->uploaded = true;
     if (e->last_sim == s) e->last_sim = nullptr;

@@ -11,7 +11,7 @@ constexpr int kBlock = 256;        // threads per block (4 wave64)
 constexpr int kSortTile = 2048;    // keys per radix-sort tile
 constexpr int kSortThreads = 512;  // threads per radix-sort tile
 constexpr int kSortItems = kSortTile / kSortThreads;  // keys per thread
-constexpr int kScanItems = 2;      // counts per thread in the node-offset scan
+constexpr int kScanItems = 4;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr int kKeyThreads = 1024;  // threads of a key-building block (one sort tile)
@@ -24,9 +24,9 @@ constexpr int kSampleOver = 4;     // splitter samples per bucket (sampling from
 constexpr int kSplitThreads = 1024;  // threads of the splitter block
 constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort tiles
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
-constexpr int kNodeTile = 512;     // sorted bodies per node-building block
-constexpr int kNodeThreads = 256;  // threads per node-building block (two bodies each)
-constexpr int kNodeLds = 896;      // nodes per tile built in LDS (larger tiles: in HBM)
+constexpr int kNodeTile = 1024;    // sorted bodies per node-building block
+constexpr int kNodeThreads = 512;  // threads per node-building block (two bodies each)
+constexpr int kNodeLds = 1792;     // nodes per tile built in LDS (larger tiles: in HBM)
 constexpr int kTopSlots = 65;      // tree levels 0..64: one tile-spanning node each
 constexpr int kMaxTieRun = 256;    // longest run of equal keys ordered by the second key word
 constexpr int kComSpanGroups = 5;   // level groups (of 4 waves) per tile in k_com_span
@@ -97,6 +97,7 @@ struct StepCounters {
     uint32_t gen;           // bumped per step: versions tile-spanning node slots
     uint32_t top_levels;    // 1 + deepest level of a tile-spanning node (0: none)
     uint64_t sort_tag;      // batch whose sorted keys the sample sort left (0: none)
+    uint64_t ext_tag;       // batch whose current positions the extent partials hold (0: none)
     uint32_t split_prev;    // this step's splitters may come from those keys
     uint32_t pad;
 };

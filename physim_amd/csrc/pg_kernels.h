@@ -166,12 +166,16 @@ __device__ __forceinline__ uint64_t block_max_u64(uint64_t v, uint64_t* s_wave) 
 
 // transformers.rs:126-131: max |coord| over a segment (exact). Each block
 // reduces one chunk of kExtentChunk bodies of one segment into partial[];
-// NaN/Inf coordinates are flagged.
+// NaN/Inf coordinates are flagged. Nothing to do when the batch's previous
+// integrating walk left the partials of the current positions (ext_tag).
 __global__ __launch_bounds__(kBlock) void k_extent_partial(const SegDesc* __restrict__ desc,
                                                            const uint32_t* __restrict__ chunk_seg,
                                                            const uint32_t* __restrict__ chunk_idx,
-                                                           uint64_t* __restrict__ partial) {
+                                                           uint64_t* __restrict__ partial,
+                                                           const StepCounters* __restrict__ sc,
+                                                           uint64_t tag) {
     __shared__ uint64_t s_wave[kBlock / 64];
+    if (tag != 0 && sc->ext_tag == tag) return;
     const SegDesc& d = desc[chunk_seg[blockIdx.x]];
     const uint64_t j0 = (uint64_t)chunk_idx[blockIdx.x] * kExtentChunk;
     const uint64_t j1 = min(j0 + kExtentChunk, d.n);
@@ -233,6 +237,7 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
         // batch's previous step (sort_tag 0: the radix sort, which leaves none)
         sc->split_prev = sort_tag != 0 && sc->sort_tag == sort_tag ? 1u : 0u;
         sc->sort_tag = sort_tag;
+        sc->ext_tag = 0;  // the partials are read here; k_scan_gather clears them
     }
     // the tile's extent when it lies in one segment (block-uniform branch)
     uint64_t mblock = 0;
@@ -437,6 +442,77 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
             atomicOr(err, kErrCapacity);
         }
     }
+}
+
+// The reference's tree keeps splitting bodies that share every level a key
+// holds (octree.rs:92-125, down to depth 64). After the sort, each run of
+// equal keys (in practice two or three bodies of a dense core) gets the
+// next lmax levels of the same descent as second key words, and is put in
+// (second word, index) order by the run's first thread (insertion sort).
+// Equal second words, or a run longer than kMaxTieRun, are a depth error.
+template <int DIM>
+__device__ void order_tie_run(const SegDesc* __restrict__ desc, uint64_t total,
+                              const uint64_t* __restrict__ skeys, uint32_t* __restrict__ svals,
+                              uint64_t* __restrict__ k2, uint32_t nseg, int segbits, int lmax,
+                              uint64_t g) {
+    if (g + 1 >= total) return;
+    const uint64_t key = skeys[g];
+    if (skeys[g + 1] != key || (g > 0 && skeys[g - 1] == key)) return;  // run heads only
+    const uint32_t s = seg_of_key(key, segbits);
+    if (s >= nseg) return;
+    const SegDesc& d = desc[s];
+    uint64_t r = g + 1;
+    while (r + 1 < total && skeys[r + 1] == key && r + 1 - g < (uint64_t)kMaxTieRun) ++r;
+    if (r + 1 < total && skeys[r + 1] == key) {
+        atomicOr(&d.st->err, kErrDepth);
+        return;
+    }
+    const double E = bits_f64(d.st->extent_bits);
+    for (uint64_t j = g; j <= r; ++j) {
+        const uint64_t src = (uint64_t)svals[j] - d.off;
+        if (src >= d.n) {
+            atomicOr(&d.st->err, kErrCapacity);
+            return;
+        }
+        const double px = d.sx[src], py = d.sy[src], pz = d.sz[src];
+        double ext = E, cx = 0.0, cy = 0.0, cz = 0.0;
+        uint64_t kk = 0;
+        for (int l = 0; l < 2 * lmax; ++l) {
+            uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
+            if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
+            const double h2 = half(ext);
+            cx = (o & 1u) ? cx + h2 : cx - h2;
+            cy = (o & 2u) ? cy + h2 : cy - h2;
+            if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+            ext = h2;
+            if (l >= lmax) kk = (kk << DIM) | o;
+        }
+        k2[j] = kk;
+    }
+    for (uint64_t j = g + 1; j <= r; ++j) {
+        const uint64_t kv = k2[j];
+        const uint32_t vv = svals[j];
+        uint64_t m = j;
+        while (m > g && (k2[m - 1] > kv || (k2[m - 1] == kv && svals[m - 1] > vv))) {
+            k2[m] = k2[m - 1];
+            svals[m] = svals[m - 1];
+            --m;
+        }
+        k2[m] = kv;
+        svals[m] = vv;
+    }
+}
+
+// Tie runs after the radix sort (the sample sort's bucket blocks order their
+// own).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ desc, uint64_t total,
+                                                 const uint64_t* __restrict__ skeys,
+                                                 uint32_t* __restrict__ svals,
+                                                 uint64_t* __restrict__ k2, uint32_t nseg,
+                                                 int segbits, int lmax) {
+    order_tie_run<DIM>(desc, total, skeys, svals, k2, nseg, segbits, lmax,
+                       (uint64_t)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // ------------------------------------------------------------ sample sort
@@ -839,12 +915,14 @@ __device__ __forceinline__ void bucket_sort_regs(uint64_t* __restrict__ keys,
 // (4) One block per bucket: sorts keys[base, base + count) (with vals) by
 // key. Up to `cap` keys (at most kBucketCap) by a bitonic sort held in
 // registers; more by stable LSD passes in HBM, ping-ponging with tkeys /
-// tvals over the same range. Equal keys come out in no fixed order; k_ties
-// orders them.
+// tvals over the same range. Equal keys fall in one bucket: the block then
+// orders their runs (order_tie_run).
+template <int DIM>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t* __restrict__ tkeys,
     uint32_t* __restrict__ tvals, const uint32_t* __restrict__ bcount,
-    const uint32_t* __restrict__ bbase, uint32_t cap) {
+    const uint32_t* __restrict__ bbase, uint32_t cap, const SegDesc* __restrict__ desc,
+    uint64_t total, uint64_t* __restrict__ k2, uint32_t nseg, int segbits, int lmax) {
     constexpr int kWaves = kBucketThreads / 64;
     static_assert(kBucketCap == 8 * kBucketThreads, "up to eight keys per thread");
     // LDS: the bitonic exchange arrays, or (HBM path) the ranking tables
@@ -870,6 +948,9 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
         if (cnt <= 2u * kBucketThreads) bucket_sort_regs<2>(keys, vals, base, cnt, s_bk, s_bv);
         else if (cnt <= 4u * kBucketThreads) bucket_sort_regs<4>(keys, vals, base, cnt, s_bk, s_bv);
         else bucket_sort_regs<8>(keys, vals, base, cnt, s_bk, s_bv);
+        __syncthreads();
+        for (uint32_t i = tid; i < cnt; i += kBucketThreads)
+            order_tie_run<DIM>(desc, total, keys, vals, k2, nseg, segbits, lmax, base + i);
         return;
     }
     // HBM: stable LSD passes over the range, chunk by chunk in order
@@ -923,67 +1004,9 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
             vals[base + i] = va[i];
         }
     }
-}
-
-// The reference's tree keeps splitting bodies that share every level a key
-// holds (octree.rs:92-125, down to depth 64). After the sort, each run of
-// equal keys (in practice two or three bodies of a dense core) gets the
-// next lmax levels of the same descent as second key words, and is put in
-// their order (insertion sort by the run's first thread). Equal second
-// words, or a run longer than kMaxTieRun, are a depth error.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ desc, uint64_t total,
-                                                 const uint64_t* __restrict__ skeys,
-                                                 uint32_t* __restrict__ svals,
-                                                 uint64_t* __restrict__ k2, uint32_t nseg,
-                                                 int segbits, int lmax) {
-    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g + 1 >= total) return;
-    const uint64_t key = skeys[g];
-    if (skeys[g + 1] != key || (g > 0 && skeys[g - 1] == key)) return;  // run heads only
-    const uint32_t s = seg_of_key(key, segbits);
-    if (s >= nseg) return;
-    const SegDesc& d = desc[s];
-    uint64_t r = g + 1;
-    while (r + 1 < total && skeys[r + 1] == key && r + 1 - g < (uint64_t)kMaxTieRun) ++r;
-    if (r + 1 < total && skeys[r + 1] == key) {
-        atomicOr(&d.st->err, kErrDepth);
-        return;
-    }
-    const double E = bits_f64(d.st->extent_bits);
-    for (uint64_t j = g; j <= r; ++j) {
-        const uint64_t src = (uint64_t)svals[j] - d.off;
-        if (src >= d.n) {
-            atomicOr(&d.st->err, kErrCapacity);
-            return;
-        }
-        const double px = d.sx[src], py = d.sy[src], pz = d.sz[src];
-        double ext = E, cx = 0.0, cy = 0.0, cz = 0.0;
-        uint64_t kk = 0;
-        for (int l = 0; l < 2 * lmax; ++l) {
-            uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
-            if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
-            const double h2 = half(ext);
-            cx = (o & 1u) ? cx + h2 : cx - h2;
-            cy = (o & 2u) ? cy + h2 : cy - h2;
-            if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-            ext = h2;
-            if (l >= lmax) kk = (kk << DIM) | o;
-        }
-        k2[j] = kk;
-    }
-    for (uint64_t j = g + 1; j <= r; ++j) {
-        const uint64_t kv = k2[j];
-        const uint32_t vv = svals[j];
-        uint64_t m = j;
-        while (m > g && (k2[m - 1] > kv || (k2[m - 1] == kv && svals[m - 1] > vv))) {
-            k2[m] = k2[m - 1];
-            svals[m] = svals[m - 1];
-            --m;
-        }
-        k2[m] = kv;
-        svals[m] = vv;
-    }
+    __syncthreads();
+    for (uint32_t i = tid; i < cnt; i += kBucketThreads)
+        order_tie_run<DIM>(desc, total, keys, vals, k2, nseg, segbits, lmax, base + i);
 }
 
 // Per sorted body: its common-prefix lengths with its neighbours give the
@@ -1003,7 +1026,7 @@ __global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ des
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const SegDesc* __restrict__ desc, const uint32_t* __restrict__ tile_start,
-    const uint64_t* __restrict__ skeys,
+    uint64_t* __restrict__ ext_partial, uint32_t nchunks, const uint64_t* __restrict__ skeys,
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
     uint32_t* __restrict__ node_local, uint32_t* __restrict__ tile_tot,
     uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_first,
@@ -1015,6 +1038,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const uint32_t tile = blockIdx.x;
     const uint64_t tbase = tile_start[tile], tend = tile_start[tile + 1];
     if (tid < kTopSlots) s_firstlo[tid] = ~0u;
+    // the extent partials were read by k_keys: cleared for the walk's maxima
+    if (tile == 0)
+        for (uint32_t c = tid; c < nchunks; c += kBlock) ext_partial[c] = 0;
     __syncthreads();
 
 #pragma unroll 2
@@ -1805,7 +1831,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
                                                  const uint64_t* __restrict__ skeys, int segbits,
                                                  const uint32_t* __restrict__ first_node,
                                                  const NodeHot* __restrict__ hot, uint64_t cap,
-                                                 uint32_t nseg, double* __restrict__ acc_out) {
+                                                 uint32_t nseg, double* __restrict__ acc_out,
+                                                 uint64_t* __restrict__ ext_partial,
+                                                 StepCounters* __restrict__ sc, uint64_t tag) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = g < total;
     uint32_t s = 0;
@@ -1893,43 +1921,101 @@ __global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ des
             atomicAdd((unsigned long long*)&desc[s0].st->interactions, c);
         cpend = cpend && !mine;
     }
-    if (!valid) return;
-    const SegDesc& d = desc[s];
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    if (!fixed) {
-        if (FP64) {
-            ax = f64[0] / m; ay = f64[1] / m; az = f64[2] / m;
-        } else if (m == 0.0) {
-            ax = ay = az = __builtin_nan("");
+    uint64_t emax = 0;  // the new position's max |coord| (bit pattern)
+    uint32_t chunk = ~0u;
+    bool nonfinite = false;
+    if (valid) {
+        const SegDesc& d = desc[s];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (!fixed) {
+            if (FP64) {
+                ax = f64[0] / m; ay = f64[1] / m; az = f64[2] / m;
+            } else if (m == 0.0) {
+                ax = ay = az = __builtin_nan("");
+            } else {
+                ax = (double)ka.sx - (double)ka.cx;
+                ay = (double)ka.sy - (double)ka.cy;
+                az = (double)ka.sz - (double)ka.cz;
+            }
+        }
+        if (d.count) d.counts[d.did[j]] = count;
+        if (INTEGRATE) {
+            // verlet.rs:24-82. After the Taylor first step the velocity is
+            // (x' - x) / dt of the stored positions and is formed on download.
+            const double dt = d.dt;
+            const double dt2 = dt * dt;
+            double nx, ny, nz;
+            if (d.st->history == 0) {
+                const double ux = d.dvx[j], uy = d.dvy[j], uz = d.dvz[j];
+                nx = x + ux * dt + 0.5 * ax * dt2;
+                ny = y + uy * dt + 0.5 * ay * dt2;
+                nz = z + uz * dt + 0.5 * az * dt2;
+                d.dvx[j] = ux + ax * dt;
+                d.dvy[j] = uy + ay * dt;
+                d.dvz[j] = uz + az * dt;
+            } else {
+                nx = 2.0 * x - d.dpx[j] + ax * dt2;
+                ny = 2.0 * y - d.dpy[j] + ay * dt2;
+                nz = 2.0 * z - d.dpz[j] + az * dt2;
+            }
+            d.dx[j] = nx; d.dy[j] = ny; d.dz[j] = nz;
+            d.dpx[j] = x; d.dpy[j] = y; d.dpz[j] = z;
+            emax = f64_bits(fmax(fmax(fabs(nx), fabs(ny)), fabs(nz)));
+            nonfinite = !isfinite(nx) || !isfinite(ny) || !isfinite(nz);
+            chunk = d.chunk0 + (uint32_t)(j / kExtentChunk);
+            if (nonfinite) atomicOr(&d.st->err, kErrNonFinite);
         } else {
-            ax = (double)ka.sx - (double)ka.cx;
-            ay = (double)ka.sy - (double)ka.cy;
-            az = (double)ka.sz - (double)ka.cz;
+            double* o = acc_out + 3 * (d.off + d.did[j]);
+            o[0] = ax; o[1] = ay; o[2] = az;
         }
     }
-    if (d.count) d.counts[d.did[j]] = count;
     if (INTEGRATE) {
-        // verlet.rs:24-82. After the Taylor first step the velocity is
-        // (x' - x) / dt of the stored positions and is formed on download.
-        const double dt = d.dt;
-        const double dt2 = dt * dt;
-        if (d.st->history == 0) {
-            const double ux = d.dvx[j], uy = d.dvy[j], uz = d.dvz[j];
-            d.dx[j] = x + ux * dt + 0.5 * ax * dt2;
-            d.dy[j] = y + uy * dt + 0.5 * ay * dt2;
-            d.dz[j] = z + uz * dt + 0.5 * az * dt2;
-            d.dvx[j] = ux + ax * dt;
-            d.dvy[j] = uy + ay * dt;
-            d.dvz[j] = uz + az * dt;
-        } else {
-            d.dx[j] = 2.0 * x - d.dpx[j] + ax * dt2;
-            d.dy[j] = 2.0 * y - d.dpy[j] + ay * dt2;
-            d.dz[j] = 2.0 * z - d.dpz[j] + az * dt2;
+        // the next step's extent partials (what k_extent_partial computes):
+        // per chunk the wave's maximum; a wave's first chunk is merged with
+        // the other waves' in LDS (a block lies in one chunk unless a segment
+        // starts in it), so a block issues one atomic per chunk
+        __shared__ unsigned long long s_pv[kBlock / 64];
+        __shared__ uint32_t s_pc[kBlock / 64];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) s_pc[wave] = ~0u;
+        bool pend = chunk != ~0u, first = true;
+        while (__ballot(pend)) {
+            const int lead = __ffsll((long long)__ballot(pend)) - 1;
+            const uint32_t c0 = uniform_u32(__shfl(chunk, lead));
+            const bool mine = pend && chunk == c0;
+            unsigned long long v = mine ? (unsigned long long)emax : 0ull;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long t = __shfl_xor(v, o);
+                v = t > v ? t : v;
+            }
+            if (lane == lead) {
+                if (first) {
+                    s_pc[wave] = c0;
+                    s_pv[wave] = v;
+                } else {
+                    atomicMax((unsigned long long*)&ext_partial[c0], v);
+                }
+            }
+            first = false;
+            pend = pend && !mine;
         }
-        d.dpx[j] = x; d.dpy[j] = y; d.dpz[j] = z;
-    } else {
-        double* o = acc_out + 3 * (d.off + d.did[j]);
-        o[0] = ax; o[1] = ay; o[2] = az;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 0; w < kBlock / 64; ++w) {
+                const uint32_t c = s_pc[w];
+                if (c == ~0u) continue;
+                unsigned long long v = s_pv[w];
+                for (int w2 = w + 1; w2 < kBlock / 64; ++w2) {
+                    if (s_pc[w2] == c) {
+                        v = s_pv[w2] > v ? s_pv[w2] : v;
+                        s_pc[w2] = ~0u;
+                    }
+                }
+                atomicMax((unsigned long long*)&ext_partial[c], v);
+            }
+            if (blockIdx.x == 0) sc->ext_tag = tag;
+        }
     }
 }
 
