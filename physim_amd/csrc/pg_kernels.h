@@ -796,6 +796,37 @@ __device__ __forceinline__ int block_top_diff_bit(uint64_t x, uint64_t* s_or) {
 // elements in registers, between threads of a wave by shuffles, and the
 // widest ones through LDS (s_k / s_v, element i at (i % E) * kBucketThreads +
 // i / E, conflict-free). Ascending in (key, value).
+// The value of lane (lane ^ M) of a wave, on the VALU instead of a
+// bpermute through the LDS unit: DPP quad_perm for M = 1, 2; row_ror:n
+// (lane i of a row of 16 takes lane i - n mod 16) for M = 8, and for M = 4
+// by lane (row_ror:4 where bit 2 is set, row_ror:12 where it is clear);
+// permlane16/32_swap of the value with itself for M = 16, 32 (the swap
+// leaves the partner row / half in one of its two results).
+template <int M>
+__device__ __forceinline__ uint32_t xor_lanes(uint32_t v) {
+    static_assert(M == 1 || M == 2 || M == 4 || M == 8 || M == 16 || M == 32, "lane distance");
+    const uint32_t lane = threadIdx.x & 63u;
+    if (M == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    if (M == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    if (M == 4) {
+        const uint32_t r4 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xF, 0xF, false);
+        const uint32_t r12 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x12C, 0xF, 0xF, false);
+        return (lane & 4u) ? r4 : r12;
+    }
+    if (M == 8) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);
+    if (M == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16u) ? p[0] : p[1];
+    }
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32u) ? p[0] : p[1];
+}
+
+template <int M>
+__device__ __forceinline__ uint64_t xor_lanes64(uint64_t v) {
+    return (uint64_t)xor_lanes<M>((uint32_t)v) | ((uint64_t)xor_lanes<M>((uint32_t)(v >> 32)) << 32);
+}
+
 // FULL: order by (key, value); else by key alone (equal keys in no fixed
 // order), which needs no key to equal the padding ~0.
 template <bool FULL>
@@ -830,6 +861,20 @@ __device__ __forceinline__ void bitonic_regs(uint64_t (&key)[E], uint32_t (&val)
     }
 }
 
+// One bitonic stage against the partner thread t ^ M of the same wave.
+template <bool FULL, int E, int M>
+__device__ __forceinline__ void bitonic_wave(uint64_t (&key)[E], uint32_t (&val)[E], bool keep_min) {
+    uint64_t ok[E];
+    uint32_t ov[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        ok[e] = xor_lanes64<M>(key[e]);
+        ov[e] = xor_lanes<M>(val[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) bitonic_keep<FULL>(key[e], val[e], ok[e], ov[e], keep_min);
+}
+
 template <bool FULL, int E>
 __device__ void bitonic_block(uint64_t (&key)[E], uint32_t (&val)[E], uint64_t* s_k,
                               uint32_t* s_v) {
@@ -862,16 +907,15 @@ __device__ void bitonic_block(uint64_t (&key)[E], uint32_t (&val)[E], uint64_t* 
                     for (int e = 0; e < E; ++e) bitonic_keep<FULL>(key[e], val[e], ok[e], ov[e], keep_min);
                     __syncthreads();
                 } else {
-                    // partner in the same wave: shuffles
-                    uint64_t ok[E];
-                    uint32_t ov[E];
-#pragma unroll
-                    for (int e = 0; e < E; ++e) {
-                        ok[e] = __shfl_xor(key[e], (int)m);
-                        ov[e] = __shfl_xor(val[e], (int)m);
+                    // partner in the same wave: lane exchanges on the VALU
+                    switch (m) {
+                        case 1: bitonic_wave<FULL, E, 1>(key, val, keep_min); break;
+                        case 2: bitonic_wave<FULL, E, 2>(key, val, keep_min); break;
+                        case 4: bitonic_wave<FULL, E, 4>(key, val, keep_min); break;
+                        case 8: bitonic_wave<FULL, E, 8>(key, val, keep_min); break;
+                        case 16: bitonic_wave<FULL, E, 16>(key, val, keep_min); break;
+                        default: bitonic_wave<FULL, E, 32>(key, val, keep_min); break;
                     }
-#pragma unroll
-                    for (int e = 0; e < E; ++e) bitonic_keep<FULL>(key[e], val[e], ok[e], ov[e], keep_min);
                 }
             } else {
                 // within the thread
@@ -918,7 +962,7 @@ __device__ __forceinline__ void bucket_sort_regs(uint64_t* __restrict__ keys,
 // tvals over the same range. Equal keys fall in one bucket: the block then
 // orders their runs (order_tie_run).
 template <int DIM>
-__global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
+__global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_bucket_sort(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t* __restrict__ tkeys,
     uint32_t* __restrict__ tvals, const uint32_t* __restrict__ bcount,
     const uint32_t* __restrict__ bbase, uint32_t cap, const SegDesc* __restrict__ desc,
