@@ -26,7 +26,7 @@ constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
 constexpr int kNodeTile = 1024;    // sorted bodies per node-building block
 constexpr int kNodeThreads = 512;  // threads per node-building block (two bodies each)
-constexpr int kNodeLds = 1792;     // nodes per tile built in LDS (larger tiles: in HBM)
+constexpr int kNodeLds = 1792;     // nodes per tile built one per thread (larger: body by body)
 constexpr int kTopSlots = 65;      // tree levels 0..64: one tile-spanning node each
 constexpr int kMaxTieRun = 256;    // longest run of equal keys ordered by the second key word
 constexpr int kComSpanGroups = 5;   // level groups (of 4 waves) per tile in k_com_span

@@ -1319,33 +1319,59 @@ __device__ __forceinline__ uint32_t block_excl_min(uint32_t v, uint32_t* s_wave)
     return ex;
 }
 
-// Mass and centre of mass of an internal node from its children, in octant
-// order (children are p+1 and then each child's skip until p's skip).
-__device__ __forceinline__ void com_from_children(NodeHot* hot, uint32_t p) {
-    double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
-    const uint32_t send = hot[p].skip;
-    for (uint32_t c = p + 1u; c < send;) {
-        const double m = hot[c].mass;
-        M += m;
-        X += m * hot[c].mx;
-        Y += m * hot[c].my;
-        Z += m * hot[c].mz;
-        const uint32_t nxt = hot[c].skip;
-        c = nxt > c ? nxt : c + 1u;
-    }
-    const double inv = 1.0 / M;
-    hot[p].mx = X * inv;
-    hot[p].my = Y * inv;
-    hot[p].mz = Z * inv;
-    hot[p].mass = M;
+// Mass and centre of mass of a node from the mass sums of its bodies.
+__device__ __forceinline__ Sum4 com_of(const Sum4& s) {
+    const double inv = 1.0 / s.m;
+    return Sum4{s.m, s.x * inv, s.y * inv, s.z * inv};
 }
 
-// Local-node tags of the LDS reduction (low 7 bits: level, at most 64).
-constexpr uint8_t kTagInternal = 0x80;  // internal node whose subtree lies in the tile
-constexpr uint8_t kTagLevel = 0x7F;
+// Mass sums of the staged bodies [a, e), in body order.
+__device__ __forceinline__ Sum4 range_sum(const double* s_m, const double* s_x, const double* s_y,
+                                          const double* s_z, uint32_t a, uint32_t e) {
+    Sum4 r{0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = a; b < e; ++b) {
+        const double m = s_m[b];
+        r.m += m;
+        r.x += m * s_x[b];
+        r.y += m * s_y[b];
+        r.z += m * s_z[b];
+    }
+    return r;
+}
+
+// The same sums by one wave: lane l folds bodies a + l, a + l + 64, ... in
+// order, then a butterfly over the lanes; lane 0's association is fixed.
+__device__ __forceinline__ Sum4 range_sum_wave(const double* s_m, const double* s_x,
+                                               const double* s_y, const double* s_z, uint32_t a,
+                                               uint32_t e) {
+    Sum4 r{0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = a + (threadIdx.x & 63u); b < e; b += 64u) {
+        const double m = s_m[b];
+        r.m += m;
+        r.x += m * s_x[b];
+        r.y += m * s_y[b];
+        r.z += m * s_z[b];
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        r.m += __shfl_xor(r.m, o);
+        r.x += __shfl_xor(r.x, o);
+        r.y += __shfl_xor(r.y, o);
+        r.z += __shfl_xor(r.z, o);
+    }
+    return r;
+}
+
+// Internal nodes of more bodies than this have their sums taken by a wave.
+constexpr uint32_t kSmallNode = 64;
+// Queue of such nodes per tile: at one level they hold disjoint body sets,
+// so a tile has at most kNodeTile / (kSmallNode + 1) per level.
+constexpr int kBigNodes = 1024;
+static_assert(kTopSlots * (kNodeTile / (kSmallNode + 1)) <= kBigNodes, "big-node queue bound");
+static_assert(kNodeTile <= 1024, "queue entries hold 10-bit body indices");
 
 // A node record as four 16-byte chunks: (cx, cy), (cz, mx), (my, mz),
-// (mass, skip, level | octant << 16).
+// (mass, skip, level | octant << 16), or as eight 8-byte words.
 static_assert(sizeof(NodeHot) == 64, "one node record per 64-byte line");
 static_assert(offsetof(NodeHot, mass) == 48 && offsetof(NodeHot, skip) == 56 &&
                   offsetof(NodeHot, level) == 60 && offsetof(NodeHot, octant) == 62,
@@ -1354,17 +1380,14 @@ __device__ __forceinline__ uint4 pack2(double a, double b) {
     const uint64_t x = f64_bits(a), y = f64_bits(b);
     return make_uint4((uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32));
 }
+__device__ __forceinline__ uint2 pack1(double a) {
+    const uint64_t x = f64_bits(a);
+    return make_uint2((uint32_t)x, (uint32_t)(x >> 32));
+}
 __device__ __forceinline__ uint4 pack_tail(double mass, uint32_t skip, uint32_t level,
                                            uint32_t octant) {
     const uint64_t x = f64_bits(mass);
     return make_uint4((uint32_t)x, (uint32_t)(x >> 32), skip, level | (octant << 16));
-}
-__device__ __forceinline__ void store_node(NodeHot* p, const NodeHot& h) {
-    uint4* o = reinterpret_cast<uint4*>(p);
-    o[0] = pack2(h.cx, h.cy);
-    o[1] = pack2(h.cz, h.mx);
-    o[2] = pack2(h.my, h.mz);
-    o[3] = pack_tail(h.mass, h.skip, h.level, h.octant);
 }
 
 // Builds the tree and its masses and centres of mass, one block per node tile
@@ -1380,23 +1403,20 @@ __device__ __forceinline__ void store_node(NodeHot* p, const NodeHot& h) {
 // ends before the first later body starting a node at its level or above (the
 // scanned per-level table), so no search leaves the tile.
 //
-// A tile of at most kNodeLds nodes is built in LDS, one thread per node: the
-// bodies fill the node -> owner table and their leaves; each node then finds
-// its range, skip and tag; an internal node whose bodies all lie in the tile
-// has its whole subtree there, so its mass and centre of mass are reduced in
-// LDS, deepest level first, from its children in octant order (the
-// arithmetic of com_from_children); last, every record is written once, by
-// the thread of its node (one descent per node). A tile with more nodes
-// (clustered bodies) builds
-// them body by body into the global records and runs the same level loop
-// there.
+// Bodies (keys, levels, positions, masses) are staged in LDS. A tile of at
+// most kNodeLds nodes is built one thread per node (a node -> owner table);
+// a larger one (clustered bodies), body by body. Each record is written by
+// the thread that finds its range; an internal node whose bodies all lie in
+// the tile takes its mass and centre of mass from the sums of those bodies in
+// body order (by its thread up to kSmallNode bodies, else by a wave after the
+// node pass).
 //
 // An internal node whose range leaves the tile is left to k_com_span: it is
 // recorded (generation-tagged) at slot tile * kTopSlots + level, and
 // sc->top_levels bounds the levels k_com_span visits. For those nodes the
 // block also writes the tile's partial mass sums (below).
 template <int DIM>
-__global__ __launch_bounds__(kNodeThreads) void k_nodes(
+__global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_nodes(
     const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off,
     const uint32_t* __restrict__ tile_start, uint64_t total, const uint64_t* __restrict__ skeys_g,
     uint32_t nseg, int segbits, int lmax, const uint32_t* __restrict__ node_local,
@@ -1409,13 +1429,11 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     __shared__ uint64_t s_keys[kNodeTile];
     __shared__ uint32_t s_first[kNodeTile + 1];
     __shared__ uint8_t s_lo[kNodeTile];
-    __shared__ double s_m[kNodeLds], s_x[kNodeLds], s_y[kNodeLds], s_z[kNodeLds];
-    __shared__ uint16_t s_skip[kNodeLds];  // skip - first node, at most the tile's node count
+    __shared__ double s_bm[kNodeTile], s_bx[kNodeTile], s_by[kNodeTile], s_bz[kNodeTile];
     __shared__ uint16_t s_own[kNodeLds];
-    __shared__ uint32_t s_spanskip[kTopSlots];  // skip of the tile-spanning node per level
-    __shared__ uint8_t s_tag[kNodeLds];
+    __shared__ uint32_t s_big[kBigNodes];
     __shared__ uint64_t s_next;  // key just after the tile
-    __shared__ uint32_t s_maxl, s_topl;
+    __shared__ uint32_t s_topl, s_nbig;
     __shared__ Sum4 s_w4[kNodeThreads / 64];
     __shared__ uint32_t s_wmin[kNodeThreads / 64];
     const uint64_t gen = (uint64_t)sc->gen << 32;
@@ -1426,17 +1444,17 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     const uint32_t nend = tile_base[tile + 1];
     const uint32_t n1 = (uint32_t)min((uint64_t)nend, cap);
     const uint32_t nloc = n1 > n0 ? n1 - n0 : 0u;
-    const bool in_lds = nloc <= (uint32_t)kNodeLds;
+    const bool node_par = nloc <= (uint32_t)kNodeLds;
     // a node tile lies in one segment
     const SegDesc& d = desc[seg_of(seg_off, nseg, b0)];
     const double ext0 = bits_f64(d.st->extent_bits);
     if (threadIdx.x == 0) {
         s_first[nb] = nend;
         s_next = b1 < total ? skeys_g[b1] : 0ull;  // read only when the segment continues past b1
-        s_maxl = 0;
         s_topl = 0;
+        s_nbig = 0;
     }
-    // Bodies: staging, the node -> owner table (LDS tiles), checks, leaves.
+    // Bodies: staging, the node -> owner table, checks.
     for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
         const uint64_t i = b0 + li;
         const uint64_t key = skeys_g[i];
@@ -1447,10 +1465,14 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
         s_lo[li] = (uint8_t)lo;
         s_first[li] = f;
         first_node[i] = f;
-        if (in_lds)
+        if (node_par)
             for (uint32_t nd = f; nd < fn && nd < n1; ++nd) s_own[nd - n0] = (uint16_t)li;
         const uint64_t j = i - d.off;
         const double x = d.dx[j], y = d.dy[j], z = d.dz[j], m = d.dm[j];
+        s_bm[li] = m;
+        s_bx[li] = x;
+        s_by[li] = y;
+        s_bz[li] = z;
         const bool more = j + 1 < d.n;
         const double x1 = more ? d.dx[j + 1] : 0.0, y1 = more ? d.dy[j + 1] : 0.0,
                      z1 = more ? d.dz[j + 1] : 0.0;
@@ -1460,21 +1482,11 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
         // (octree.rs:69-80); the device reports them instead
         if (more && fabs(x - x1) < 1e-9 && fabs(y - y1) < 1e-9 && fabs(z - z1) < 1e-9)
             atomicOr(&d.st->err, kErrCoincident);
-        atomicMax(&s_maxl, hi);
         // read first: once the running maximum is reached almost no lane
         // needs the atomic (a stale read only costs an extra atomic)
         if (d.count && hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(&d.st->max_level, hi);
-        if (in_lds) {
-            if (fn - 1u < n1) {
-                const uint32_t leaf = fn - 1u - n0;
-                s_m[leaf] = m;
-                s_x[leaf] = x;
-                s_y[leaf] = y;
-                s_z[leaf] = z;
-            }
-        }
     }
     __syncthreads();
 
@@ -1505,80 +1517,60 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
         }
         return s_first[last + 1 - b0];
     };
-    // a tile-spanning node is the only one of its level that starts in the
-    // tile and leaves it
-    auto record_span = [&](uint32_t nd, int L, uint64_t last, uint32_t skip) {
-        if (L >= kTopSlots) return;
-        top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
-        top_last[(uint64_t)tile * kTopSlots + L] = (uint32_t)last;
-        s_spanskip[L] = skip;
-        atomicMax(&s_topl, (uint32_t)L + 1u);
+    // The record of node nd (level L, owned by local body li, geometric
+    // centre c, octant oct). A tile-spanning node (the only one of its level
+    // that starts in the tile and leaves it) gets zero mass here and its own
+    // from k_com_span; an internal node of more than kSmallNode bodies is
+    // queued, its mass words written by the wave pass.
+    auto emit = [&](uint32_t nd, uint32_t li, int L, bool leaf, double cx, double cy, double cz,
+                    uint32_t oct) {
+        uint32_t skip;
+        Sum4 c{0.0, 0.0, 0.0, 0.0};
+        bool queued = false;
+        if (leaf) {
+            skip = s_first[li + 1];
+            c = Sum4{s_bm[li], s_bx[li], s_by[li], s_bz[li]};
+        } else {
+            uint64_t last;
+            bool spans;
+            skip = internal_skip(b0 + li, s_keys[li], L, last, spans);
+            if (spans) {
+                top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
+                top_last[(uint64_t)tile * kTopSlots + L] = (uint32_t)last;
+                atomicMax(&s_topl, (uint32_t)L + 1u);
+            } else {
+                const uint32_t e = (uint32_t)(last - b0) + 1u;
+                if (e - li > kSmallNode) {
+                    queued = true;
+                    s_big[atomicAdd(&s_nbig, 1u)] = li | ((e - 1u) << 10) | ((uint32_t)L << 20);
+                } else {
+                    c = com_of(range_sum(s_bm, s_bx, s_by, s_bz, li, e));
+                }
+            }
+        }
+        uint2* w = reinterpret_cast<uint2*>(hot + nd);
+        const uint4 tail = pack_tail(c.m, skip, (uint32_t)L, oct);
+        if (queued) {
+            reinterpret_cast<uint4*>(w)[0] = pack2(cx, cy);
+            w[2] = pack1(cz);
+            w[7] = make_uint2(tail.z, tail.w);
+        } else {
+            uint4* rec = reinterpret_cast<uint4*>(w);
+            rec[0] = pack2(cx, cy);
+            rec[1] = pack2(cz, c.x);
+            rec[2] = pack2(c.y, c.z);
+            rec[3] = tail;
+        }
     };
 
-    if (in_lds) {
-        // Nodes: level, range, skip and tag; spanning nodes get zero mass
-        // here and theirs from k_com_span.
+    if (node_par) {
+        // one thread per node: the centre by the reference's descent from
+        // the segment's root
         for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
             const uint32_t li = s_own[k];
-            const uint32_t nbase = s_first[li], nnext = s_first[li + 1];
+            const uint32_t nbase = s_first[li];
             const int L = (int)s_lo[li] + (int)(n0 + k - nbase);
-            const bool leaf = n0 + k + 1u == nnext;
-            bool spans = false;
-            uint32_t skip = nnext;
-            if (!leaf) {
-                uint64_t last;
-                skip = internal_skip(b0 + li, s_keys[li], L, last, spans);
-                s_m[k] = 0.0;
-                s_x[k] = 0.0;
-                s_y[k] = 0.0;
-                s_z[k] = 0.0;
-                if (spans) record_span(n0 + k, L, last, skip);
-            }
-            s_skip[k] = (uint16_t)min(skip - n0, nloc);
-            s_tag[k] = (uint8_t)L | (leaf || spans ? 0 : kTagInternal);
-        }
-        __syncthreads();
-        // internal nodes sit above the deepest leaf; a thread's (at most
-        // kNodeLds / kNodeThreads) node tags stay in registers
-        const int maxl = (int)min(s_maxl, (uint32_t)kTopSlots);
-        constexpr int kPerThread = kNodeLds / kNodeThreads + (kNodeLds % kNodeThreads ? 1 : 0);
-        uint8_t mytag[kPerThread];
-#pragma unroll
-        for (int r = 0; r < kPerThread; ++r) {
-            const uint32_t k = threadIdx.x + (uint32_t)r * kNodeThreads;
-            mytag[r] = k < nloc ? s_tag[k] : (uint8_t)0;
-        }
-        for (int L = maxl - 1; L >= 0; --L) {
-            const uint8_t want = (uint8_t)L | kTagInternal;
-#pragma unroll
-            for (int r = 0; r < kPerThread; ++r) {
-                if (mytag[r] != want) continue;
-                const uint32_t k = threadIdx.x + (uint32_t)r * kNodeThreads;
-                double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
-                const uint32_t send = s_skip[k];
-                for (uint32_t c = k + 1u; c < send;) {
-                    const double m = s_m[c];
-                    M += m;
-                    X += m * s_x[c];
-                    Y += m * s_y[c];
-                    Z += m * s_z[c];
-                    const uint32_t nxt = s_skip[c];
-                    c = nxt > c ? nxt : c + 1u;
-                }
-                const double inv = 1.0 / M;
-                s_m[k] = M;
-                s_x[k] = X * inv;
-                s_y[k] = Y * inv;
-                s_z[k] = Z * inv;
-            }
-            __syncthreads();
-        }
-        // Records, one per thread: the centre by the reference's descent
-        // from the segment's root, the rest from LDS.
-        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
-            const uint32_t li = s_own[k];
-            const uint8_t tag = s_tag[k];
-            const int L = tag & kTagLevel;
+            const bool leaf = n0 + k + 1u == s_first[li + 1];
             const uint64_t key = s_keys[li];
             const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
             double ext = ext0, cx = 0.0, cy = 0.0, cz = 0.0;
@@ -1591,27 +1583,17 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 ext = h2;
             }
             const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
-            // a skip out of the tile: the next tile's first node, or the
-            // recorded one of a spanning node (neither leaf nor internal)
-            const uint32_t sk = s_skip[k];
-            const bool spanning = !(tag & kTagInternal) && n0 + k + 1u != s_first[li + 1];
-            const uint32_t skip = sk < nloc ? n0 + sk : spanning ? s_spanskip[L] : nend;
-            uint4* rec = reinterpret_cast<uint4*>(hot + n0 + k);
-            rec[0] = pack2(cx, cy);
-            rec[1] = pack2(cz, s_x[k]);
-            rec[2] = pack2(s_y[k], s_z[k]);
-            rec[3] = pack_tail(s_m[k], skip, (uint32_t)L, oct);
+            emit(n0 + k, li, L, leaf, cx, cy, cz, oct);
         }
     } else {
-        // More nodes than LDS holds: body by body into the global records.
+        // one thread per body: one descent through the body's nodes
         for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
-            const uint64_t i = b0 + li;
             const uint64_t key = s_keys[li];
             const int lo = s_lo[li];
             const uint32_t nbase = s_first[li];
             const int cnt = (int)(s_first[li + 1] - nbase);
             const int hi = lo + cnt - 1;
-            const uint64_t kx = hi > lmax ? k2[i] : 0ull;
+            const uint64_t kx = hi > lmax ? k2[b0 + li] : 0ull;
             double ext = ext0;
             double cx = 0.0, cy = 0.0, cz = 0.0;
             for (int L = 0; L < lo; ++L) {
@@ -1626,25 +1608,8 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 const int L = lo + k;
                 const uint64_t nd = (uint64_t)nbase + (uint64_t)k;
                 if (nd >= cap) break;  // reported above
-                NodeHot h;
-                h.cx = cx; h.cy = cy; h.cz = cz;
-                if (L == hi) {
-                    const uint64_t j = i - d.off;
-                    h.mx = d.dx[j]; h.my = d.dy[j]; h.mz = d.dz[j];
-                    h.mass = d.dm[j];
-                    h.skip = s_first[li + 1];
-                } else {
-                    h.mx = h.my = h.mz = 0.0;
-                    h.mass = 0.0;
-                    uint64_t last;
-                    bool spans;
-                    h.skip = internal_skip(i, key, L, last, spans);
-                    if (spans) record_span((uint32_t)nd, L, last, h.skip);
-                }
-                h.level = (uint16_t)L;
-                h.octant = L > 0 ? (uint8_t)key_digit<DIM>(key, kx, lmax, L - 1) : 0;
-                h.pad = 0;
-                store_node(hot + nd, h);
+                const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
+                emit((uint32_t)nd, li, L, k + 1 == cnt, cx, cy, cz, oct);
                 if (k + 1 < cnt) {
                     const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
                     const double h2 = half(ext);
@@ -1655,22 +1620,7 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 }
             }
         }
-        __threadfence();
-        __syncthreads();
-        // the same level loop on the global records, made visible across the
-        // block by device-scope fences
-        const int maxl = (int)min(s_maxl, (uint32_t)kTopSlots);
-        for (int L = maxl - 1; L >= 0; --L) {
-            for (uint32_t nd = n0 + threadIdx.x; nd < n1; nd += kNodeThreads) {
-                // internal and inside the tile: its subtree ends by the tile's last node
-                if ((int)hot[nd].level == L && hot[nd].skip != nd + 1u && hot[nd].skip <= n1)
-                    com_from_children(hot, nd);
-            }
-            __threadfence();
-            __syncthreads();
-        }
     }
-    if (threadIdx.x == 0 && s_topl) atomicMax(&sc->top_levels, s_topl);
 
     // Partial mass sums for the tile-spanning nodes (k_com_span): the tile
     // total; per level L, the suffix from the first body of the level-L
@@ -1678,19 +1628,14 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
     // tile and runs into the next one (out_part); and the prefix up to the
     // end of the level-L ancestor of the tile's first body, where that one
     // started before the tile and ends in it (in_part). Bodies come in
-    // contiguous pairs per thread; every sum has a fixed association.
+    // contiguous pairs per thread; every sum has a fixed association. (The
+    // scans' barriers also order the node pass before the wave pass below.)
     // mass sums and level of local body li (zero / ~0 when absent)
     auto body_sum = [&](uint32_t li) -> Sum4 {
         if (li >= nb) return Sum4{0.0, 0.0, 0.0, 0.0};
-        const uint32_t leaf = s_first[li + 1] - 1u;
-        if (leaf >= n1) return Sum4{0.0, 0.0, 0.0, 0.0};
-        double m, x, y, z;
-        if (in_lds) {
-            m = s_m[leaf - n0]; x = s_x[leaf - n0]; y = s_y[leaf - n0]; z = s_z[leaf - n0];
-        } else {
-            m = hot[leaf].mass; x = hot[leaf].mx; y = hot[leaf].my; z = hot[leaf].mz;
-        }
-        return Sum4{m, m * x, m * y, m * z};
+        if (s_first[li + 1] - 1u >= n1) return Sum4{0.0, 0.0, 0.0, 0.0};
+        const double m = s_bm[li];
+        return Sum4{m, m * s_bx[li], m * s_by[li], m * s_bz[li]};
     };
     auto body_lo = [&](uint32_t li) -> uint32_t { return li < nb ? (uint32_t)s_lo[li] : ~0u; };
     const uint32_t l0 = 2u * threadIdx.x, l1 = l0 + 1u;
@@ -1749,6 +1694,22 @@ __global__ __launch_bounds__(kNodeThreads) void k_nodes(
                 for (int L = (int)body_lo(l0); L <= lam1 && L < mn0; ++L)
                     out_part[(uint64_t)tile * kTopSlots + L] = sf;
             }
+        }
+    }
+    if (threadIdx.x == 0 && s_topl) atomicMax(&sc->top_levels, s_topl);
+
+    // Queued internal nodes, one wave each: mass, centre of mass.
+    const uint32_t nbig = s_nbig;
+    for (uint32_t q = threadIdx.x >> 6; q < nbig; q += kNodeThreads / 64) {
+        const uint32_t ent = s_big[q];
+        const uint32_t li = ent & 1023u, lastl = (ent >> 10) & 1023u;
+        const int L = (int)(ent >> 20);
+        const Sum4 c = com_of(range_sum_wave(s_bm, s_bx, s_by, s_bz, li, lastl + 1u));
+        if ((threadIdx.x & 63u) == 0) {
+            uint2* w = reinterpret_cast<uint2*>(hot + s_first[li] + (uint32_t)(L - (int)s_lo[li]));
+            w[3] = pack1(c.x);
+            reinterpret_cast<uint4*>(w)[2] = pack2(c.y, c.z);
+            w[6] = pack1(c.m);
         }
     }
 }

@@ -142,6 +142,35 @@ def test_pairs_deeper_than_one_key_word(engine, precision):
     assert np.all(np.isfinite(sim.download()["x"]))
 
 
+def test_clustered_tile_tree(engine):
+    """Close pairs throughout a tile (each pair ~2e-7 apart, ~24 levels deep):
+    the tile has far more nodes than bodies, so it is built body by body, and
+    its upper cells hold hundreds of bodies (their sums taken by a wave).
+    Topology and centres bit-exact, masses and centres of mass to rounding,
+    accelerations as for any other state."""
+    rng = np.random.default_rng(21)
+    s = small_cube(700, seed=21)
+    twin = s.copy()
+    for ax in ("x", "y", "z"):
+        twin[ax] += rng.uniform(-1e-7, 1e-7, len(s)) + np.where(rng.random(len(s)) < 0.5, -2e-8, 2e-8)
+    s = P.concat([s, twin])
+    sim = P.Sim(engine, s, kind="astro2", precision="fp64", **CFG1)
+    a = sim.accel()
+    rec, fl = sim.tree()
+    t = O.Tree.from_state(s, 3)
+    orec, ofl = t.serialize()
+    assert t.merges == 0 and t.fault == 0
+    assert len(rec) > 4 * len(s)  # the first tile alone exceeds the node-parallel table
+    np.testing.assert_array_equal(rec[:, :3], orec[:, :3])
+    np.testing.assert_array_equal(fl[:, 4:8], ofl[:, 4:8])
+    np.testing.assert_allclose(fl[:, 0], ofl[:, 0], rtol=1e-13)
+    np.testing.assert_allclose(fl[:, 1:4], ofl[:, 1:4], rtol=1e-13, atol=1e-15)
+    leaf = rec[:, 2] >= 0
+    np.testing.assert_array_equal(fl[leaf, 1:4], ofl[leaf, 1:4])
+    ref, _, _ = O.bh_transform(s, CFG1["theta"], CFG1["e"])
+    assert fp64_ok(rel_err(a, ref))
+
+
 def octant_path_key(p, extent, dim, levels):
     c = [0.0, 0.0, 0.0]
     h = extent
