@@ -1339,36 +1339,87 @@ __device__ __forceinline__ Sum4 range_sum(const double* s_m, const double* s_x, 
     return r;
 }
 
-// The same sums by one wave: lane l folds bodies a + l, a + l + 64, ... in
-// order, then a butterfly over the lanes; lane 0's association is fixed.
-__device__ __forceinline__ Sum4 range_sum_wave(const double* s_m, const double* s_x,
-                                               const double* s_y, const double* s_z, uint32_t a,
-                                               uint32_t e) {
+// r += r of lane ^ M.
+template <int M>
+__device__ __forceinline__ void fold_lanes(Sum4& r) {
+    r.m += bits_f64(xor_lanes64<M>(f64_bits(r.m)));
+    r.x += bits_f64(xor_lanes64<M>(f64_bits(r.x)));
+    r.y += bits_f64(xor_lanes64<M>(f64_bits(r.y)));
+    r.z += bits_f64(xor_lanes64<M>(f64_bits(r.z)));
+}
+
+// The same sums by a group of 16 lanes (a DPP row): lane l of the group
+// folds bodies a + l, a + l + 16, ... in order, then a butterfly over the
+// group; the group's lane 0 holds a fixed association.
+__device__ __forceinline__ Sum4 range_sum_row(const double* s_m, const double* s_x,
+                                              const double* s_y, const double* s_z, uint32_t a,
+                                              uint32_t e) {
     Sum4 r{0.0, 0.0, 0.0, 0.0};
-    for (uint32_t b = a + (threadIdx.x & 63u); b < e; b += 64u) {
+    for (uint32_t b = a + (threadIdx.x & 15u); b < e; b += 16u) {
         const double m = s_m[b];
         r.m += m;
         r.x += m * s_x[b];
         r.y += m * s_y[b];
         r.z += m * s_z[b];
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        r.m += __shfl_xor(r.m, o);
-        r.x += __shfl_xor(r.x, o);
-        r.y += __shfl_xor(r.y, o);
-        r.z += __shfl_xor(r.z, o);
-    }
+    fold_lanes<8>(r);
+    fold_lanes<4>(r);
+    fold_lanes<2>(r);
+    fold_lanes<1>(r);
     return r;
 }
 
-// Internal nodes of more bodies than this have their sums taken by a wave.
+// The reference's descent to a cell (child centres, octree.rs:160-215 /
+// quadtree.rs:161-194):
+// n levels down from a cell of half-width ext and centre c, the half-width
+// halves and the centre moves by it along each axis, up where the key's
+// octant bit is set, else down. w holds the digits to follow inverted and
+// left-aligned (the next digit's DIM bits on top, x lowest), so a digit's
+// inverted bit is the sign of that axis' move; c + (-h) is c - h exactly.
+template <int DIM>
+__device__ __forceinline__ void descend_w(uint64_t w, int n, double& ext, double& cx, double& cy,
+                                          double& cz) {
+    for (int l = 0; l < n; ++l) {
+        const uint32_t top = (uint32_t)(w >> 32);
+        ext = half(ext);
+        const uint64_t hb = f64_bits(ext);
+        const uint32_t lo = (uint32_t)hb, hi = (uint32_t)(hb >> 32);
+        cx += __hiloint2double((int)(hi | ((top << (DIM - 1)) & 0x80000000u)), (int)lo);
+        cy += __hiloint2double((int)(hi | ((top << (DIM - 2)) & 0x80000000u)), (int)lo);
+        if (DIM == 3) cz += __hiloint2double((int)(hi | (top & 0x80000000u)), (int)lo);
+        w <<= DIM;
+    }
+}
+// Levels [from, to) of a body's path: the first key word's digits, then the
+// second's (levels past lmax, bodies whose first words tie).
+template <int DIM>
+__device__ __forceinline__ void descend(uint64_t key, uint64_t kx, int lmax, int from, int to,
+                                        double& ext, double& cx, double& cy, double& cz) {
+    const int a = min(to, lmax);
+    if (from < a) descend_w<DIM>(~key << (64 - DIM * (lmax - from)), a - from, ext, cx, cy, cz);
+    if (to > lmax) {
+        const int f2 = max(from, lmax);
+        descend_w<DIM>(~kx << (64 - DIM * (2 * lmax - f2)), to - f2, ext, cx, cy, cz);
+    }
+}
+// Levels of a tile's common path kept per tile (the rest descend from there).
+constexpr int kPathLevels = 16;
+
+// Internal nodes of more bodies than this (one thread per node) or than
+// kSmallNode (one thread per body) are queued and summed by lane groups.
+constexpr uint32_t kTinyNode = 8;
 constexpr uint32_t kSmallNode = 64;
-// Queue of such nodes per tile: at one level they hold disjoint body sets,
-// so a tile has at most kNodeTile / (kSmallNode + 1) per level.
-constexpr int kBigNodes = 1024;
-static_assert(kTopSlots * (kNodeTile / (kSmallNode + 1)) <= kBigNodes, "big-node queue bound");
-static_assert(kNodeTile <= 1024, "queue entries hold 10-bit body indices");
+// The queue: one thread per node, a tile has at most kNodeLds nodes of which
+// kNodeTile are leaves; an entry is the local node (11 bits) and the range's
+// last body (10 bits). One thread per body: internal nodes of more than
+// kSmallNode bodies hold disjoint body sets at one level, so a tile has at
+// most kNodeTile / (kSmallNode + 1) per level; an entry (16-bit, two per
+// word) is the owning body (10 bits) and the level (6 bits: an internal node
+// lies above a leaf, so below the deepest level, 64).
+constexpr int kNodeQueue = kNodeLds - kNodeTile;
+static_assert(kNodeLds <= 2048 && kNodeTile <= 1024, "queue entries: 11-bit node, 10-bit body");
+static_assert(kTopSlots * (kNodeTile / (kSmallNode + 1)) <= 2 * kNodeQueue && kTopSlots <= 65,
+              "queue bound of the body-parallel build");
 
 // A node record as four 16-byte chunks: (cx, cy), (cz, mx), (my, mz),
 // (mass, skip, level | octant << 16), or as eight 8-byte words.
@@ -1431,9 +1482,11 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     __shared__ uint8_t s_lo[kNodeTile];
     __shared__ double s_bm[kNodeTile], s_bx[kNodeTile], s_by[kNodeTile], s_bz[kNodeTile];
     __shared__ uint16_t s_own[kNodeLds];
-    __shared__ uint32_t s_big[kBigNodes];
+    __shared__ uint32_t s_q[kNodeQueue];  // queued internal nodes (see above)
+    __shared__ double s_path[kPathLevels][4];  // half-width and centre along the common path
     __shared__ uint64_t s_next;  // key just after the tile
-    __shared__ uint32_t s_topl, s_nbig;
+    __shared__ uint32_t s_topl, s_nq;
+    __shared__ int s_lt;  // levels of the path shared by all the tile's bodies (capped)
     __shared__ Sum4 s_w4[kNodeThreads / 64];
     __shared__ uint32_t s_wmin[kNodeThreads / 64];
     const uint64_t gen = (uint64_t)sc->gen << 32;
@@ -1452,7 +1505,21 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         s_first[nb] = nend;
         s_next = b1 < total ? skeys_g[b1] : 0ull;  // read only when the segment continues past b1
         s_topl = 0;
-        s_nbig = 0;
+        s_nq = 0;
+        // every node of the tile lies on or below this path: its cells are
+        // the ancestors of every body
+        const uint64_t ka = nb ? skeys_g[b0] : 0ull, kz = nb ? skeys_g[b1 - 1] : 0ull;
+        const int lt = min(key_lcp<DIM>(ka, kz, lmax), kPathLevels - 1);
+        double e = ext0, x = 0.0, y = 0.0, z = 0.0;
+        for (int l = 0;; ++l) {
+            s_path[l][0] = e;
+            s_path[l][1] = x;
+            s_path[l][2] = y;
+            s_path[l][3] = z;
+            if (l == lt) break;
+            descend<DIM>(ka, 0ull, lmax, l, l + 1, e, x, y, z);
+        }
+        s_lt = lt;
     }
     // Bodies: staging, the node -> owner table, checks.
     for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
@@ -1489,6 +1556,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             atomicMax(&d.st->max_level, hi);
     }
     __syncthreads();
+    const int lt = s_lt;
 
     // Do keys kj (body j) and key (with second word kx) share `level` levels?
     auto same_at = [&](uint64_t kj, uint64_t j, uint64_t key, uint64_t kx, int level) -> bool {
@@ -1520,8 +1588,8 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     // The record of node nd (level L, owned by local body li, geometric
     // centre c, octant oct). A tile-spanning node (the only one of its level
     // that starts in the tile and leaves it) gets zero mass here and its own
-    // from k_com_span; an internal node of more than kSmallNode bodies is
-    // queued, its mass words written by the wave pass.
+    // from k_com_span; a larger internal node is queued, its mass words
+    // written by the group pass.
     auto emit = [&](uint32_t nd, uint32_t li, int L, bool leaf, double cx, double cy, double cz,
                     uint32_t oct) {
         uint32_t skip;
@@ -1540,12 +1608,19 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
                 atomicMax(&s_topl, (uint32_t)L + 1u);
             } else {
                 const uint32_t e = (uint32_t)(last - b0) + 1u;
-                if (e - li > kSmallNode) {
-                    queued = true;
-                    s_big[atomicAdd(&s_nbig, 1u)] = li | ((e - 1u) << 10) | ((uint32_t)L << 20);
-                } else {
-                    c = com_of(range_sum(s_bm, s_bx, s_by, s_bz, li, e));
+                if (e - li > (node_par ? kTinyNode : kSmallNode)) {
+                    // (the bounds above hold unless the node capacity cut
+                    // the tile short, an error; then the thread sums)
+                    const uint32_t q = atomicAdd(&s_nq, 1u);
+                    if (node_par && q < (uint32_t)kNodeQueue) {
+                        s_q[q] = (nd - n0) | ((e - 1u) << 11);
+                        queued = true;
+                    } else if (!node_par && q < 2u * kNodeQueue) {
+                        reinterpret_cast<uint16_t*>(s_q)[q] = (uint16_t)(li | ((uint32_t)L << 10));
+                        queued = true;
+                    }
                 }
+                if (!queued) c = com_of(range_sum(s_bm, s_bx, s_by, s_bz, li, e));
             }
         }
         uint2* w = reinterpret_cast<uint2*>(hot + nd);
@@ -1573,15 +1648,9 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             const bool leaf = n0 + k + 1u == s_first[li + 1];
             const uint64_t key = s_keys[li];
             const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
-            double ext = ext0, cx = 0.0, cy = 0.0, cz = 0.0;
-            for (int l = 0; l < L; ++l) {
-                const uint32_t o = key_digit<DIM>(key, kx, lmax, l);
-                const double h2 = half(ext);
-                cx = (o & 1u) ? cx + h2 : cx - h2;
-                cy = (o & 2u) ? cy + h2 : cy - h2;
-                if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-                ext = h2;
-            }
+            const int f = min(L, lt);
+            double ext = s_path[f][0], cx = s_path[f][1], cy = s_path[f][2], cz = s_path[f][3];
+            descend<DIM>(key, kx, lmax, f, L, ext, cx, cy, cz);
             const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
             emit(n0 + k, li, L, leaf, cx, cy, cz, oct);
         }
@@ -1594,30 +1663,16 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             const int cnt = (int)(s_first[li + 1] - nbase);
             const int hi = lo + cnt - 1;
             const uint64_t kx = hi > lmax ? k2[b0 + li] : 0ull;
-            double ext = ext0;
-            double cx = 0.0, cy = 0.0, cz = 0.0;
-            for (int L = 0; L < lo; ++L) {
-                const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
-                const double h2 = half(ext);
-                cx = (o & 1u) ? cx + h2 : cx - h2;
-                cy = (o & 2u) ? cy + h2 : cy - h2;
-                if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-                ext = h2;
-            }
+            const int f = min(lo, lt);
+            double ext = s_path[f][0], cx = s_path[f][1], cy = s_path[f][2], cz = s_path[f][3];
+            descend<DIM>(key, kx, lmax, f, lo, ext, cx, cy, cz);
             for (int k = 0; k < cnt; ++k) {
                 const int L = lo + k;
                 const uint64_t nd = (uint64_t)nbase + (uint64_t)k;
                 if (nd >= cap) break;  // reported above
                 const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
                 emit((uint32_t)nd, li, L, k + 1 == cnt, cx, cy, cz, oct);
-                if (k + 1 < cnt) {
-                    const uint32_t o = key_digit<DIM>(key, kx, lmax, L);
-                    const double h2 = half(ext);
-                    cx = (o & 1u) ? cx + h2 : cx - h2;
-                    cy = (o & 2u) ? cy + h2 : cy - h2;
-                    if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-                    ext = h2;
-                }
+                if (k + 1 < cnt) descend<DIM>(key, kx, lmax, L, L + 1, ext, cx, cy, cz);
             }
         }
     }
@@ -1698,15 +1753,30 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     }
     if (threadIdx.x == 0 && s_topl) atomicMax(&sc->top_levels, s_topl);
 
-    // Queued internal nodes, one wave each: mass, centre of mass.
-    const uint32_t nbig = s_nbig;
-    for (uint32_t q = threadIdx.x >> 6; q < nbig; q += kNodeThreads / 64) {
-        const uint32_t ent = s_big[q];
-        const uint32_t li = ent & 1023u, lastl = (ent >> 10) & 1023u;
-        const int L = (int)(ent >> 20);
-        const Sum4 c = com_of(range_sum_wave(s_bm, s_bx, s_by, s_bz, li, lastl + 1u));
-        if ((threadIdx.x & 63u) == 0) {
-            uint2* w = reinterpret_cast<uint2*>(hot + s_first[li] + (uint32_t)(L - (int)s_lo[li]));
+    // Queued internal nodes, one group of 16 lanes each: mass, centre of
+    // mass.
+    const uint32_t nq = min(s_nq, node_par ? (uint32_t)kNodeQueue : 2u * kNodeQueue);
+    for (uint32_t q = threadIdx.x >> 4; q < nq; q += kNodeThreads / 16) {
+        uint32_t li, e, nd;
+        if (node_par) {
+            const uint32_t ent = s_q[q];
+            const uint32_t k = ent & 2047u;
+            li = s_own[k];
+            e = (ent >> 11) + 1u;
+            nd = n0 + k;
+        } else {
+            const uint32_t ent = reinterpret_cast<const uint16_t*>(s_q)[q];
+            li = ent & 1023u;
+            const int L = (int)(ent >> 10);
+            uint64_t last;
+            bool spans;
+            internal_skip(b0 + li, s_keys[li], L, last, spans);  // (within the tile)
+            e = (uint32_t)(last - b0) + 1u;
+            nd = s_first[li] + (uint32_t)(L - (int)s_lo[li]);
+        }
+        const Sum4 c = com_of(range_sum_row(s_bm, s_bx, s_by, s_bz, li, e));
+        if ((threadIdx.x & 15u) == 0) {
+            uint2* w = reinterpret_cast<uint2*>(hot + nd);
             w[3] = pack1(c.x);
             reinterpret_cast<uint4*>(w)[2] = pack2(c.y, c.z);
             w[6] = pack1(c.m);
