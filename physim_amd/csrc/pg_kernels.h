@@ -1258,65 +1258,83 @@ struct Sum4 {
     }
 };
 
-__device__ __forceinline__ Sum4 shfl_dir4(const Sum4& v, int o, bool down) {
-    Sum4 r;
-    r.m = down ? __shfl_down(v.m, o) : __shfl_up(v.m, o);
-    r.x = down ? __shfl_down(v.x, o) : __shfl_up(v.x, o);
-    r.y = down ? __shfl_down(v.y, o) : __shfl_up(v.y, o);
-    r.z = down ? __shfl_down(v.z, o) : __shfl_up(v.z, o);
-    return r;
+// DPP moves across a wave's lanes (CTRL: the DPP control, ROWS: the rows
+// written); a lane without a source, or in a row not written, takes old.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t b = f64_bits(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xF, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xF, false);
+    return bits_f64(((uint64_t)hi << 32) | lo);  // old: +0.0
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ Sum4 dpp_sum4(const Sum4& v) {
+    return Sum4{dpp_f64<CTRL, ROWS>(v.m), dpp_f64<CTRL, ROWS>(v.x), dpp_f64<CTRL, ROWS>(v.y),
+                dpp_f64<CTRL, ROWS>(v.z)};
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, CTRL, ROWS, 0xF, false);  // old: ~0u
+}
+// Inclusive sum and minimum over a wave's lanes in lane order: steps of 1, 2,
+// 4 and 8 lanes within rows of 16 (row_shr), then rows 1 and 3 take the last
+// lane of the row before (row_bcast:15), rows 2 and 3 lane 31 (row_bcast:31).
+// The association is fixed.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void scan_step(Sum4& v, uint32_t& m) {
+    v.add(dpp_sum4<CTRL, ROWS>(v));
+    m = min(m, dpp_u32<CTRL, ROWS>(m));
+}
+__device__ __forceinline__ void wave_incl_scan(Sum4& v, uint32_t& m) {
+    scan_step<0x111, 0xF>(v, m);
+    scan_step<0x112, 0xF>(v, m);
+    scan_step<0x114, 0xF>(v, m);
+    scan_step<0x118, 0xF>(v, m);
+    scan_step<0x142, 0xA>(v, m);
+    scan_step<0x143, 0xC>(v, m);
 }
 
-// Exclusive scan of one Sum4 per thread over a block of NT threads, in
-// thread order (REV: from the last thread down), and the block total. Fixed
-// association, so the result is deterministic.
-template <int NT, bool REV>
-__device__ __forceinline__ Sum4 block_excl_scan4(Sum4 v, Sum4* s_wave, Sum4* total) {
+// Exclusive scans of two (Sum4, u32) pairs per thread over a block of NT
+// threads in thread order: sums (fixed association) and minimums (~0u for
+// the first thread), and the first pair's sum over the block.
+template <int NT>
+__device__ __forceinline__ void block_excl_scan2(Sum4& a, uint32_t& ma, Sum4& b, uint32_t& mb,
+                                                 Sum4 (*s_w)[NT / 64], uint32_t (*s_m)[NT / 64],
+                                                 Sum4* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    Sum4 inc = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const Sum4 t = shfl_dir4(inc, o, REV);
-        if (REV ? lane + o < 64 : lane >= o) inc.add(t);
+    wave_incl_scan(a, ma);
+    wave_incl_scan(b, mb);
+    if (lane == 63) {
+        s_w[0][wave] = a;
+        s_w[1][wave] = b;
+        s_m[0][wave] = ma;
+        s_m[1][wave] = mb;
     }
-    Sum4 ex = shfl_dir4(inc, 1, REV);
-    if (REV ? lane == 63 : lane == 0) ex = Sum4{0.0, 0.0, 0.0, 0.0};
-    if (REV ? lane == 0 : lane == 63) s_wave[wave] = inc;
+    // exclusive within the wave: the lane before's inclusive value (wave_shr:1)
+    Sum4 xa = dpp_sum4<0x138, 0xF>(a), xb = dpp_sum4<0x138, 0xF>(b);
+    uint32_t ya = dpp_u32<0x138, 0xF>(ma), yb = dpp_u32<0x138, 0xF>(mb);
     __syncthreads();
-    Sum4 base{0.0, 0.0, 0.0, 0.0}, tot{0.0, 0.0, 0.0, 0.0};
+    Sum4 ba{0.0, 0.0, 0.0, 0.0}, bb{0.0, 0.0, 0.0, 0.0}, tot{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
     for (int w = 0; w < NT / 64; ++w) {
-        const int ww = REV ? NT / 64 - 1 - w : w;
-        const Sum4 c = s_wave[ww];
-        if (REV ? ww > wave : ww < wave) base.add(c);
-        tot.add(c);
+        const Sum4 ca = s_w[0][w];
+        if (w < wave) {
+            ba.add(ca);
+            bb.add(s_w[1][w]);
+            ya = min(ya, s_m[0][w]);
+            yb = min(yb, s_m[1][w]);
+        }
+        tot.add(ca);
     }
     __syncthreads();
+    ba.add(xa);
+    bb.add(xb);
+    a = ba;
+    b = bb;
+    ma = ya;
+    mb = yb;
     *total = tot;
-    base.add(ex);
-    return base;
-}
-
-// Exclusive minimum of one u32 per thread over a block of NT threads, in
-// thread order or (REV) from the last thread down; ~0u for the first.
-template <int NT, bool REV>
-__device__ __forceinline__ uint32_t block_excl_min(uint32_t v, uint32_t* s_wave) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = REV ? __shfl_down(inc, o) : __shfl_up(inc, o);
-        if (REV ? lane + o < 64 : lane >= o) inc = min(inc, t);
-    }
-    uint32_t ex = REV ? __shfl_down(inc, 1) : __shfl_up(inc, 1);
-    if (REV ? lane == 63 : lane == 0) ex = ~0u;
-    if (REV ? lane == 0 : lane == 63) s_wave[wave] = inc;
-    __syncthreads();
-#pragma unroll 1
-    for (int w = 0; w < NT / 64; ++w)
-        if (REV ? w > wave : w < wave) ex = min(ex, s_wave[w]);
-    __syncthreads();
-    return ex;
 }
 
 // Mass and centre of mass of a node from the mass sums of its bodies.
@@ -1403,7 +1421,7 @@ __device__ __forceinline__ void descend(uint64_t key, uint64_t kx, int lmax, int
     }
 }
 // Levels of a tile's common path kept per tile (the rest descend from there).
-constexpr int kPathLevels = 16;
+constexpr int kPathLevels = 12;
 
 // Internal nodes of more bodies than this (one thread per node) or than
 // kSmallNode (one thread per body) are queued and summed by lane groups.
@@ -1487,8 +1505,8 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     __shared__ uint64_t s_next;  // key just after the tile
     __shared__ uint32_t s_topl, s_nq;
     __shared__ int s_lt;  // levels of the path shared by all the tile's bodies (capped)
-    __shared__ Sum4 s_w4[kNodeThreads / 64];
-    __shared__ uint32_t s_wmin[kNodeThreads / 64];
+    __shared__ Sum4 s_w4[2][kNodeThreads / 64];
+    __shared__ uint32_t s_wmin[2][kNodeThreads / 64];
     const uint64_t gen = (uint64_t)sc->gen << 32;
     const uint32_t tile = blockIdx.x, tiles = gridDim.x;
     const uint64_t b0 = tile_start[tile], b1 = tile_start[tile + 1];
@@ -1684,7 +1702,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     // end of the level-L ancestor of the tile's first body, where that one
     // started before the tile and ends in it (in_part). Bodies come in
     // contiguous pairs per thread; every sum has a fixed association. (The
-    // scans' barriers also order the node pass before the wave pass below.)
+    // scans' barriers also order the node pass before the group pass below.)
     // mass sums and level of local body li (zero / ~0 when absent)
     auto body_sum = [&](uint32_t li) -> Sum4 {
         if (li >= nb) return Sum4{0.0, 0.0, 0.0, 0.0};
@@ -1693,17 +1711,20 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         return Sum4{m, m * s_bx[li], m * s_by[li], m * s_bz[li]};
     };
     auto body_lo = [&](uint32_t li) -> uint32_t { return li < nb ? (uint32_t)s_lo[li] : ~0u; };
+    // the prefix side takes bodies l0, l1 per thread, the suffix side the
+    // mirrored pair r0, r1, so both are scans in thread order
     const uint32_t l0 = 2u * threadIdx.x, l1 = l0 + 1u;
-    const uint32_t pair_lo = min(body_lo(l0), body_lo(l1));
+    const uint32_t r1 = (uint32_t)kNodeTile - 1u - 2u * threadIdx.x, r0 = r1 - 1u;
+    Sum4 pre = body_sum(l0), suf = body_sum(r0), total4;
+    pre.add(body_sum(l1));
+    suf.add(body_sum(r1));
+    uint32_t mnp = min(body_lo(l0), body_lo(l1)), mns = min(body_lo(r0), body_lo(r1));
+    block_excl_scan2<kNodeThreads>(pre, mnp, suf, mns, s_w4, s_wmin, &total4);
     {
         // prefix side: tile total, and in_part at the end of each ancestor
         // of the first body (levels <= lam0) that ends in the tile
-        Sum4 pair = body_sum(l0);
-        pair.add(body_sum(l1));
-        Sum4 total4;
-        const Sum4 pre = block_excl_scan4<kNodeThreads, false>(pair, s_w4, &total4);
         // (levels are at most 64: clamp "none" before comparing as int)
-        const int mn = (int)min(block_excl_min<kNodeThreads, false>(pair_lo, s_wmin), 255u);
+        const int mn = (int)min(mnp, 255u);
         if (threadIdx.x == 0) tot_part[tile] = total4;
         const int lam0 = nb ? (int)s_lo[0] - 1 : -1;
         // body li ends the level-L ancestor iff lo[li] <= L < every lo
@@ -1724,29 +1745,25 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     {
         // suffix side: out_part from the first body of each ancestor of the
         // last body (levels <= lam1) that starts in the tile and runs on
-        Sum4 pair = body_sum(l0);
-        pair.add(body_sum(l1));
-        Sum4 total4;
-        const Sum4 suf = block_excl_scan4<kNodeThreads, true>(pair, s_w4, &total4);
-        const int mn = (int)min(block_excl_min<kNodeThreads, true>(pair_lo, s_wmin), 255u);
+        const int mn = (int)min(mns, 255u);
         int lam1 = -1;
         if (nb && b1 < total && seg_of_key(s_next, segbits) == seg_of_key(s_keys[nb - 1], segbits))
             lam1 = min(key_lcp2<DIM>(s_keys[nb - 1], s_next, k2, b1 - 1, b1, lmax), kTopSlots - 1);
         // body li starts the level-L ancestor iff lo[li] <= L < every lo
         // after it
-        if (l1 < nb && (int)body_lo(l1) <= lam1 && (int)body_lo(l1) < mn) {
+        if (r1 < nb && (int)body_lo(r1) <= lam1 && (int)body_lo(r1) < mn) {
             Sum4 sf = suf;
-            sf.add(body_sum(l1));
-            for (int L = (int)body_lo(l1); L <= lam1 && L < mn; ++L)
+            sf.add(body_sum(r1));
+            for (int L = (int)body_lo(r1); L <= lam1 && L < mn; ++L)
                 out_part[(uint64_t)tile * kTopSlots + L] = sf;
         }
-        if (l0 < nb) {
-            const int mn0 = min(mn, (int)min(body_lo(l1), 255u));
-            if ((int)body_lo(l0) <= lam1 && (int)body_lo(l0) < mn0) {
+        if (r0 < nb) {
+            const int mn0 = min(mn, (int)min(body_lo(r1), 255u));
+            if ((int)body_lo(r0) <= lam1 && (int)body_lo(r0) < mn0) {
                 Sum4 sf = suf;
-                sf.add(body_sum(l1));
-                sf.add(body_sum(l0));
-                for (int L = (int)body_lo(l0); L <= lam1 && L < mn0; ++L)
+                sf.add(body_sum(r1));
+                sf.add(body_sum(r0));
+                for (int L = (int)body_lo(r0); L <= lam1 && L < mn0; ++L)
                     out_part[(uint64_t)tile * kTopSlots + L] = sf;
             }
         }
