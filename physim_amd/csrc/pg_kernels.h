@@ -93,18 +93,42 @@ __device__ __forceinline__ int key_lcp2(uint64_t a, uint64_t b, const uint64_t* 
     return l < lmax ? l : lmax + key_lcp<DIM>(k2[ia], k2[ib], lmax);
 }
 
+// DPP move across a wave's lanes (CTRL: the DPP control, ROWS: the rows
+// written); a lane without a source, or in a row not written, takes old.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t v, uint32_t old) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWS, 0xF, false);
+}
+constexpr int kDppWaveShr1 = 0x138;  // lane l takes lane l - 1 (lane 0: old)
+
+// Inclusive scan over a wave's lanes in lane order with an associative
+// operation of identity id: steps of 1, 2, 4 and 8 lanes within rows of 16
+// (row_shr), then rows 1 and 3 take the last lane of the row before
+// (row_bcast:15), rows 2 and 3 lane 31 (row_bcast:31).
+template <class Op>
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, uint32_t id, Op op) {
+    v = op(dpp_move<0x111, 0xF>(v, id), v);
+    v = op(dpp_move<0x112, 0xF>(v, id), v);
+    v = op(dpp_move<0x114, 0xF>(v, id), v);
+    v = op(dpp_move<0x118, 0xF>(v, id), v);
+    v = op(dpp_move<0x142, 0xA>(v, id), v);
+    v = op(dpp_move<0x143, 0xC>(v, id), v);
+    return v;
+}
+struct OpAdd {
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+};
+struct OpMin {
+    __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return min(a, b); }
+};
+
 // Exclusive scan of one value per thread over a block of NT threads
 // (s_wave holds NT / 64 values).
 template <int NT = kBlock>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave,
                                                          uint32_t* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(inc, o);
-        if (lane >= o) inc += t;
-    }
+    const uint32_t inc = wave_incl_scan_u32(v, 0u, OpAdd{});
     if (lane == 63) s_wave[wave] = inc;
     __syncthreads();
     uint32_t wbase = 0, tot = 0;
@@ -1160,15 +1184,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     }
 }
 
-// Inclusive scan over a block of NT threads with an associative operation.
+// Inclusive scan over a block of NT threads with an associative operation
+// of identity id.
 template <int NT, class Op>
-__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s_wave, Op op) {
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t id,
+                                                         Op op) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(v, o);
-        if (lane >= o) v = op(t, v);
-    }
+    v = wave_incl_scan_u32(v, id, op);
     if (lane == 63) s_wave[wave] = v;
     __syncthreads();
     for (int w = 0; w < wave; ++w) v = op(s_wave[w], v);
@@ -1204,17 +1226,16 @@ __global__ __launch_bounds__(kScanThreads) void k_tree_scan(
         return;
     }
     const uint32_t L = blockIdx.x - 1;
-    auto opmin = [](uint32_t a, uint32_t c) { return a < c ? a : c; };
     // suffix minimum: thread r walks the tiles from the end
     const uint32_t rb = threadIdx.x * per;
     uint32_t mn = ~0u;
     for (uint32_t k = 0; k < per; ++k)
         if (rb + k < tiles) mn = min(mn, lvl_first[(uint64_t)(tiles - 1 - (rb + k)) * kTopSlots + L]);
-    uint32_t rrun = block_inclusive_scan<kScanThreads>(mn, s_wave, opmin);
-    rrun = __shfl_up(rrun, 1);
+    uint32_t rrun = block_inclusive_scan<kScanThreads>(mn, s_wave, ~0u, OpMin{});
+    // exclusive: the thread before's value (lane 0: the earlier waves')
+    rrun = dpp_move<kDppWaveShr1, 0xF>(rrun, ~0u);
     {
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) rrun = ~0u;
+        const int wave = threadIdx.x >> 6;
         for (int w = 0; w < wave; ++w) rrun = min(rrun, s_wave[w]);
     }
     for (uint32_t k = 0; k < per; ++k) {
@@ -1258,24 +1279,18 @@ struct Sum4 {
     }
 };
 
-// DPP moves across a wave's lanes (CTRL: the DPP control, ROWS: the rows
-// written); a lane without a source, or in a row not written, takes old.
+// dpp_move of a double, old +0.0.
 template <int CTRL, int ROWS>
 __device__ __forceinline__ double dpp_f64(double v) {
     const uint64_t b = f64_bits(v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xF, false);
-    const uint32_t hi =
-        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xF, false);
-    return bits_f64(((uint64_t)hi << 32) | lo);  // old: +0.0
+    const uint32_t lo = dpp_move<CTRL, ROWS>((uint32_t)b, 0u);
+    const uint32_t hi = dpp_move<CTRL, ROWS>((uint32_t)(b >> 32), 0u);
+    return bits_f64(((uint64_t)hi << 32) | lo);
 }
 template <int CTRL, int ROWS>
 __device__ __forceinline__ Sum4 dpp_sum4(const Sum4& v) {
     return Sum4{dpp_f64<CTRL, ROWS>(v.m), dpp_f64<CTRL, ROWS>(v.x), dpp_f64<CTRL, ROWS>(v.y),
                 dpp_f64<CTRL, ROWS>(v.z)};
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, CTRL, ROWS, 0xF, false);  // old: ~0u
 }
 // Inclusive sum and minimum over a wave's lanes in lane order: steps of 1, 2,
 // 4 and 8 lanes within rows of 16 (row_shr), then rows 1 and 3 take the last
@@ -1284,7 +1299,7 @@ __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
 template <int CTRL, int ROWS>
 __device__ __forceinline__ void scan_step(Sum4& v, uint32_t& m) {
     v.add(dpp_sum4<CTRL, ROWS>(v));
-    m = min(m, dpp_u32<CTRL, ROWS>(m));
+    m = min(m, dpp_move<CTRL, ROWS>(m, ~0u));
 }
 __device__ __forceinline__ void wave_incl_scan(Sum4& v, uint32_t& m) {
     scan_step<0x111, 0xF>(v, m);
@@ -1312,8 +1327,8 @@ __device__ __forceinline__ void block_excl_scan2(Sum4& a, uint32_t& ma, Sum4& b,
         s_m[1][wave] = mb;
     }
     // exclusive within the wave: the lane before's inclusive value (wave_shr:1)
-    Sum4 xa = dpp_sum4<0x138, 0xF>(a), xb = dpp_sum4<0x138, 0xF>(b);
-    uint32_t ya = dpp_u32<0x138, 0xF>(ma), yb = dpp_u32<0x138, 0xF>(mb);
+    Sum4 xa = dpp_sum4<kDppWaveShr1, 0xF>(a), xb = dpp_sum4<kDppWaveShr1, 0xF>(b);
+    uint32_t ya = dpp_move<kDppWaveShr1, 0xF>(ma, ~0u), yb = dpp_move<kDppWaveShr1, 0xF>(mb, ~0u);
     __syncthreads();
     Sum4 ba{0.0, 0.0, 0.0, 0.0}, bb{0.0, 0.0, 0.0, 0.0}, tot{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
