@@ -78,6 +78,7 @@ struct pg_engine {
     uint32_t* sort_tot = nullptr;  // per-digit totals of one pass
     uint16_t* bkt = nullptr;       // sample sort: bucket of each key
     uint64_t* spl = nullptr;       // sample sort: splitters
+    uint64_t* quant = nullptr;     // sample sort: finer quantiles per bucket
     uint32_t* bcount = nullptr;    // sample sort: keys per bucket
     uint32_t* bfill = nullptr;     // sample sort: reserved keys per bucket
     uint32_t* bbase = nullptr;     // sample sort: first key of each bucket
@@ -383,7 +384,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
     const uint64_t o_bk = take(bodies * 2), o_sp = take(kMaxBuckets * 8);
     const uint64_t o_bc = take(kMaxBuckets * 4), o_bf = take(kMaxBuckets * 4);
-    const uint64_t o_bb = take(kMaxBuckets * 4);
+    const uint64_t o_bb = take(kMaxBuckets * 4), o_qt = take((uint64_t)kMaxBuckets * kQuantiles * 8);
     const uint64_t o_nl = take((bodies + 1) * 4), o_tt = take((node_tiles + 2) * 4);
     const uint64_t o_tb = take((node_tiles + 2) * 4), o_ts = take((node_tiles + 2) * 4);
     const uint64_t o_op = take((node_tiles + 2) * kTopSlots * sizeof(Sum4));
@@ -427,6 +428,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->sort_tot = (uint32_t*)(b + o_st);
     e->bkt = (uint16_t*)(b + o_bk);
     e->spl = (uint64_t*)(b + o_sp);
+    e->quant = (uint64_t*)(b + o_qt);
     e->bcount = (uint32_t*)(b + o_bc);
     e->bfill = (uint32_t*)(b + o_bf);
     e->bbase = (uint32_t*)(b + o_bb);
@@ -553,7 +555,8 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         });
         launch(e, cap, "bucket_count", [&] {
             k_bucket_count<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[0], total, e->spl, nb,
-                                                                e->bkt, e->bcount);
+                                                                e->bkt, e->bcount, e->keys[1],
+                                                                e->quant, e->sc);
         });
         launch(e, cap, "bucket_scatter", [&] {
             k_bucket_scatter<<<sort_tiles, kSortThreads, 0, st>>>(
@@ -563,7 +566,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         launch(e, cap, "bucket_sort", [&] {
             k_bucket_sort<DIM><<<nb, kBucketThreads, 0, st>>>(
                 e->keys[1], e->vals[1], e->keys[0], e->vals[0], e->bcount, e->bbase,
-                e->bucket_cap, desc, total, e->k2, bp.nseg, bp.segbits, bp.lmax);
+                e->bucket_cap, desc, total, e->k2, bp.nseg, bp.segbits, bp.lmax, e->quant, e->sc);
         });
     } else {
         for (int p = 0; p < bp.passes; ++p) {

@@ -15,13 +15,14 @@ constexpr int kScanItems = 4;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
 constexpr int kKeyThreads = 1024;  // threads of a key-building block (one sort tile)
-constexpr int kBucketKeys = 1856;  // keys per bucket of the sample sort (on average; steady-state
-                                   // buckets stay within 2048, the four-keys-per-thread sort)
+constexpr int kBucketKeys = 1664;  // keys per bucket of the sample sort (on average; steady-state
+                                   // buckets stay within 2048, the slot sort's reach)
 constexpr int kBucketCap = 4096;   // largest bucket sorted in LDS (larger ones: in HBM)
 constexpr int kBucketThreads = 512;   // threads of a bucket-sorting block (up to 8 keys each)
 constexpr int kMaxBuckets = 1024;  // sample sort up to kMaxBuckets * kBucketKeys keys
 constexpr int kSampleOver = 4;     // splitter samples per bucket (sampling from unsorted keys)
 constexpr int kSplitThreads = 1024;  // threads of the splitter block
+constexpr int kQuantiles = 256;    // per bucket: quantiles of the previous sorted keys
 constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort tiles
 constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
 constexpr int kNodeTile = 1024;    // sorted bodies per node-building block
@@ -99,7 +100,7 @@ struct StepCounters {
     uint64_t sort_tag;      // batch whose sorted keys the sample sort left (0: none)
     uint64_t ext_tag;       // batch whose current positions the extent partials hold (0: none)
     uint32_t split_prev;    // this step's splitters may come from those keys
-    uint32_t pad;
+    uint32_t quantiles;     // they did: finer quantiles per bucket were taken too
 };
 
 }  // namespace pg

@@ -556,15 +556,17 @@ __global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ des
 __global__ __launch_bounds__(kSplitThreads) void k_splitters(
     const uint64_t* __restrict__ prev, const uint64_t* __restrict__ keys, uint64_t n, uint32_t nb,
     uint64_t* __restrict__ spl, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bfill,
-    const StepCounters* __restrict__ sc) {
+    StepCounters* __restrict__ sc) {
     __shared__ uint64_t s[kMaxBuckets * kSampleOver];
     __shared__ uint32_t s_unsorted;
     for (uint32_t b = threadIdx.x; b < nb; b += kSplitThreads) {
         bcount[b] = 0;
         bfill[b] = 0;
     }
+    const bool prev_ok = sc->split_prev != 0;
+    if (threadIdx.x == 0) sc->quantiles = 0;
     if (nb <= 1) return;
-    if (sc->split_prev) {
+    if (prev_ok) {
         // quantiles of the previous step's sorted keys, if they are in order
         if (threadIdx.x == 0) s_unsorted = 0;
         for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads)
@@ -576,6 +578,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters(
         __syncthreads();
         if (!s_unsorted) {
             for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads) spl[b] = s[b];
+            if (threadIdx.x == 0) sc->quantiles = 1;
             return;
         }
         __syncthreads();
@@ -614,14 +617,21 @@ __device__ __forceinline__ uint32_t bucket_of(const uint64_t* s_spl, uint32_t nb
     return lo;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_bucket_count(const uint64_t* __restrict__ keys,
-                                                               uint64_t n,
-                                                               const uint64_t* __restrict__ spl,
-                                                               uint32_t nb,
-                                                               uint16_t* __restrict__ bkt,
-                                                               uint32_t* __restrict__ bcount) {
+// With the previous step's quantiles as splitters, it also copies kQuantiles
+// finer ones per bucket (bucket b's i-th: prev[(b kQuantiles + i) n / (nb
+// kQuantiles)], the first being the bucket's lower splitter) for the slot sort.
+__global__ __launch_bounds__(kSortThreads) void k_bucket_count(
+    const uint64_t* __restrict__ keys, uint64_t n, const uint64_t* __restrict__ spl, uint32_t nb,
+    uint16_t* __restrict__ bkt, uint32_t* __restrict__ bcount, const uint64_t* __restrict__ prev,
+    uint64_t* __restrict__ quant, const StepCounters* __restrict__ sc) {
     __shared__ uint64_t s_spl[kMaxBuckets];
     __shared__ uint32_t s_h[kMaxBuckets];
+    if (sc->quantiles) {
+        const uint64_t m = (uint64_t)nb * kQuantiles;
+        for (uint64_t t = (uint64_t)blockIdx.x * kSortThreads + threadIdx.x; t < m;
+             t += (uint64_t)gridDim.x * kSortThreads)
+            quant[t] = prev[t * n / m];
+    }
     for (uint32_t b = threadIdx.x; b < nb; b += kSortThreads) {
         s_h[b] = 0;
         if (b + 1 < nb) s_spl[b] = spl[b];
@@ -980,6 +990,241 @@ __device__ __forceinline__ void bucket_sort_regs(uint64_t* __restrict__ keys,
     }
 }
 
+// Slot sort of a bucket of at most kSlots keys (buckets hold kBucketKeys on
+// average). Keys are spread over kSlots slots by a monotone map: a coarse bin
+// (between two of the bucket's quantiles of the previous step's keys, which
+// follow the keys' distribution at every scale; without them, the top
+// kCoarseBits bits of the key's offset from the bucket's smallest), bins
+// getting slots in proportion to their key counts (a piecewise-linear fit of
+// the bucket's key distribution), and the key's place within its bin picking
+// the slot within the bin's. Each slot so holds one
+// contiguous key range, the slots in key order, a key or two each. One LDS
+// atomic per key counts its slot and returns its rank there; the scanned
+// counts place every key; each thread then insertion-sorts the keys of its
+// four slots (a few, all in place but within their slot), and a slot of more
+// than kSlotRun keys is sorted by a wave (one key per lane). Equal keys end in
+// no fixed order, as in the bitonic sort. Returns false, with the bucket
+// untouched, when a slot holds more than 64 keys (clustered keys: the bitonic
+// sort takes the bucket).
+constexpr int kSlotBits = 11;
+constexpr uint32_t kSlots = 1u << kSlotBits;
+constexpr uint32_t kSlotRun = 16;
+constexpr int kCoarseBits = 8;
+constexpr uint32_t kCoarse = 1u << kCoarseBits;
+static_assert(kQuantiles == (int)kCoarse, "a coarse bin between consecutive quantiles");
+constexpr uint32_t kWideSlots = 64;  // slots of more than kSlotRun keys sorted by waves
+
+// Sorts one (key, value) per lane across a wave by (key, value), ascending in
+// lane order (a register bitonic network).
+__device__ __forceinline__ void wave_sort64(uint64_t& key, uint32_t& val) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t k1[1] = {key};
+    uint32_t v1[1] = {val};
+#define PG_WAVE_STAGE(K, H) \
+    bitonic_wave<true, 1, H>(k1, v1, ((lane & (H)) == 0) == ((lane & (K)) == 0));
+    PG_WAVE_STAGE(2, 1)
+    PG_WAVE_STAGE(4, 2) PG_WAVE_STAGE(4, 1)
+    PG_WAVE_STAGE(8, 4) PG_WAVE_STAGE(8, 2) PG_WAVE_STAGE(8, 1)
+    PG_WAVE_STAGE(16, 8) PG_WAVE_STAGE(16, 4) PG_WAVE_STAGE(16, 2) PG_WAVE_STAGE(16, 1)
+    PG_WAVE_STAGE(32, 16) PG_WAVE_STAGE(32, 8) PG_WAVE_STAGE(32, 4) PG_WAVE_STAGE(32, 2)
+    PG_WAVE_STAGE(32, 1)
+    PG_WAVE_STAGE(64, 32) PG_WAVE_STAGE(64, 16) PG_WAVE_STAGE(64, 8) PG_WAVE_STAGE(64, 4)
+    PG_WAVE_STAGE(64, 2) PG_WAVE_STAGE(64, 1)
+#undef PG_WAVE_STAGE
+    key = k1[0];
+    val = v1[0];
+}
+
+// The offset `off` of a key within a bin of width `width` (off <= width) as
+// a 16-bit fraction: monotone in off.
+__device__ __forceinline__ uint32_t frac16(uint64_t off, uint64_t width) {
+    const int bl = width ? 64 - __clzll((long long)width) : 0;
+    return bl > 16 ? (uint32_t)(off >> (bl - 16)) : (uint32_t)(off << (16 - bl));
+}
+static_assert(kSlots == 4 * kBucketThreads, "four slots (and up to four keys) per thread");
+
+template <int M>
+__device__ __forceinline__ void minmax_lanes(uint64_t& lo, uint64_t& hi) {
+    const uint64_t a = xor_lanes64<M>(lo), b = xor_lanes64<M>(hi);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+}
+
+__device__ __forceinline__ bool bucket_sort_slots(uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals, uint64_t base,
+                                                  uint32_t cnt, uint8_t* s_raw,
+                                                  const uint64_t* __restrict__ quant) {
+    constexpr int kWaves = kBucketThreads / 64;
+    uint64_t* s_k = reinterpret_cast<uint64_t*>(s_raw);
+    uint32_t* s_v = reinterpret_cast<uint32_t*>(s_raw + kSlots * 8);
+    // two 16-bit fields per word: slot counts, then slot starts
+    uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_raw + kSlots * 12);
+    uint64_t* s_red = reinterpret_cast<uint64_t*>(s_raw + kSlots * 14);
+    uint32_t* s_scan = reinterpret_cast<uint32_t*>(s_raw + kSlots * 14 + 16 * kWaves);
+    // coarse bins: counts, then first slot | slot count << 16
+    uint32_t* s_bin = s_scan + kWaves;
+    uint64_t* s_q = reinterpret_cast<uint64_t*>(s_bin + kCoarse);  // the bucket's quantiles
+    uint32_t* s_wide = reinterpret_cast<uint32_t*>(s_q + kCoarse);   // count, then wide slots
+    const uint32_t t = threadIdx.x;
+    if (t == 0) s_wide[0] = 0;
+    const int lane = (int)(t & 63u), wave = (int)(t >> 6);
+    s_pos[2 * t] = 0;
+    s_pos[2 * t + 1] = 0;
+    if (t < kCoarse) {
+        s_bin[t] = 0;
+        if (quant) s_q[t] = quant[t];
+    }
+    uint64_t key[4];
+    uint32_t val[4];
+    uint64_t lo = ~0ull, hi = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t i = (uint32_t)e * kBucketThreads + t;
+        key[e] = i < cnt ? keys[base + i] : 0ull;
+        val[e] = i < cnt ? vals[base + i] : 0u;
+        if (i < cnt) {
+            lo = key[e] < lo ? key[e] : lo;
+            hi = key[e] > hi ? key[e] : hi;
+        }
+    }
+    minmax_lanes<1>(lo, hi);
+    minmax_lanes<2>(lo, hi);
+    minmax_lanes<4>(lo, hi);
+    minmax_lanes<8>(lo, hi);
+    minmax_lanes<16>(lo, hi);
+    minmax_lanes<32>(lo, hi);
+    if (lane == 0) {
+        s_red[wave] = lo;
+        s_red[kWaves + wave] = hi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint64_t a = s_red[w], b = s_red[kWaves + w];
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    const uint64_t range = hi - lo;
+    const int bits = range ? 64 - __clzll((long long)range) : 0;
+    const int sh = bits > kCoarseBits ? bits - kCoarseBits : 0;
+    uint32_t slot[4], rank[4], frac[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t i = (uint32_t)e * kBucketThreads + t;
+        if (quant) {
+            // the last quantile <= key (the first if none)
+            uint32_t j = 0;
+#pragma unroll
+            for (uint32_t st = kCoarse / 2; st > 0; st >>= 1)
+                if (s_q[j + st] <= key[e]) j += st;
+            const uint64_t a = j ? s_q[j] : (s_q[0] < lo ? s_q[0] : lo);
+            const uint64_t z = j + 1 < kCoarse ? s_q[j + 1] : (s_q[kCoarse - 1] > hi ? s_q[kCoarse - 1] : hi);
+            slot[e] = j;  // the coarse bin, for now
+            frac[e] = frac16(key[e] - a, z - a);
+        } else {
+            slot[e] = (uint32_t)((key[e] - lo) >> sh);
+            frac[e] = frac16((key[e] - lo) - ((uint64_t)slot[e] << sh), (1ull << sh) - 1ull);
+        }
+        if (i < cnt) atomicAdd(&s_bin[slot[e]], 1u);
+    }
+    __syncthreads();
+    {
+        // bin b's slots: [kSlots * (keys before b) / cnt, ... of b + 1)
+        const uint32_t c = t < kCoarse ? s_bin[t] : 0u;
+        uint32_t all;
+        const uint32_t before = block_exclusive_scan<kBucketThreads>(c, s_scan, &all);
+        if (t < kCoarse) {
+            const uint32_t f = (uint32_t)(((uint64_t)before * kSlots) / cnt);
+            const uint32_t l = (uint32_t)(((uint64_t)(before + c) * kSlots) / cnt);
+            s_bin[t] = f | ((l - f) << 16);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t i = (uint32_t)e * kBucketThreads + t;
+        rank[e] = 0;
+        if (i < cnt) {
+            const uint32_t b = s_bin[slot[e]];
+            slot[e] = min((b & 0xFFFFu) + ((frac[e] * (b >> 16)) >> 16), kSlots - 1u);
+            const uint32_t h = (slot[e] & 1u) * 16u;
+            rank[e] = (atomicAdd(&s_pos[slot[e] >> 1], 1u << h) >> h) & 0xFFFFu;
+        }
+    }
+    __syncthreads();
+    // this thread's slots 4t .. 4t + 3
+    const uint32_t w0 = s_pos[2 * t], w1 = s_pos[2 * t + 1];
+    const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16, c2 = w1 & 0xFFFFu, c3 = w1 >> 16;
+    if (__syncthreads_or(max(max(c0, c1), max(c2, c3)) > 64u)) return false;
+    const uint32_t cs[4] = {c0, c1, c2, c3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (cs[k] > kSlotRun) {
+            const uint32_t q = atomicAdd(&s_wide[0], 1u);
+            if (q < kWideSlots) s_wide[1 + q] = 4u * t + (uint32_t)k;
+        }
+    uint32_t all;
+    const uint32_t s0 = block_exclusive_scan<kBucketThreads>(c0 + c1 + c2 + c3, s_scan, &all);
+    const uint32_t s1 = s0 + c0, s2 = s1 + c1, s3 = s2 + c2, s4 = s3 + c3;
+    // (the scan's barriers ordered the wide-slot count)
+    if (s_wide[0] > kWideSlots) return false;
+    s_pos[2 * t] = s0 | (s1 << 16);
+    s_pos[2 * t + 1] = s2 | (s3 << 16);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t i = (uint32_t)e * kBucketThreads + t;
+        if (i < cnt) {
+            const uint32_t p =
+                ((s_pos[slot[e] >> 1] >> ((slot[e] & 1u) * 16u)) & 0xFFFFu) + rank[e];
+            s_k[p] = key[e];
+            s_v[p] = val[e];
+        }
+    }
+    __syncthreads();
+    const uint32_t ss[5] = {s0, s1, s2, s3, s4};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (ss[k + 1] - ss[k] > kSlotRun) continue;  // a wave's
+        for (uint32_t a = ss[k] + 1u; a < ss[k + 1]; ++a) {
+            const uint64_t x = s_k[a];
+            const uint32_t xv = s_v[a];
+            uint32_t b = a;
+            while (b > ss[k] && s_k[b - 1] > x) {
+                s_k[b] = s_k[b - 1];
+                s_v[b] = s_v[b - 1];
+                --b;
+            }
+            s_k[b] = x;
+            s_v[b] = xv;
+        }
+    }
+    const uint32_t nwide = s_wide[0];
+    for (uint32_t q = (uint32_t)wave; q < nwide; q += kWaves) {
+        const uint32_t sl = s_wide[1 + q];
+        const uint32_t a = (s_pos[sl >> 1] >> ((sl & 1u) * 16u)) & 0xFFFFu;
+        const uint32_t z = sl + 1u < kSlots ? (s_pos[(sl + 1u) >> 1] >> (((sl + 1u) & 1u) * 16u)) & 0xFFFFu : cnt;
+        const uint32_t c = z - a;  // kSlotRun < c <= 64
+        uint64_t x = (uint32_t)lane < c ? s_k[a + lane] : ~0ull;
+        uint32_t xv = (uint32_t)lane < c ? s_v[a + lane] : ~0u;
+        wave_sort64(x, xv);
+        if ((uint32_t)lane < c) {
+            s_k[a + lane] = x;
+            s_v[a + lane] = xv;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t i = (uint32_t)e * kBucketThreads + t;
+        if (i < cnt) {
+            keys[base + i] = s_k[i];
+            vals[base + i] = s_v[i];
+        }
+    }
+    return true;
+}
+
 // (4) One block per bucket: sorts keys[base, base + count) (with vals) by
 // key. Up to `cap` keys (at most kBucketCap) by a bitonic sort held in
 // registers; more by stable LSD passes in HBM, ping-ponging with tkeys /
@@ -990,12 +1235,16 @@ __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t* __restrict__ tkeys,
     uint32_t* __restrict__ tvals, const uint32_t* __restrict__ bcount,
     const uint32_t* __restrict__ bbase, uint32_t cap, const SegDesc* __restrict__ desc,
-    uint64_t total, uint64_t* __restrict__ k2, uint32_t nseg, int segbits, int lmax) {
+    uint64_t total, uint64_t* __restrict__ k2, uint32_t nseg, int segbits, int lmax,
+    const uint64_t* __restrict__ quant, const StepCounters* __restrict__ sc) {
     constexpr int kWaves = kBucketThreads / 64;
     static_assert(kBucketCap == 8 * kBucketThreads, "up to eight keys per thread");
     // LDS: the bitonic exchange arrays, or (HBM path) the ranking tables
     constexpr int kRankBytes = (kWaves * 256 + 3 * 256 + kWaves) * 4 + kWaves * 8;
     constexpr int kSortBytes = kBucketCap * 12;
+    static_assert(kSlots * 14 + 16 * kWaves + 4 * kWaves + 12 * kCoarse + 4 * (1 + kWideSlots) <=
+                      kSortBytes,
+                  "slot sort in the sort space");
     __shared__ __align__(16) uint8_t s_raw[kSortBytes > kRankBytes ? kSortBytes : kRankBytes];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t cnt = bcount[blockIdx.x];
@@ -1013,9 +1262,14 @@ __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(
     if (cnt <= min(cap, (uint32_t)kBucketCap)) {
         uint64_t* s_bk = reinterpret_cast<uint64_t*>(s_raw);
         uint32_t* s_bv = reinterpret_cast<uint32_t*>(s_raw + kBucketCap * 8);
-        if (cnt <= 2u * kBucketThreads) bucket_sort_regs<2>(keys, vals, base, cnt, s_bk, s_bv);
-        else if (cnt <= 4u * kBucketThreads) bucket_sort_regs<4>(keys, vals, base, cnt, s_bk, s_bv);
-        else bucket_sort_regs<8>(keys, vals, base, cnt, s_bk, s_bv);
+        if (cnt > kSlots ||
+            !bucket_sort_slots(keys, vals, base, cnt, s_raw,
+                               sc->quantiles ? quant + (uint64_t)blockIdx.x * kQuantiles : nullptr)) {
+            __syncthreads();  // (the slot sort's tables share the space)
+            if (cnt <= 2u * kBucketThreads) bucket_sort_regs<2>(keys, vals, base, cnt, s_bk, s_bv);
+            else if (cnt <= 4u * kBucketThreads) bucket_sort_regs<4>(keys, vals, base, cnt, s_bk, s_bv);
+            else bucket_sort_regs<8>(keys, vals, base, cnt, s_bk, s_bv);
+        }
         __syncthreads();
         for (uint32_t i = tid; i < cnt; i += kBucketThreads)
             order_tie_run<DIM>(desc, total, keys, vals, k2, nseg, segbits, lmax, base + i);

@@ -8,8 +8,8 @@ Bars (stated per test):
     <= 1e-8. The only
     differences are summation order and the rounding order of each cell's
     centre of mass (the reference folds bodies in insertion order, the device
-    sums children bottom-up; the reference's fold over thousands of bodies
-    itself accumulates ~1e-13 relative rounding). A heavy body whose own ancestor cell is accepted
+    sums a cell's bodies in Morton order; the reference's fold over thousands
+    of bodies itself accumulates ~1e-13 relative rounding). A heavy body whose own ancestor cell is accepted
     (the reference's opening rule allows it) sees that cell's centre of mass
     ~1e-10 away, so the pair direction is a cancellation and inherits the COM
     rounding difference; with theta = 0 (no pseudo-particles) the bar is 1e-12
@@ -302,11 +302,28 @@ def test_graph_and_eager_steps_identical(engine, kind):
 
 
 # ------------------------------------------------------------ key sorting
+def clustered_state(n=40000, seed=13):
+    """A cube with tight clusters of 20 to 200 bodies (2e-5 across): a
+    bucket's keys crowd into a sliver of its key range, so slots of the
+    bucket sort overflow (sorted by a wave, or the bucket by the bitonic
+    sort)."""
+    rng = np.random.default_rng(seed)
+    s = small_cube(n, seed=seed)
+    extra = []
+    for size in (20, 30, 45, 60, 100, 200):
+        b = np.repeat(s[[int(rng.integers(n))]], size)
+        for ax in ("x", "y", "z"):
+            b[ax] = b[ax] + rng.uniform(-1e-5, 1e-5, size)
+        extra.append(b)
+    return P.concat([s] + extra)
+
+
 @pytest.mark.parametrize("make", [
     lambda: P.gen_plummer_virial(60000, seed=12, mass=1.0, a=0.1),  # random order: uneven buckets
     lambda: small_cube(100003, seed=3),
     lambda: close_pairs_state(3000, seed=6),                         # equal keys (ties)
-], ids=["plummer60k", "cube100k", "ties"])
+    clustered_state,
+], ids=["plummer60k", "cube100k", "ties", "clusters"])
 def test_sample_sort_equals_radix_sort(engine, make):
     """The sample sort (buckets in LDS, or in HBM past a small bucket cap)
     and the LSD radix sort give the same Morton order, tree and steps."""
