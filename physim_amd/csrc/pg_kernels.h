@@ -232,9 +232,49 @@ __device__ __forceinline__ void wave_count_digit(uint32_t* s_h, uint32_t d) {
         atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
 }
 
+// Bit b of q at bit DIM * b (q < 2^(64 / DIM)).
+template <int DIM>
+__device__ __forceinline__ uint64_t spread_bits(uint64_t q) {
+    if (DIM == 3) {
+        q &= 0x1FFFFFull;
+        q = (q | (q << 32)) & 0x1F00000000FFFFull;
+        q = (q | (q << 16)) & 0x1F0000FF0000FFull;
+        q = (q | (q << 8)) & 0x100F00F00F00F00Full;
+        q = (q | (q << 4)) & 0x10C30C30C30C30C3ull;
+        q = (q | (q << 2)) & 0x1249249249249249ull;
+    } else {
+        q &= 0xFFFFFFFFull;
+        q = (q | (q << 16)) & 0x0000FFFF0000FFFFull;
+        q = (q | (q << 8)) & 0x00FF00FF00FF00FFull;
+        q = (q | (q << 4)) & 0x0F0F0F0F0F0F0F0Full;
+        q = (q | (q << 2)) & 0x3333333333333333ull;
+        q = (q | (q << 1)) & 0x5555555555555555ull;
+    }
+    return q;
+}
+
+// One axis of the descent below without it: with exact arithmetic the axis'
+// digit at level l (p > c) is bit (lmax - 1 - l) of floor(w), w = (p / E + 1)
+// 2^(lmax - 1), every cell boundary of levels < lmax being an integer in w.
+// The reference's centres are rounded, by at most lmax ulp(E) in all (below
+// 2^(lmax - 48) in w), and w here is within 2^(lmax - 51) of its exact
+// value; so a coordinate farther than 2^(lmax - 41) (in w) from every
+// integer has the descent's digits. Returns false (with q = 0) for any other
+// coordinate or a non-finite one: those take the descent.
+__device__ __forceinline__ bool axis_digits(double p, double inv, double scale, double margin,
+                                            uint64_t& q) {
+    const double w = (p * inv + 1.0) * scale;
+    const double f = w - floor(w);
+    const bool ok = f > margin && f < 1.0 - margin;  // (false for NaN)
+    q = ok ? (uint64_t)w : 0ull;
+    return ok;
+}
+
 // Morton key by exact FP64 descent (octree.rs:160-215 / quadtree.rs:161-194):
 // at each level the octant is (x > cx) | (y > cy) << 1 [| (z > cz) << 2] and
-// the child centre is c -/+ extent/2, rounded as the reference rounds it.
+// the child centre is c -/+ extent/2, rounded as the reference rounds it;
+// a body clear of every cell boundary takes the same digits from its scaled
+// coordinates (axis_digits).
 // One block per radix-sort tile of kSortTile bodies (the batch's segments are
 // concatenated), which also writes the tile's digit counts of the first radix
 // pass (digit-major: cnt[digit * tiles + tile]). The extent
@@ -293,17 +333,29 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
             const uint64_t j = g - d.off;
             const double px = d.sx[j], py = d.sy[j], pz = d.sz[j];
             double ext = bits_f64(m);
-            double cx = 0.0, cy = 0.0, cz = 0.0;
-            uint64_t key = 0;
-            for (int l = 0; l < lmax; ++l) {
-                uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
-                if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
-                const double h2 = half(ext);
-                cx = (o & 1u) ? cx + h2 : cx - h2;
-                cy = (o & 2u) ? cy + h2 : cy - h2;
-                if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-                ext = h2;
-                key = (key << DIM) | o;
+            const double inv = 1.0 / ext;
+            const double scale = ldexp(1.0, lmax - 1), margin = ldexp(1.0, lmax - 41);
+            uint64_t qx, qy, qz = 0;
+            // (an extent this far from 1 makes 1/E or E's halvings subnormal)
+            bool fast = ext >= 0x1p-900 && ext <= 0x1p+1000;
+            fast &= axis_digits(px, inv, scale, margin, qx);
+            fast &= axis_digits(py, inv, scale, margin, qy);
+            if (DIM == 3) fast &= axis_digits(pz, inv, scale, margin, qz);
+            uint64_t key = spread_bits<DIM>(qx) | (spread_bits<DIM>(qy) << 1);
+            if (DIM == 3) key |= spread_bits<DIM>(qz) << 2;
+            if (!fast) {
+                double cx = 0.0, cy = 0.0, cz = 0.0;
+                key = 0;
+                for (int l = 0; l < lmax; ++l) {
+                    uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
+                    if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
+                    const double h2 = half(ext);
+                    cx = (o & 1u) ? cx + h2 : cx - h2;
+                    cy = (o & 2u) ? cy + h2 : cy - h2;
+                    if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+                    ext = h2;
+                    key = (key << DIM) | o;
+                }
             }
             if (segbits) key |= (uint64_t)s << (64 - segbits);
             keys[g] = key;

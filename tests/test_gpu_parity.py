@@ -203,6 +203,32 @@ def test_morton_order_and_permutation(engine, kind, dim, levels):
     np.testing.assert_array_equal(perm, order.astype(np.uint32))
 
 
+@pytest.mark.parametrize("kind,dim,levels", [("astro2", 3, 21), ("astro", 2, 32)])
+def test_morton_keys_on_cell_boundaries(engine, kind, dim, levels):
+    """Coordinates on cell boundaries of every level, and one ulp either side
+    (the device keys bodies clear of every boundary from their scaled
+    coordinates, these by the FP64 descent itself): keys equal the descent's."""
+    rng = np.random.default_rng(17)
+    s = small_cube(3000, seed=17)
+    s["x"][0] = s["y"][0] = s["z"][0] = 1.0  # the extent: 1
+    for i in range(1, len(s)):
+        ax = ("x", "y", "z")[i % dim]
+        lev = int(rng.integers(1, levels + 1))
+        frac = float(int(rng.integers(0, 2 ** 20)) % (1 << lev)) / float(1 << lev)
+        p = 2.0 * frac - 1.0
+        p = (p, np.nextafter(p, 2.0), np.nextafter(p, -2.0))[i % 3]
+        s[ax][i] = max(p, -1.0)
+    assert O.extent(s) == 1.0
+    sim = P.Sim(engine, s, kind=kind, **CFG1)
+    sim.accel()
+    keys, perm = sim.morton()
+    ref = np.array([octant_path_key((b["x"], b["y"], b["z"]), 1.0, dim, levels) for b in s],
+                   dtype=np.uint64)
+    order = np.lexsort((np.arange(len(s)), ref))
+    np.testing.assert_array_equal(keys, ref[order])
+    np.testing.assert_array_equal(perm, order.astype(np.uint32))
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
 def test_simple_astro_matches_oracle(engine, precision, tol):
     s = small_cube(3000, seed=4)
