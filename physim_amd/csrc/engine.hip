@@ -537,8 +537,8 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     SegDesc* desc = e->desc[q];
     const uint64_t total = bp.total;
     launch(e, cap, "extent", [&] {
-        k_extent_partial<<<bp.nchunks, kBlock, 0, st>>>(desc, e->ext_seg, e->ext_chunk,
-                                                        e->ext_partial, e->sc, bp.sort_tag);
+        k_extent_partial<<<std::min<uint32_t>(bp.nchunks, kExtentBlocks), kBlock, 0, st>>>(
+            desc, e->ext_seg, e->ext_chunk, e->ext_partial, e->sc, bp.sort_tag, bp.nchunks);
     });
     const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
     launch(e, cap, "keys", [&] {

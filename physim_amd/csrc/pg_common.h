@@ -24,7 +24,8 @@ constexpr int kSampleOver = 4;     // splitter samples per bucket (sampling from
 constexpr int kSplitThreads = 1024;  // threads of the splitter block
 constexpr int kQuantiles = 256;    // per bucket: quantiles of the previous sorted keys
 constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort tiles
-constexpr int kExtentChunk = 4096; // bodies per extent-reduction block
+constexpr int kExtentChunk = 4096; // bodies per extent partial
+constexpr uint32_t kExtentBlocks = 256;  // blocks of the extent pass (each takes chunks in turn)
 constexpr int kNodeTile = 1024;    // sorted bodies per node-building block
 constexpr int kNodeThreads = 512;  // threads per node-building block (two bodies each)
 constexpr int kNodeLds = 1792;     // nodes per tile built one per thread (larger: body by body)

@@ -197,23 +197,26 @@ __global__ __launch_bounds__(kBlock) void k_extent_partial(const SegDesc* __rest
                                                            const uint32_t* __restrict__ chunk_idx,
                                                            uint64_t* __restrict__ partial,
                                                            const StepCounters* __restrict__ sc,
-                                                           uint64_t tag) {
+                                                           uint64_t tag, uint32_t nchunks) {
     __shared__ uint64_t s_wave[kBlock / 64];
     if (tag != 0 && sc->ext_tag == tag) return;
-    const SegDesc& d = desc[chunk_seg[blockIdx.x]];
-    const uint64_t j0 = (uint64_t)chunk_idx[blockIdx.x] * kExtentChunk;
-    const uint64_t j1 = min(j0 + kExtentChunk, d.n);
-    uint64_t m = 0;
-    bool bad = false;
-    for (uint64_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
-        const double x = d.sx[j], y = d.sy[j], z = d.sz[j];
-        bad |= !isfinite(x) || !isfinite(y) || !isfinite(z);
-        const uint64_t b = f64_bits(fmax(fmax(fabs(x), fabs(y)), fabs(z)));
-        m = b > m ? b : m;
+    // (a small grid: the skip above is the common case)
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const SegDesc& d = desc[chunk_seg[c]];
+        const uint64_t j0 = (uint64_t)chunk_idx[c] * kExtentChunk;
+        const uint64_t j1 = min(j0 + kExtentChunk, d.n);
+        uint64_t m = 0;
+        bool bad = false;
+        for (uint64_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
+            const double x = d.sx[j], y = d.sy[j], z = d.sz[j];
+            bad |= !isfinite(x) || !isfinite(y) || !isfinite(z);
+            const uint64_t b = f64_bits(fmax(fmax(fabs(x), fabs(y)), fabs(z)));
+            m = b > m ? b : m;
+        }
+        if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(&d.st->err, kErrNonFinite);
+        m = block_max_u64(m, s_wave);
+        if (threadIdx.x == 0) partial[c] = m;
     }
-    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(&d.st->err, kErrNonFinite);
-    m = block_max_u64(m, s_wave);
-    if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
 // Adds one digit per lane (256 = none) to the block's LDS digit counters:
@@ -678,17 +681,7 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_count(
     uint64_t* __restrict__ quant, const StepCounters* __restrict__ sc) {
     __shared__ uint64_t s_spl[kMaxBuckets];
     __shared__ uint32_t s_h[kMaxBuckets];
-    if (sc->quantiles) {
-        const uint64_t m = (uint64_t)nb * kQuantiles;
-        for (uint64_t t = (uint64_t)blockIdx.x * kSortThreads + threadIdx.x; t < m;
-             t += (uint64_t)gridDim.x * kSortThreads)
-            quant[t] = prev[t * n / m];
-    }
-    for (uint32_t b = threadIdx.x; b < nb; b += kSortThreads) {
-        s_h[b] = 0;
-        if (b + 1 < nb) s_spl[b] = spl[b];
-    }
-    __syncthreads();
+    // the independent loads first: the tile's keys, the splitters, the flag
     const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
     uint64_t key[kSortItems];
 #pragma unroll
@@ -696,6 +689,18 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_count(
         const uint64_t idx = base + (uint64_t)k * kSortThreads + threadIdx.x;
         key[k] = idx < n ? keys[idx] : 0ull;
     }
+    const bool copy_quant = sc->quantiles != 0;
+    for (uint32_t b = threadIdx.x; b < nb; b += kSortThreads) {
+        s_h[b] = 0;
+        if (b + 1 < nb) s_spl[b] = spl[b];
+    }
+    if (copy_quant) {
+        const uint64_t m = (uint64_t)nb * kQuantiles;
+        for (uint64_t t = (uint64_t)blockIdx.x * kSortThreads + threadIdx.x; t < m;
+             t += (uint64_t)gridDim.x * kSortThreads)
+            quant[t] = prev[t * n / m];
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint64_t idx = base + (uint64_t)k * kSortThreads + threadIdx.x;
@@ -725,6 +730,9 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint16_t s_b[kSortTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the bucket sizes of this thread's two buckets, loaded early
+    const uint32_t b0 = 2u * (uint32_t)tid, b1 = b0 + 1u;
+    const uint32_t n0 = b0 < nb ? bcount[b0] : 0u, n1 = b1 < nb ? bcount[b1] : 0u;
     for (int i = tid; i < kWaves * kMaxBuckets; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
@@ -738,18 +746,23 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
         val[k] = valid ? vin[idx] : 0u;
         bk[k] = valid ? (uint32_t)bkt[idx] : kMaxBuckets;
     }
-    // rank within the wave among equal buckets (match-any over 10 bits)
+    // rank within the wave among equal buckets (match-any over 10 bits; the
+    // keys arrive nearly sorted, so a wave's keys mostly share one bucket)
     const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t b = bk[k];
         const bool valid = b < (uint32_t)kMaxBuckets;
         uint64_t peers = __ballot(valid);
+        const uint32_t bf = (uint32_t)__builtin_amdgcn_readlane(
+            (int)b, peers ? __ffsll((long long)peers) - 1 : 0);
+        if (__ballot(valid && b != bf)) {
 #pragma unroll
-        for (int bit = 0; bit < 10; ++bit) {
-            const bool on = (b >> bit) & 1u;
-            const uint64_t bb = __ballot(on);
-            peers &= on ? bb : ~bb;
+            for (int bit = 0; bit < 10; ++bit) {
+                const bool on = (b >> bit) & 1u;
+                const uint64_t bb = __ballot(on);
+                peers &= on ? bb : ~bb;
+            }
         }
         if (!valid) peers = 1ull << lane;
         const int leader = __ffsll((long long)peers) - 1;
@@ -764,7 +777,6 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
     __syncthreads();
     // buckets 2t and 2t + 1: counts over the waves, tile-local starts,
     // global bucket bases, and the reserved output ranges
-    const uint32_t b0 = 2u * (uint32_t)tid, b1 = b0 + 1u;
     uint32_t c0 = 0, c1 = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
@@ -774,7 +786,6 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
         c0 += x0;
         c1 += x1;
     }
-    const uint32_t n0 = b0 < nb ? bcount[b0] : 0u, n1 = b1 < nb ? bcount[b1] : 0u;
     uint32_t dummy;
     const uint32_t l0 = block_exclusive_scan<kSortThreads>(c0 + c1, s_scan, &dummy);
     const uint32_t g0 = block_exclusive_scan<kSortThreads>(n0 + n1, s_scan, &dummy);
@@ -1749,14 +1760,15 @@ constexpr int kPathLevels = 12;
 constexpr uint32_t kTinyNode = 8;
 constexpr uint32_t kSmallNode = 64;
 // The queue: one thread per node, a tile has at most kNodeLds nodes of which
-// kNodeTile are leaves; an entry is the local node (11 bits) and the range's
-// last body (10 bits). One thread per body: internal nodes of more than
+// kNodeTile are leaves; an entry is the owning body (10 bits), the range's
+// last body (10 bits) and the node's place among the owner's (6 bits). One
+// thread per body: internal nodes of more than
 // kSmallNode bodies hold disjoint body sets at one level, so a tile has at
 // most kNodeTile / (kSmallNode + 1) per level; an entry (16-bit, two per
 // word) is the owning body (10 bits) and the level (6 bits: an internal node
 // lies above a leaf, so below the deepest level, 64).
 constexpr int kNodeQueue = kNodeLds - kNodeTile;
-static_assert(kNodeLds <= 2048 && kNodeTile <= 1024, "queue entries: 11-bit node, 10-bit body");
+static_assert(kNodeTile <= 1024 && kTopSlots <= 65, "queue entries: 10-bit bodies, 6-bit places");
 static_assert(kTopSlots * (kNodeTile / (kSmallNode + 1)) <= 2 * kNodeQueue && kTopSlots <= 65,
               "queue bound of the body-parallel build");
 
@@ -1794,12 +1806,13 @@ __device__ __forceinline__ uint4 pack_tail(double mass, uint32_t skip, uint32_t 
 // scanned per-level table), so no search leaves the tile.
 //
 // Bodies (keys, levels, positions, masses) are staged in LDS. A tile of at
-// most kNodeLds nodes is built one thread per node (a node -> owner table);
-// a larger one (clustered bodies), body by body. Each record is written by
+// most kNodeLds nodes is built in two passes, its leaves one thread per body
+// and then its internal nodes one thread per node; a larger one (clustered
+// bodies), body by body. Each record is written by
 // the thread that finds its range; an internal node whose bodies all lie in
 // the tile takes its mass and centre of mass from the sums of those bodies in
-// body order (by its thread up to kSmallNode bodies, else by a wave after the
-// node pass).
+// body order (by its thread up to kTinyNode bodies, or kSmallNode body by
+// body; else by a group of lanes after the node passes).
 //
 // An internal node whose range leaves the tile is left to k_com_span: it is
 // recorded (generation-tagged) at slot tile * kTopSlots + level, and
@@ -1820,11 +1833,11 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     __shared__ uint32_t s_first[kNodeTile + 1];
     __shared__ uint8_t s_lo[kNodeTile];
     __shared__ double s_bm[kNodeTile], s_bx[kNodeTile], s_by[kNodeTile], s_bz[kNodeTile];
-    __shared__ uint16_t s_own[kNodeLds];
+    __shared__ uint16_t s_ilist[kNodeQueue];  // a node-parallel tile's internal nodes
     __shared__ uint32_t s_q[kNodeQueue];  // queued internal nodes (see above)
     __shared__ double s_path[kPathLevels][4];  // half-width and centre along the common path
     __shared__ uint64_t s_next;  // key just after the tile
-    __shared__ uint32_t s_topl, s_nq;
+    __shared__ uint32_t s_topl, s_nq, s_ni;
     __shared__ int s_lt;  // levels of the path shared by all the tile's bodies (capped)
     __shared__ Sum4 s_w4[2][kNodeThreads / 64];
     __shared__ uint32_t s_wmin[2][kNodeThreads / 64];
@@ -1845,6 +1858,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         s_next = b1 < total ? skeys_g[b1] : 0ull;  // read only when the segment continues past b1
         s_topl = 0;
         s_nq = 0;
+        s_ni = 0;
         // every node of the tile lies on or below this path: its cells are
         // the ancestors of every body
         const uint64_t ka = nb ? skeys_g[b0] : 0ull, kz = nb ? skeys_g[b1 - 1] : 0ull;
@@ -1860,7 +1874,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         }
         s_lt = lt;
     }
-    // Bodies: staging, the node -> owner table, checks.
+    // Bodies: staging, checks.
     for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
         const uint64_t i = b0 + li;
         const uint64_t key = skeys_g[i];
@@ -1871,8 +1885,6 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         s_lo[li] = (uint8_t)lo;
         s_first[li] = f;
         first_node[i] = f;
-        if (node_par)
-            for (uint32_t nd = f; nd < fn && nd < n1; ++nd) s_own[nd - n0] = (uint16_t)li;
         const uint64_t j = i - d.off;
         const double x = d.dx[j], y = d.dy[j], z = d.dz[j], m = d.dm[j];
         s_bm[li] = m;
@@ -1952,7 +1964,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
                     // the tile short, an error; then the thread sums)
                     const uint32_t q = atomicAdd(&s_nq, 1u);
                     if (node_par && q < (uint32_t)kNodeQueue) {
-                        s_q[q] = (nd - n0) | ((e - 1u) << 11);
+                        s_q[q] = li | ((e - 1u) << 10) | ((nd - s_first[li]) << 20);
                         queued = true;
                     } else if (!node_par && q < 2u * kNodeQueue) {
                         reinterpret_cast<uint16_t*>(s_q)[q] = (uint16_t)(li | ((uint32_t)L << 10));
@@ -1977,21 +1989,38 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         }
     };
 
+    // node nd (level L) of local body li: its centre by the reference's
+    // descent (from the tile's common path), then its record
+    auto build = [&](uint32_t nd, uint32_t li, int L, bool leaf) {
+        const uint64_t key = s_keys[li];
+        const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
+        const int f = min(L, lt);
+        double ext = s_path[f][0], cx = s_path[f][1], cy = s_path[f][2], cz = s_path[f][3];
+        descend<DIM>(key, kx, lmax, f, L, ext, cx, cy, cz);
+        const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
+        emit(nd, li, L, leaf, cx, cy, cz, oct);
+    };
     if (node_par) {
-        // one thread per node: the centre by the reference's descent from
-        // the segment's root
-        for (uint32_t k = threadIdx.x; k < nloc; k += kNodeThreads) {
-            const uint32_t li = s_own[k];
-            const uint32_t nbase = s_first[li];
-            const int L = (int)s_lo[li] + (int)(n0 + k - nbase);
-            const bool leaf = n0 + k + 1u == s_first[li + 1];
-            const uint64_t key = s_keys[li];
-            const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
-            const int f = min(L, lt);
-            double ext = s_path[f][0], cx = s_path[f][1], cy = s_path[f][2], cz = s_path[f][3];
-            descend<DIM>(key, kx, lmax, f, L, ext, cx, cy, cz);
-            const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
-            emit(n0 + k, li, L, leaf, cx, cy, cz, oct);
+        // leaves, one thread per body, which also lists the body's internal
+        // nodes; then the internal nodes, one thread each (a node-parallel
+        // tile has at most kNodeQueue: the leaves are kNodeTile of its
+        // kNodeLds nodes, unless the capacity cut it short, an error)
+        for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
+            const uint32_t f = s_first[li], fn = s_first[li + 1];
+            if (fn - 1u < n1) build(fn - 1u, li, (int)s_lo[li] + (int)(fn - 1u - f), true);
+            const uint32_t ni = (min(fn - 1u, n1) > f) ? min(fn - 1u, n1) - f : 0u;
+            if (ni) {
+                const uint32_t p = atomicAdd(&s_ni, ni);
+                for (uint32_t j = 0; j < ni && p + j < (uint32_t)kNodeQueue; ++j)
+                    s_ilist[p + j] = (uint16_t)(li | (j << 10));
+            }
+        }
+        __syncthreads();
+        const uint32_t ni = min(s_ni, (uint32_t)kNodeQueue);
+        for (uint32_t q = threadIdx.x; q < ni; q += kNodeThreads) {
+            const uint32_t ent = s_ilist[q];
+            const uint32_t li = ent & 1023u, j = ent >> 10;
+            build(s_first[li] + j, li, (int)s_lo[li] + (int)j, false);
         }
     } else {
         // one thread per body: one descent through the body's nodes
@@ -2098,10 +2127,9 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         uint32_t li, e, nd;
         if (node_par) {
             const uint32_t ent = s_q[q];
-            const uint32_t k = ent & 2047u;
-            li = s_own[k];
-            e = (ent >> 11) + 1u;
-            nd = n0 + k;
+            li = ent & 1023u;
+            e = ((ent >> 10) & 1023u) + 1u;
+            nd = s_first[li] + (ent >> 20);
         } else {
             const uint32_t ent = reinterpret_cast<const uint16_t*>(s_q)[q];
             li = ent & 1023u;
