@@ -83,6 +83,7 @@ struct pg_engine {
     uint32_t* bfill = nullptr;     // sample sort: reserved keys per bucket
     uint32_t* bbase = nullptr;     // sample sort: first key of each bucket
     uint32_t max_buckets = kMaxBuckets;  // larger batches use the radix sort
+    uint64_t ext_valid = 0;  // batch whose extent partials the last step left (0: none)
     uint32_t bucket_cap = kBucketCap;    // larger buckets are sorted in HBM
     uint32_t* node_local = nullptr;  // per-body node offset within its tile
     uint32_t* tile_tot = nullptr;    // nodes per tile
@@ -417,6 +418,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
                             (unsigned long long)total, hipGetErrorString(he));
     char* b = (char*)p;
     e->scratch = p;
+    e->ext_valid = 0;
     e->scratch_bytes = total;
     e->keys[0] = (uint64_t*)(b + o_k0);
     e->keys[1] = (uint64_t*)(b + o_k1);
@@ -531,15 +533,18 @@ static inline unsigned grid_for(uint64_t n, unsigned cap) {
     return (unsigned)std::min<uint64_t>(g, cap);
 }
 
+// `extent`: the extent partials may be stale (the last step did not leave
+// them), so the extent pass runs.
 template <int DIM, bool FP64, bool INTEG>
-static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap) {
+static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap, bool extent) {
     hipStream_t st = e->stream;
     SegDesc* desc = e->desc[q];
     const uint64_t total = bp.total;
-    launch(e, cap, "extent", [&] {
-        k_extent_partial<<<std::min<uint32_t>(bp.nchunks, kExtentBlocks), kBlock, 0, st>>>(
-            desc, e->ext_seg, e->ext_chunk, e->ext_partial, e->sc, bp.sort_tag, bp.nchunks);
-    });
+    if (extent)
+        launch(e, cap, "extent", [&] {
+            k_extent_partial<<<std::min<uint32_t>(bp.nchunks, kExtentBlocks), kBlock, 0, st>>>(
+                desc, e->ext_seg, e->ext_chunk, e->ext_partial, e->sc, bp.sort_tag, bp.nchunks);
+        });
     const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
     launch(e, cap, "keys", [&] {
         k_keys<DIM><<<sort_tiles, kKeyThreads, 0, st>>>(
@@ -636,7 +641,7 @@ static void enqueue_direct_step(pg_engine* e, const BatchPlan& bp, int q, bool c
     });
 }
 
-static void enqueue_step(pg_engine* e, const BatchPlan& bp, int q, bool cap) {
+static void enqueue_step(pg_engine* e, const BatchPlan& bp, int q, bool cap, bool extent) {
     const bool f64 = bp.precision == PG_PREC_FP64;
     if (bp.kind == PG_KIND_DIRECT) {
         if (bp.integrate) {
@@ -650,19 +655,19 @@ static void enqueue_step(pg_engine* e, const BatchPlan& bp, int q, bool cap) {
     }
     if (bp.dim == 3) {
         if (bp.integrate) {
-            if (f64) enqueue_tree_step<3, true, true>(e, bp, q, cap);
-            else enqueue_tree_step<3, false, true>(e, bp, q, cap);
+            if (f64) enqueue_tree_step<3, true, true>(e, bp, q, cap, extent);
+            else enqueue_tree_step<3, false, true>(e, bp, q, cap, extent);
         } else {
-            if (f64) enqueue_tree_step<3, true, false>(e, bp, q, cap);
-            else enqueue_tree_step<3, false, false>(e, bp, q, cap);
+            if (f64) enqueue_tree_step<3, true, false>(e, bp, q, cap, extent);
+            else enqueue_tree_step<3, false, false>(e, bp, q, cap, extent);
         }
     } else {
         if (bp.integrate) {
-            if (f64) enqueue_tree_step<2, true, true>(e, bp, q, cap);
-            else enqueue_tree_step<2, false, true>(e, bp, q, cap);
+            if (f64) enqueue_tree_step<2, true, true>(e, bp, q, cap, extent);
+            else enqueue_tree_step<2, false, true>(e, bp, q, cap, extent);
         } else {
-            if (f64) enqueue_tree_step<2, true, false>(e, bp, q, cap);
-            else enqueue_tree_step<2, false, false>(e, bp, q, cap);
+            if (f64) enqueue_tree_step<2, true, false>(e, bp, q, cap, extent);
+            else enqueue_tree_step<2, false, false>(e, bp, q, cap, extent);
         }
     }
 }
@@ -847,7 +852,19 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     }
     const bool flips = bp.kind != PG_KIND_DIRECT || integrate;
     uint32_t done = 0;
-    if (integrate && e->graphs && !e->profiling && steps >= 2) {
+    // An integrating tree step's walk leaves the next step's extent partials:
+    // the extent pass runs only where the last step did not (after uploads,
+    // other batches, force-only or direct steps). Graphs hold steps that
+    // follow such a step; a call that starts without one runs its first two
+    // steps eagerly (keeping the graph's parities).
+    bool ext_ok = bp.kind != PG_KIND_DIRECT && e->ext_valid != 0 && e->ext_valid == bp.sort_tag;
+    if (integrate && e->graphs && !e->profiling && steps >= 2 && !ext_ok && steps >= 4) {
+        enqueue_step(e, bp, 0, false, true);
+        enqueue_step(e, bp, 1, false, false);
+        done = 2;
+        ext_ok = true;
+    }
+    if (integrate && e->graphs && !e->profiling && steps >= 2 && ext_ok) {
         // signature: sims, parities, sizes, flags and the scratch it was captured on
         std::vector<uint64_t> sig;
         sig.push_back((uint64_t)(uintptr_t)e->scratch);
@@ -862,8 +879,8 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         if (!exec) {
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(e, bp, 0, true);
-            enqueue_step(e, bp, 1, true);
+            enqueue_step(e, bp, 0, true, false);
+            enqueue_step(e, bp, 1, true, false);
             HIP_TRY(hipStreamEndCapture(e->stream, &graph));
             HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             (void)hipGraphDestroy(graph);
@@ -876,9 +893,11 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         for (; done + 2 <= steps; done += 2) HIP_TRY(hipGraphLaunch(exec, e->stream));
     }
     for (; done < steps; ++done) {
-        enqueue_step(e, bp, (int)(done & 1), false);
+        enqueue_step(e, bp, (int)(done & 1), false, !ext_ok);
+        ext_ok = true;
         if (!integrate) break;
     }
+    e->ext_valid = integrate && bp.kind != PG_KIND_DIRECT ? bp.sort_tag : 0;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->profiling) ev_collect(e);
@@ -1024,6 +1043,7 @@ extern "C" int pg_sim_upload(pg_sim* s, const Entity* state, uint64_t n) {
     HIP_TRY(hipMemsetAsync(s->st, 0, sizeof(SegState), e->stream));
     // new positions: no extent partials are current
     if (e->sc) HIP_TRY(hipMemsetAsync(&e->sc->ext_tag, 0, sizeof(uint64_t), e->stream));
+    e->ext_valid = 0;
     HIP_TRY(hipStreamSynchronize(e->stream));
     s->uploaded = true;
     if (e->last_sim == s) e->last_sim = nullptr;

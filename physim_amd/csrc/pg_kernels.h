@@ -2267,8 +2267,10 @@ __device__ __forceinline__ void pair_fp64(double ax, double ay, double az, doubl
 // through its skip index; the wave descends into a node iff some lane opened
 // it. Then the Stormer update (verlet.rs:24-82) in place on the permuted
 // state, or the acceleration into acc_out (upload order) without it.
+// (eight waves per SIMD: the walk waits on its chain of node loads, and more
+// waves hide more of it; the spilled scalar registers cost less)
 template <int DIM, bool FP64, bool INTEGRATE>
-__global__ __launch_bounds__(kBlock) void k_walk(const SegDesc* __restrict__ desc, uint64_t total,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_walk(const SegDesc* __restrict__ desc, uint64_t total,
                                                  const uint64_t* __restrict__ skeys, int segbits,
                                                  const uint32_t* __restrict__ first_node,
                                                  const NodeHot* __restrict__ hot, uint64_t cap,
