@@ -84,6 +84,7 @@ struct pg_engine {
     uint32_t* bbase = nullptr;     // sample sort: first key of each bucket
     uint32_t max_buckets = kMaxBuckets;  // larger batches use the radix sort
     uint64_t ext_valid = 0;  // batch whose extent partials the last step left (0: none)
+    uint64_t sorted_valid = 0;  // batch whose sample-sorted keys keys[1] holds (0: none)
     uint32_t bucket_cap = kBucketCap;    // larger buckets are sorted in HBM
     uint32_t* node_local = nullptr;  // per-body node offset within its tile
     uint32_t* tile_tot = nullptr;    // nodes per tile
@@ -275,6 +276,7 @@ extern "C" int pg_engine_set_sort(pg_engine* e, uint32_t max_buckets, uint32_t b
     for (auto& g : e->graph_cache)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     e->graph_cache.clear();
+    e->sorted_valid = 0;  // the bucket layout may change
     e->max_buckets = max_buckets;
     e->bucket_cap = bucket_cap;
     return PG_OK;
@@ -419,6 +421,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     char* b = (char*)p;
     e->scratch = p;
     e->ext_valid = 0;
+    e->sorted_valid = 0;
     e->scratch_bytes = total;
     e->keys[0] = (uint64_t*)(b + o_k0);
     e->keys[1] = (uint64_t*)(b + o_k1);
@@ -534,9 +537,12 @@ static inline unsigned grid_for(uint64_t n, unsigned cap) {
 }
 
 // `extent`: the extent partials may be stale (the last step did not leave
-// them), so the extent pass runs.
+// them), so the extent pass runs. `fused`: keys[1] holds this batch's
+// previous sample-sorted keys, so k_keys takes the splitters from them and
+// counts the buckets itself (no k_splitters, no k_bucket_count).
 template <int DIM, bool FP64, bool INTEG>
-static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap, bool extent) {
+static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap, bool extent,
+                              bool fused) {
     hipStream_t st = e->stream;
     SegDesc* desc = e->desc[q];
     const uint64_t total = bp.total;
@@ -550,19 +556,22 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         k_keys<DIM><<<sort_tiles, kKeyThreads, 0, st>>>(
             desc, e->seg_off, bp.nseg, total, bp.segbits, bp.lmax, e->ext_partial, e->sc,
             INTEG ? 1 : 0, bp.buckets ? bp.sort_tag : 0ull, e->keys[0], e->vals[0], e->sort_cnt,
-            sort_tiles);
+            sort_tiles, fused && bp.buckets ? 1 : 0, e->keys[1], bp.buckets, e->bkt, e->bcount,
+            e->quant);
     });
     if (bp.buckets) {
         const uint32_t nb = bp.buckets;
-        launch(e, cap, "splitters", [&] {
-            k_splitters<<<1, kSplitThreads, 0, st>>>(e->keys[1], e->keys[0], total, nb, e->spl,
-                                                     e->bcount, e->bfill, e->sc);
-        });
-        launch(e, cap, "bucket_count", [&] {
-            k_bucket_count<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[0], total, e->spl, nb,
-                                                                e->bkt, e->bcount, e->keys[1],
-                                                                e->quant, e->sc);
-        });
+        if (!fused) {
+            launch(e, cap, "splitters", [&] {
+                k_splitters<<<1, kSplitThreads, 0, st>>>(e->keys[1], e->keys[0], total, nb, e->spl,
+                                                         e->bcount, e->bfill, e->sc);
+            });
+            launch(e, cap, "bucket_count", [&] {
+                k_bucket_count<<<sort_tiles, kSortThreads, 0, st>>>(e->keys[0], total, e->spl, nb,
+                                                                    e->bkt, e->bcount, e->keys[1],
+                                                                    e->quant, e->sc);
+            });
+        }
         launch(e, cap, "bucket_scatter", [&] {
             k_bucket_scatter<<<sort_tiles, kSortThreads, 0, st>>>(
                 e->keys[0], e->vals[0], e->bkt, total, e->bcount, e->bfill, e->bbase, nb,
@@ -570,7 +579,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         });
         launch(e, cap, "bucket_sort", [&] {
             k_bucket_sort<DIM><<<nb, kBucketThreads, 0, st>>>(
-                e->keys[1], e->vals[1], e->keys[0], e->vals[0], e->bcount, e->bbase,
+                e->keys[1], e->vals[1], e->keys[0], e->vals[0], e->bcount, e->bfill, e->bbase,
                 e->bucket_cap, desc, total, e->k2, bp.nseg, bp.segbits, bp.lmax, e->quant, e->sc);
         });
     } else {
@@ -641,7 +650,8 @@ static void enqueue_direct_step(pg_engine* e, const BatchPlan& bp, int q, bool c
     });
 }
 
-static void enqueue_step(pg_engine* e, const BatchPlan& bp, int q, bool cap, bool extent) {
+static void enqueue_step(pg_engine* e, const BatchPlan& bp, int q, bool cap, bool extent,
+                         bool fused) {
     const bool f64 = bp.precision == PG_PREC_FP64;
     if (bp.kind == PG_KIND_DIRECT) {
         if (bp.integrate) {
@@ -655,19 +665,19 @@ static void enqueue_step(pg_engine* e, const BatchPlan& bp, int q, bool cap, boo
     }
     if (bp.dim == 3) {
         if (bp.integrate) {
-            if (f64) enqueue_tree_step<3, true, true>(e, bp, q, cap, extent);
-            else enqueue_tree_step<3, false, true>(e, bp, q, cap, extent);
+            if (f64) enqueue_tree_step<3, true, true>(e, bp, q, cap, extent, fused);
+            else enqueue_tree_step<3, false, true>(e, bp, q, cap, extent, fused);
         } else {
-            if (f64) enqueue_tree_step<3, true, false>(e, bp, q, cap, extent);
-            else enqueue_tree_step<3, false, false>(e, bp, q, cap, extent);
+            if (f64) enqueue_tree_step<3, true, false>(e, bp, q, cap, extent, fused);
+            else enqueue_tree_step<3, false, false>(e, bp, q, cap, extent, fused);
         }
     } else {
         if (bp.integrate) {
-            if (f64) enqueue_tree_step<2, true, true>(e, bp, q, cap, extent);
-            else enqueue_tree_step<2, false, true>(e, bp, q, cap, extent);
+            if (f64) enqueue_tree_step<2, true, true>(e, bp, q, cap, extent, fused);
+            else enqueue_tree_step<2, false, true>(e, bp, q, cap, extent, fused);
         } else {
-            if (f64) enqueue_tree_step<2, true, false>(e, bp, q, cap, extent);
-            else enqueue_tree_step<2, false, false>(e, bp, q, cap, extent);
+            if (f64) enqueue_tree_step<2, true, false>(e, bp, q, cap, extent, fused);
+            else enqueue_tree_step<2, false, false>(e, bp, q, cap, extent, fused);
         }
     }
 }
@@ -857,14 +867,21 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     // other batches, force-only or direct steps). Graphs hold steps that
     // follow such a step; a call that starts without one runs its first two
     // steps eagerly (keeping the graph's parities).
-    bool ext_ok = bp.kind != PG_KIND_DIRECT && e->ext_valid != 0 && e->ext_valid == bp.sort_tag;
-    if (integrate && e->graphs && !e->profiling && steps >= 2 && !ext_ok && steps >= 4) {
-        enqueue_step(e, bp, 0, false, true);
-        enqueue_step(e, bp, 1, false, false);
+    // Likewise the sample sort takes its splitters from the batch's previous
+    // sorted keys where the last step left them (fused into k_keys).
+    const bool tree = bp.kind != PG_KIND_DIRECT;
+    bool ext_ok = tree && e->ext_valid != 0 && e->ext_valid == bp.sort_tag;
+    bool srt_ok = tree && bp.buckets && e->sorted_valid != 0 && e->sorted_valid == bp.sort_tag;
+    const bool warm_needs_sort = tree && bp.buckets;
+    auto warm = [&] { return !tree || (ext_ok && (srt_ok || !warm_needs_sort)); };
+    if (integrate && e->graphs && !e->profiling && steps >= 4 && !warm()) {
+        enqueue_step(e, bp, 0, false, !ext_ok, srt_ok);
+        enqueue_step(e, bp, 1, false, false, warm_needs_sort);
         done = 2;
         ext_ok = true;
+        srt_ok = warm_needs_sort;
     }
-    if (integrate && e->graphs && !e->profiling && steps >= 2 && ext_ok) {
+    if (integrate && e->graphs && !e->profiling && steps >= 2 && warm()) {
         // signature: sims, parities, sizes, flags and the scratch it was captured on
         std::vector<uint64_t> sig;
         sig.push_back((uint64_t)(uintptr_t)e->scratch);
@@ -879,8 +896,8 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         if (!exec) {
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(e, bp, 0, true, false);
-            enqueue_step(e, bp, 1, true, false);
+            enqueue_step(e, bp, 0, true, false, warm_needs_sort);
+            enqueue_step(e, bp, 1, true, false, warm_needs_sort);
             HIP_TRY(hipStreamEndCapture(e->stream, &graph));
             HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             (void)hipGraphDestroy(graph);
@@ -893,11 +910,13 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         for (; done + 2 <= steps; done += 2) HIP_TRY(hipGraphLaunch(exec, e->stream));
     }
     for (; done < steps; ++done) {
-        enqueue_step(e, bp, (int)(done & 1), false, !ext_ok);
+        enqueue_step(e, bp, (int)(done & 1), false, !ext_ok, srt_ok);
         ext_ok = true;
+        srt_ok = warm_needs_sort;
         if (!integrate) break;
     }
-    e->ext_valid = integrate && bp.kind != PG_KIND_DIRECT ? bp.sort_tag : 0;
+    e->ext_valid = integrate && tree ? bp.sort_tag : 0;
+    e->sorted_valid = warm_needs_sort ? bp.sort_tag : 0;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->profiling) ev_collect(e);

@@ -273,11 +273,28 @@ __device__ __forceinline__ bool axis_digits(double p, double inv, double scale, 
     return ok;
 }
 
+// Bucket of a key: the number of splitters (sorted, nb - 1 of them) <= key.
+__device__ __forceinline__ uint32_t bucket_of(const uint64_t* s_spl, uint32_t nb, uint64_t key) {
+    uint32_t lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_spl[mid] <= key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 // Morton key by exact FP64 descent (octree.rs:160-215 / quadtree.rs:161-194):
 // at each level the octant is (x > cx) | (y > cy) << 1 [| (z > cz) << 2] and
 // the child centre is c -/+ extent/2, rounded as the reference rounds it;
 // a body clear of every cell boundary takes the same digits from its scaled
 // coordinates (axis_digits).
+//
+// For the radix sort (sort_tag 0) each tile also counts its digits of the
+// first pass. With `fuse` (the sample sort, the batch's previous sorted keys
+// in `prev`) it does the work of k_splitters and k_bucket_count instead: the
+// same splitters (prev at the quantile positions), each key's bucket, the
+// bucket sizes (one atomic per tile and bucket; the bucket sort leaves them
+// zero) and the finer quantiles for the slot sort.
 // One block per radix-sort tile of kSortTile bodies (the batch's segments are
 // concatenated), which also writes the tile's digit counts of the first radix
 // pass (digit-major: cnt[digit * tiles + tile]). The extent
@@ -290,10 +307,24 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
     const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off, uint32_t nseg,
     uint64_t total, int segbits, int lmax, const uint64_t* __restrict__ partial,
     StepCounters* __restrict__ sc, int integrate, uint64_t sort_tag, uint64_t* __restrict__ keys,
-    uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt, uint32_t tiles) {
+    uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt, uint32_t tiles, int fuse,
+    const uint64_t* __restrict__ prev, uint32_t nb, uint16_t* __restrict__ bkt,
+    uint32_t* __restrict__ bcount, uint64_t* __restrict__ quant) {
     __shared__ uint64_t s_wave[kKeyThreads / 64];
     __shared__ uint32_t s_h[256];
+    __shared__ uint64_t s_spl[kMaxBuckets];
+    __shared__ uint32_t s_bh[kMaxBuckets];
     if (threadIdx.x < 256) s_h[threadIdx.x] = 0;
+    if (fuse) {
+        for (uint32_t b = threadIdx.x; b < nb; b += kKeyThreads) {
+            s_bh[b] = 0;
+            if (b + 1 < nb) s_spl[b] = prev[(uint64_t)(b + 1) * total / nb];
+        }
+        const uint64_t m = (uint64_t)nb * kQuantiles;
+        for (uint64_t t = (uint64_t)blockIdx.x * kKeyThreads + threadIdx.x; t < m;
+             t += (uint64_t)gridDim.x * kKeyThreads)
+            quant[t] = prev[t * total / m];
+    }
     const uint64_t g0 = (uint64_t)blockIdx.x * kSortTile;
     const uint32_t sa = seg_of(seg_off, nseg, g0);
     const uint32_t sb = seg_of(seg_off, nseg, min(g0 + (uint64_t)kSortTile, total) - 1);
@@ -304,6 +335,7 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
         // batch's previous step (sort_tag 0: the radix sort, which leaves none)
         sc->split_prev = sort_tag != 0 && sc->sort_tag == sort_tag ? 1u : 0u;
         sc->sort_tag = sort_tag;
+        if (fuse) sc->quantiles = 1;
         sc->ext_tag = 0;  // the partials are read here; k_scan_gather clears them
     }
     // the tile's extent when it lies in one segment (block-uniform branch)
@@ -318,9 +350,11 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
     } else {
         __syncthreads();
     }
+    uint64_t tk[kSortTile / kKeyThreads];  // this thread's keys, for the fused bucket count
     for (int k = 0; k < kSortTile / kKeyThreads; ++k) {
         const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
         uint32_t dig = 256u;
+        tk[k] = 0;
         if (g < total) {
             const uint32_t s = seg_of(seg_off, nseg, g);
             const SegDesc& d = desc[s];
@@ -364,11 +398,27 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
             keys[g] = key;
             vals[g] = (uint32_t)g;
             dig = (uint32_t)(key & 255u);
+            tk[k] = key;
         }
-        wave_count_digit(s_h, dig);
+        if (sort_tag == 0) wave_count_digit(s_h, dig);
     }
     __syncthreads();
-    if (threadIdx.x < 256) cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
+    if (sort_tag == 0) {
+        if (threadIdx.x < 256) cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
+    } else if (fuse) {
+#pragma unroll
+        for (int k = 0; k < kSortTile / kKeyThreads; ++k) {
+            const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
+            if (g < total) {
+                const uint32_t b = bucket_of(s_spl, nb, tk[k]);
+                bkt[g] = (uint16_t)b;
+                atomicAdd(&s_bh[b], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < nb; b += kKeyThreads)
+            if (s_bh[b]) atomicAdd(&bcount[b], s_bh[b]);
+    }
 }
 
 // Stable LSD radix sort of (key, value) pairs, 8-bit digits, three kernels
@@ -660,16 +710,6 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters(
         }
     }
     for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads) spl[b] = s[(b + 1) * kSampleOver];
-}
-
-// Bucket of a key: the number of splitters (sorted, nb - 1 of them) <= key.
-__device__ __forceinline__ uint32_t bucket_of(const uint64_t* s_spl, uint32_t nb, uint64_t key) {
-    uint32_t lo = 0, hi = nb - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_spl[mid] <= key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
 }
 
 // With the previous step's quantiles as splitters, it also copies kQuantiles
@@ -1296,7 +1336,7 @@ __device__ __forceinline__ bool bucket_sort_slots(uint64_t* __restrict__ keys,
 template <int DIM>
 __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_bucket_sort(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t* __restrict__ tkeys,
-    uint32_t* __restrict__ tvals, const uint32_t* __restrict__ bcount,
+    uint32_t* __restrict__ tvals, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bfill,
     const uint32_t* __restrict__ bbase, uint32_t cap, const SegDesc* __restrict__ desc,
     uint64_t total, uint64_t* __restrict__ k2, uint32_t nseg, int segbits, int lmax,
     const uint64_t* __restrict__ quant, const StepCounters* __restrict__ sc) {
@@ -1312,6 +1352,12 @@ __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t cnt = bcount[blockIdx.x];
     const uint64_t base = bbase[blockIdx.x];
+    // the next step's bucket count (fused into k_keys) starts from zero
+    __syncthreads();
+    if (tid == 0) {
+        bcount[blockIdx.x] = 0;
+        bfill[blockIdx.x] = 0;
+    }
     if (cnt <= 1) return;
     // ranking tables of the HBM passes
     uint8_t* s_tab = s_raw;
