@@ -14,7 +14,7 @@ constexpr int kSortItems = kSortTile / kSortThreads;  // keys per thread
 constexpr int kScanItems = 4;      // counts per thread in the node-offset scan
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kMaxPasses = 8;      // 8-bit digits over a 64-bit key
-constexpr int kKeyThreads = 1024;  // threads of a key-building block (one sort tile)
+constexpr int kKeyThreads = 512;   // threads of a key-building block (one sort tile)
 constexpr int kBucketKeys = 1664;  // keys per bucket of the sample sort (on average; steady-state
                                    // buckets stay within 2048, the slot sort's reach)
 constexpr int kBucketCap = 4096;   // largest bucket sorted in LDS (larger ones: in HBM)

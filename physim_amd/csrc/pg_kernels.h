@@ -409,9 +409,18 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
 #pragma unroll
         for (int k = 0; k < kSortTile / kKeyThreads; ++k) {
             const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
-            if (g < total) {
-                const uint32_t b = bucket_of(s_spl, nb, tk[k]);
-                bkt[g] = (uint16_t)b;
+            const bool valid = g < total;
+            const uint32_t b = valid ? bucket_of(s_spl, nb, tk[k]) : 0u;
+            if (valid) bkt[g] = (uint16_t)b;
+            // a wave's keys mostly share a bucket (nearly sorted input): then
+            // one atomic for the wave, else one per key
+            const uint64_t vm = __ballot(valid);
+            const uint32_t bf = (uint32_t)__builtin_amdgcn_readlane(
+                (int)b, vm ? __ffsll((long long)vm) - 1 : 0);
+            if (__ballot(valid && b != bf) == 0) {
+                if (vm && (int)(threadIdx.x & 63) == __ffsll((long long)vm) - 1)
+                    atomicAdd(&s_bh[bf], (uint32_t)__popcll(vm));
+            } else if (valid) {
                 atomicAdd(&s_bh[b], 1u);
             }
         }
