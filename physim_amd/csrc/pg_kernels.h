@@ -771,8 +771,11 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
     constexpr int kWaves = kSortThreads / 64;
     static_assert(kMaxBuckets == 2 * kSortThreads, "two buckets per thread in the scans");
-    __shared__ uint32_t s_wcnt[kWaves][kMaxBuckets];
-    __shared__ uint32_t s_lstart[kMaxBuckets];
+    // per wave and bucket counts, then starts (at most a tile: 16 bits), so
+    // three blocks fit in a CU
+    static_assert(kSortTile < 65536, "16-bit tile offsets");
+    __shared__ __align__(16) uint16_t s_wcnt[kWaves][kMaxBuckets];
+    __shared__ uint16_t s_lstart[kMaxBuckets];
     __shared__ uint32_t s_dst[kMaxBuckets];
     __shared__ uint32_t s_scan[kWaves];
     __shared__ uint64_t s_keys[kSortTile];
@@ -782,7 +785,8 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
     // the bucket sizes of this thread's two buckets, loaded early
     const uint32_t b0 = 2u * (uint32_t)tid, b1 = b0 + 1u;
     const uint32_t n0 = b0 < nb ? bcount[b0] : 0u, n1 = b1 < nb ? bcount[b1] : 0u;
-    for (int i = tid; i < kWaves * kMaxBuckets; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+    for (int i = tid; i < kWaves * kMaxBuckets / 2; i += kSortThreads)
+        reinterpret_cast<uint32_t*>(&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
     uint64_t key[kSortItems];
@@ -818,7 +822,7 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
         uint32_t old = 0;
         if (valid && lane == leader) {
             old = s_wcnt[wave][b];
-            s_wcnt[wave][b] = old + (uint32_t)__popcll(peers);
+            s_wcnt[wave][b] = (uint16_t)(old + (uint32_t)__popcll(peers));
         }
         old = __shfl(old, leader);
         rank[k] = old + (uint32_t)__popcll(peers & lt);
@@ -830,16 +834,16 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
         const uint32_t x0 = s_wcnt[w][b0], x1 = s_wcnt[w][b1];
-        s_wcnt[w][b0] = c0;
-        s_wcnt[w][b1] = c1;
+        s_wcnt[w][b0] = (uint16_t)c0;
+        s_wcnt[w][b1] = (uint16_t)c1;
         c0 += x0;
         c1 += x1;
     }
     uint32_t dummy;
     const uint32_t l0 = block_exclusive_scan<kSortThreads>(c0 + c1, s_scan, &dummy);
     const uint32_t g0 = block_exclusive_scan<kSortThreads>(n0 + n1, s_scan, &dummy);
-    s_lstart[b0] = l0;
-    s_lstart[b1] = l0 + c0;
+    s_lstart[b0] = (uint16_t)l0;
+    s_lstart[b1] = (uint16_t)(l0 + c0);
     s_dst[b0] = c0 ? g0 + atomicAdd(&bfill[b0], c0) : 0u;
     s_dst[b1] = c1 ? g0 + n0 + atomicAdd(&bfill[b1], c1) : 0u;
     if (blockIdx.x == 0) {
