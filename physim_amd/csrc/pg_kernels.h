@@ -1124,11 +1124,11 @@ __device__ __forceinline__ void bucket_sort_regs(uint64_t* __restrict__ keys,
 // sort takes the bucket).
 constexpr int kSlotBits = 11;
 constexpr uint32_t kSlots = 1u << kSlotBits;
-constexpr uint32_t kSlotRun = 16;
+constexpr uint32_t kSlotRun = 8;
 constexpr int kCoarseBits = 8;
 constexpr uint32_t kCoarse = 1u << kCoarseBits;
 static_assert(kQuantiles == (int)kCoarse, "a coarse bin between consecutive quantiles");
-constexpr uint32_t kWideSlots = 64;  // slots of more than kSlotRun keys sorted by waves
+constexpr uint32_t kWideSlots = 128;  // slots of more than kSlotRun keys sorted by waves
 
 // Sorts one (key, value) per lane across a wave by (key, value), ascending in
 // lane order (a register bitonic network).
