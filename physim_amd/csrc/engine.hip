@@ -102,7 +102,7 @@ struct pg_engine {
     uint32_t* ext_chunk = nullptr;
     uint64_t* ext_partial = nullptr;
     uint64_t* top_slot = nullptr;  // generation-tagged tile-spanning node per (tile, level)
-    uint32_t* top_last = nullptr;  // its last body
+    uint2* top_last = nullptr;     // its last body and that body's tile
     Sum4* out_part = nullptr;      // per tile and level: suffix sums of spanning nodes
     Sum4* in_part = nullptr;       // per tile and level: prefix sums of spanning nodes
     Sum4* tot_part = nullptr;      // per tile: mass sums of all its bodies
@@ -406,7 +406,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     const uint64_t o_es = take(chunks * 4), o_ec = take(chunks * 4), o_ep = take(chunks * 8);
     // a tile boundary is crossed by at most one node per level (levels 0..32)
     const uint64_t top = (node_tiles + 2) * kTopSlots;
-    const uint64_t o_tl = take(top * 8), o_tla = take(top * 4);
+    const uint64_t o_tl = take(top * 8), o_tla = take(top * 8);
     const uint64_t total = off;
     if (e->sims_bytes + total > e->budget)
         return pg_set_error(PG_ERR_BUDGET, "scratch of %llu bytes exceeds the HBM budget",
@@ -456,7 +456,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->ext_chunk = (uint32_t*)(b + o_ec);
     e->ext_partial = (uint64_t*)(b + o_ep);
     e->top_slot = (uint64_t*)(b + o_tl);
-    e->top_last = (uint32_t*)(b + o_tla);
+    e->top_last = (uint2*)(b + o_tla);
     e->out_part = (Sum4*)(b + o_op);
     e->in_part = (Sum4*)(b + o_ip);
     e->tot_part = (Sum4*)(b + o_tp);

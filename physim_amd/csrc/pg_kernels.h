@@ -1885,7 +1885,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     const uint32_t* __restrict__ tile_base, const uint32_t* __restrict__ lvl_first,
     uint32_t* __restrict__ first_node, const uint8_t* __restrict__ lo_level,
     NodeHot* __restrict__ hot, uint64_t cap, uint64_t* __restrict__ top_slot,
-    uint32_t* __restrict__ top_last, Sum4* __restrict__ out_part, Sum4* __restrict__ in_part,
+    uint2* __restrict__ top_last, Sum4* __restrict__ out_part, Sum4* __restrict__ in_part,
     Sum4* __restrict__ tot_part, const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc) {
     static_assert(kNodeTile <= 65536, "owner indices are 16-bit");
     __shared__ uint64_t s_keys[kNodeTile];
@@ -2014,7 +2014,9 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             skip = internal_skip(b0 + li, s_keys[li], L, last, spans);
             if (spans) {
                 top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
-                top_last[(uint64_t)tile * kTopSlots + L] = (uint32_t)last;
+                // its last body and that body's tile (the node lies in this segment)
+                top_last[(uint64_t)tile * kTopSlots + L] =
+                    make_uint2((uint32_t)last, d.tile0 + (uint32_t)((last - d.off) / kNodeTile));
                 atomicMax(&s_topl, (uint32_t)L + 1u);
             } else {
                 const uint32_t e = (uint32_t)(last - b0) + 1u;
@@ -2216,7 +2218,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
 __global__ __launch_bounds__(kBlock) void k_com_span(
     const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off, uint32_t nseg,
     const uint32_t* __restrict__ tile_start, NodeHot* __restrict__ hot,
-    const uint32_t* __restrict__ top_last, const uint64_t* __restrict__ top_slot,
+    const uint2* __restrict__ top_last, const uint64_t* __restrict__ top_slot,
     const Sum4* __restrict__ out_part, const Sum4* __restrict__ in_part,
     const Sum4* __restrict__ tot_part, const StepCounters* __restrict__ sc) {
     const uint32_t t = blockIdx.x;
@@ -2228,20 +2230,18 @@ __global__ __launch_bounds__(kBlock) void k_com_span(
         const uint64_t sv = top_slot[(uint64_t)t * kTopSlots + L];
         if ((uint32_t)(sv >> 32) != gen) continue;
         const uint32_t nd = (uint32_t)sv;
-        const uint64_t last = top_last[(uint64_t)t * kTopSlots + L];
-        const uint32_t t1 = tile_of(desc, seg_off, nseg, last);
+        const uint2 lt = top_last[(uint64_t)t * kTopSlots + L];
+        const uint64_t last = lt.x;
+        const uint32_t t1 = lt.y;
         Sum4 a{0.0, 0.0, 0.0, 0.0};
         for (uint32_t tt = t + 1u + (uint32_t)lane; tt < t1; tt += 64u) a.add(tot_part[tt]);
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            Sum4 b;
-            b.m = __shfl_xor(a.m, o);
-            b.x = __shfl_xor(a.x, o);
-            b.y = __shfl_xor(a.y, o);
-            b.z = __shfl_xor(a.z, o);
-            a = lane & o ? Sum4{b.m + a.m, b.x + a.x, b.y + a.y, b.z + a.z}
-                         : Sum4{a.m + b.m, a.x + b.x, a.y + b.y, a.z + b.z};
-        }
+        // butterfly over the lanes (lane 0's association is fixed)
+        fold_lanes<32>(a);
+        fold_lanes<16>(a);
+        fold_lanes<8>(a);
+        fold_lanes<4>(a);
+        fold_lanes<2>(a);
+        fold_lanes<1>(a);
         if (lane == 0) {
             Sum4 r = out_part[(uint64_t)t * kTopSlots + L];
             r.add(a);
