@@ -33,7 +33,9 @@ def algorithmic_bytes(kernel, n, nodes):
     tiles = (n + SORT_TILE - 1) // SORT_TILE
     table = {
         "extent": 24.0 * n,                          # read x, y, z
-        "keys": 36.0 * n + 1024.0 * tiles,           # read x, y, z; write key, index; digit counts
+        # read x, y, z; write key, index; then digit counts (radix) or each
+        # key's bucket (sample sort)
+        "keys": 36.0 * n + 2.0 * n,
         "sort_count": 8.0 * n + 1024.0 * tiles,      # read keys; write digit counts
         "sort_scan": 2048.0 * tiles,                 # read + write the count matrix
         "sort_scatter": 24.0 * n + 1024.0 * tiles,   # read + write (key, index); read offsets
