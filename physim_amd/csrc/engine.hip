@@ -1287,6 +1287,37 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
         double E;
         memcpy(&E, &st.extent_bits, 8);
         for (uint64_t k = 0; k < nn && k < cap; ++k) fl[8 * k + 7] = std::ldexp(E, -(int)hot[k].level);
+        // Leaves' centres, which the build does not store (the walk never
+        // reads them): the reference's descent (octree.rs:160-215 /
+        // quadtree.rs:161-194) along the leaf body's key to the leaf's level,
+        // the same operations as the device's.
+        std::vector<uint64_t> keys(n), k2;
+        HIP_TRY(hipMemcpy(keys.data(), e->keys[e->last_sorted], n * 8, hipMemcpyDeviceToHost));
+        const int lmax = e->last_lmax, dim = e->last_dim;
+        const uint32_t mask = (1u << dim) - 1u;
+        uint64_t owner = 0;
+        for (uint64_t k = 0; k < nn && k < cap; ++k) {
+            while (owner + 1 < n && fn[owner + 1] <= k) ++owner;
+            if (hot[k].skip != k + 1) continue;  // internal
+            const int L = (int)hot[k].level;
+            if (L > lmax && k2.empty()) {
+                k2.resize(n);
+                HIP_TRY(hipMemcpy(k2.data(), e->k2, n * 8, hipMemcpyDeviceToHost));
+            }
+            const uint64_t key = keys[owner], kx = L > lmax ? k2[owner] : 0ull;
+            double ext = E, c[3] = {0.0, 0.0, 0.0};
+            for (int l = 0; l < L; ++l) {
+                const uint32_t o =
+                    l < lmax ? (uint32_t)((key >> (dim * (lmax - 1 - l))) & mask)
+                             : (uint32_t)((kx >> (dim * (2 * lmax - 1 - l))) & mask);
+                const double h2 = ext * 0.5;
+                for (int a = 0; a < dim; ++a) c[a] = ((o >> a) & 1u) ? c[a] + h2 : c[a] - h2;
+                ext = h2;
+            }
+            fl[8 * k + 4] = c[0];
+            fl[8 * k + 5] = c[1];
+            fl[8 * k + 6] = c[2];
+        }
     }
     return PG_OK;
 }

@@ -2050,6 +2050,18 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         }
     };
 
+    // A leaf's record without its geometric centre: the walk always takes a
+    // leaf and never reads its centre (pg_sim_tree supplies it for the tree
+    // download), so the leaf pass skips the descent.
+    auto build_leaf = [&](uint32_t nd, uint32_t li, int L) {
+        const uint64_t key = s_keys[li];
+        const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
+        const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
+        uint2* w = reinterpret_cast<uint2*>(hot + nd);
+        w[3] = pack1(s_bx[li]);
+        reinterpret_cast<uint4*>(w)[2] = pack2(s_by[li], s_bz[li]);
+        reinterpret_cast<uint4*>(w)[3] = pack_tail(s_bm[li], s_first[li + 1], (uint32_t)L, oct);
+    };
     // node nd (level L) of local body li: its centre by the reference's
     // descent (from the tile's common path), then its record
     auto build = [&](uint32_t nd, uint32_t li, int L, bool leaf) {
@@ -2068,7 +2080,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         // kNodeLds nodes, unless the capacity cut it short, an error)
         for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
             const uint32_t f = s_first[li], fn = s_first[li + 1];
-            if (fn - 1u < n1) build(fn - 1u, li, (int)s_lo[li] + (int)(fn - 1u - f), true);
+            if (fn - 1u < n1) build_leaf(fn - 1u, li, (int)s_lo[li] + (int)(fn - 1u - f));
             const uint32_t ni = (min(fn - 1u, n1) > f) ? min(fn - 1u, n1) - f : 0u;
             if (ni) {
                 const uint32_t p = atomicAdd(&s_ni, ni);
