@@ -1908,7 +1908,9 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     const uint32_t nend = tile_base[tile + 1];
     const uint32_t n1 = (uint32_t)min((uint64_t)nend, cap);
     const uint32_t nloc = n1 > n0 ? n1 - n0 : 0u;
-    const bool node_par = nloc <= (uint32_t)kNodeLds;
+    // node-parallel where the internal nodes fit the list: a tile's bodies
+    // own one leaf each (a tile cut short by a segment's end has fewer)
+    const bool node_par = nloc <= nb + (uint32_t)kNodeQueue;
     // a node tile lies in one segment
     const SegDesc& d = desc[seg_of(seg_off, nseg, b0)];
     const double ext0 = bits_f64(d.st->extent_bits);

@@ -171,6 +171,54 @@ def test_clustered_tile_tree(engine):
     assert fp64_ok(rel_err(a, ref))
 
 
+def short_tile_state(npairs=200, sep=5e-4, seed=31):
+    """1024 bodies in the all-negative corner (the first node tile, sorted),
+    then `npairs` pairs `sep` apart in the all-positive corner: a second tile
+    of 2 npairs bodies whose pairs hang on chains of single-child cells, so
+    it holds far more internal nodes than a full tile's share."""
+    rng = np.random.default_rng(seed)
+    a = np.zeros(1024, P.ENTITY)
+    for ax in ("x", "y", "z"):
+        a[ax] = rng.uniform(-1.0, -0.5, 1024)
+    a["mass"] = rng.uniform(0.5, 1.5, 1024)
+    c = np.zeros(npairs, P.ENTITY)
+    for ax in ("x", "y", "z"):
+        c[ax] = rng.uniform(0.5, 1.0, npairs)
+    c["mass"] = rng.uniform(0.5, 1.5, npairs)
+    t = c.copy()
+    for ax in ("x", "y", "z"):
+        t[ax] += sep * rng.uniform(0.5, 1.0, npairs) * np.where(rng.random(npairs) < 0.5, -1, 1)
+    s = P.concat([a, c, t])
+    s["id"] = np.arange(len(s))
+    return s
+
+
+@pytest.mark.parametrize("kind,dim", [("astro2", 3), ("astro", 2)])
+def test_short_tile_many_internal_nodes(engine, kind, dim):
+    """A node tile cut short by the segment's end (400 bodies) that holds
+    more internal nodes than a full tile's node-parallel list (768) while
+    its node count stays under a full tile's limit (1792): every node is
+    built, topology and centres bit-exact, masses and centres of mass to
+    rounding."""
+    s = short_tile_state()
+    t = O.Tree.from_state(s, dim)
+    orec, ofl = t.serialize()
+    assert t.merges == 0 and t.fault == 0
+    leaves = np.nonzero(orec[:, 2] >= 0)[0]
+    nloc = len(orec) - (leaves[1023] + 1)  # nodes owned by the second tile's bodies
+    assert nloc - (len(s) - 1024) > 768 and nloc <= 1792
+    sim = P.Sim(engine, s, kind=kind, precision="fp64", **CFG1)
+    a = sim.accel()
+    rec, fl = sim.tree()
+    assert rec.shape == orec.shape
+    np.testing.assert_array_equal(rec[:, :3], orec[:, :3])
+    np.testing.assert_array_equal(fl[:, 4:8], ofl[:, 4:8])
+    np.testing.assert_allclose(fl[:, 0], ofl[:, 0], rtol=1e-13)
+    np.testing.assert_allclose(fl[:, 1:4], ofl[:, 1:4], rtol=1e-13, atol=1e-15)
+    ref, _, _ = O.bh_transform(s, CFG1["theta"], CFG1["e"], dim=dim)
+    assert fp64_ok(rel_err(a, ref))
+
+
 def octant_path_key(p, extent, dim, levels):
     c = [0.0, 0.0, 0.0]
     h = extent
