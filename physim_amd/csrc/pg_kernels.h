@@ -2350,7 +2350,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                                                  uint32_t nseg, double* __restrict__ acc_out,
                                                  uint64_t* __restrict__ ext_partial,
                                                  StepCounters* __restrict__ sc, uint64_t tag) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // Workgroups go to the 8 XCDs in turn: the blocks of one XCD take one
+    // contiguous eighth of the sorted bodies, so the node records their
+    // walks share stay in that XCD's L2 (a bijection for any grid size)
+    const uint32_t nbk = gridDim.x, xcd = blockIdx.x & 7u, per = nbk >> 3, rem = nbk & 7u;
+    const uint32_t blk = xcd * per + min(xcd, rem) + (blockIdx.x >> 3);
+    const uint64_t g = (uint64_t)blk * blockDim.x + threadIdx.x;
     bool valid = g < total;
     uint32_t s = 0;
     double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
