@@ -1479,7 +1479,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     __shared__ uint32_t s_cnt[kScanTile];
     __shared__ uint32_t s_firstlo[kTopSlots];
     const int tid = threadIdx.x;
-    const uint32_t tile = blockIdx.x;
+    // the blocks of one XCD (workgroups go to the 8 XCDs in turn) take one
+    // contiguous eighth of the tiles: the source lines that neighbouring
+    // tiles share are then fetched once, into that XCD's L2
+    const uint32_t nbk = gridDim.x, xcd = blockIdx.x & 7u, per = nbk >> 3, rem = nbk & 7u;
+    const uint32_t tile = xcd * per + min(xcd, rem) + (blockIdx.x >> 3);
     const uint64_t tbase = tile_start[tile], tend = tile_start[tile + 1];
     if (tid < kTopSlots) s_firstlo[tid] = ~0u;
     // the extent partials were read by k_keys: cleared for the walk's maxima
