@@ -13,6 +13,8 @@ namespace pg {
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ double bits_f64(uint64_t b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ uint64_t f64_bits(double d) { return (uint64_t)__double_as_longlong(d); }
+__device__ __forceinline__ float bits_f32(uint32_t b) { return __uint_as_float(b); }
+__device__ __forceinline__ uint32_t f32_bits(float f) { return __float_as_uint(f); }
 
 // The reference's `extent / 2.0` (octree.rs:97-100). Halving by a multiply
 // rounds exactly as the IEEE division does (scaling by a power of two, also
@@ -2572,7 +2574,10 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
     double f64[3] = {0.0, 0.0, 0.0};
     for (uint64_t t0 = 0; t0 < d.n; t0 += kBlock) {
         const uint64_t jj = t0 + threadIdx.x;
-        if (jj < d.n) sj[threadIdx.x] = make_double4(d.sx[jj], d.sy[jj], d.sz[jj], d.sm[jj]);
+        // mixed precision stages the mass as FP32 in the fourth word
+        if (jj < d.n)
+            sj[threadIdx.x] = make_double4(d.sx[jj], d.sy[jj], d.sz[jj],
+                                           FP64 ? d.sm[jj] : bits_f64((uint64_t)f32_bits((float)d.sm[jj])));
         __syncthreads();
         const int cnt = (int)min((uint64_t)kBlock, d.n - t0);
         if (FP64) {
@@ -2589,8 +2594,12 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
                 const float fx = (float)(b.x - x), fy = (float)(b.y - y), fz = (float)(b.z - z);
                 const float s2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
                 const float t = s2 + ef;
-                float w = (float)b.w * __builtin_amdgcn_rsqf(s2 * t * t);
-                w = s2 > 0.0f ? w : 0.0f;
+                // a body's own entry (and any equal position) has fx = fy =
+                // fz = 0: the floor (the least normal FP32) keeps its weight
+                // finite, so it adds 0; it changes no other weight unless
+                // r^2 (r^2 + e)^2 is below 2^-126
+                const float w = bits_f32((uint32_t)f64_bits(b.w)) *
+                                __builtin_amdgcn_rsqf(fmaxf(s2 * t * t, 1.17549435e-38f));
                 px = fmaf(fx, w, px);
                 py = fmaf(fy, w, py);
                 pz = fmaf(fz, w, pz);

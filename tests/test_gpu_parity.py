@@ -288,6 +288,19 @@ def test_simple_astro_matches_oracle(engine, precision, tol):
     assert np.quantile(r, 0.999) <= tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-4)])
+def test_simple_astro_close_pairs(engine, precision, tol):
+    """Pairs 3e-9 to 2e-7 apart: their separations carry the softened pair
+    force's direction, so they must keep their FP64 precision until the
+    difference is taken."""
+    s = close_pairs_state()
+    ref = O.simple_transform(s, 0.5)
+    a = P.Sim(engine, s, kind="simple_astro", precision=precision, e=0.5).accel()
+    r = rel_err(a, ref)
+    print(f"simple_astro close pairs {precision} p99.9 {np.quantile(r, 0.999):.3e} max {r.max():.3e}")
+    assert np.quantile(r, 0.999) <= tol and r.max() <= 10 * tol
+
+
 def test_theta_zero_is_direct_sum(engine):
     s = small_cube(2000, seed=6)
     ref = O.simple_transform(s, 0.5)
