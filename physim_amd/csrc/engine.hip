@@ -516,6 +516,19 @@ static void ev_collect(pg_engine* e) {
 }
 
 // ---------------------------------------------------------- step sequence
+// Source splits of a direct-sum segment of n bodies: small segments are too
+// few target tiles to fill the GPU, so up to kMaxDirectSplit blocks share a
+// target tile, each summing a contiguous range of source tiles. The count
+// follows from the segment's own size, so a segment sums alike alone and in
+// a batch.
+constexpr uint32_t kMaxDirectSplit = 4;
+static uint32_t direct_splits(uint64_t n) {
+    const uint64_t tiles = (n + kBlock - 1) / kBlock;
+    uint64_t s = n ? ((1ull << 19) + n - 1) / n : 1;
+    s = std::min<uint64_t>(std::min<uint64_t>(s, kMaxDirectSplit), tiles);
+    return (uint32_t)std::max<uint64_t>(s, 1);
+}
+
 struct BatchPlan {
     int kind = PG_KIND_OCTREE;
     int precision = PG_PREC_MIXED;
@@ -527,6 +540,7 @@ struct BatchPlan {
     uint32_t nblocks = 0;
     uint32_t nchunks = 0;
     uint32_t ntiles = 0;  // node tiles
+    uint32_t max_split = 1;  // direct sum: the most source splits of a segment
     bool integrate = true;
 };
 
@@ -644,9 +658,15 @@ static void enqueue_direct_step(pg_engine* e, const BatchPlan& bp, int q, bool c
     launch(e, cap, "prep", [&] {
         k_prep<<<1, kBlock, 0, st>>>(desc, bp.nseg, e->sc, INTEG ? 1 : 0);
     });
+    // (one timed bracket: the sum and, where sources are split, the pass
+    // that adds the partial sums and stores)
     launch(e, cap, "direct", [&] {
+        double* part = reinterpret_cast<double*>(e->hot);
         k_direct<FP64, INTEG><<<bp.nblocks, kBlock, 0, st>>>(desc, e->blk_seg, e->blk_tile,
-                                                            e->acc_out);
+                                                            part, e->acc_out);
+        if (bp.max_split > 1)
+            k_direct_finish<FP64, INTEG><<<grid_for(bp.total, 1u << 30), kBlock, 0, st>>>(
+                desc, e->seg_off, bp.nseg, bp.total, part, e->acc_out);
     });
 }
 
@@ -746,15 +766,32 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     if (nodes >= (1ull << 32)) return pg_set_error(PG_ERR_ARG, "tree too large for 32-bit nodes");
     e->h_blk_seg.clear();
     e->h_blk_tile.clear();
+    std::vector<uint32_t> nsplit(bp.nseg, 1u);
+    std::vector<uint64_t> part_off(bp.nseg, 0ull);
     if (bp.kind == PG_KIND_DIRECT) {
+        // Blocks: per segment, target tile and source split (split in the
+        // tile word's top 8 bits). A segment's partial sums (3 doubles per
+        // body and split) live in the node array, which the direct sum does
+        // not otherwise use.
+        uint64_t part = 0;
         for (uint32_t s = 0; s < bp.nseg; ++s) {
             const uint32_t nb = (uint32_t)((sims[s]->n + kBlock - 1) / kBlock);
-            for (uint32_t t = 0; t < nb; ++t) {
-                e->h_blk_seg.push_back(s);
-                e->h_blk_tile.push_back(t);
+            const uint32_t S = direct_splits(sims[s]->n);
+            for (uint32_t t = 0; t < nb; ++t)
+                for (uint32_t p = 0; p < S; ++p) {
+                    e->h_blk_seg.push_back(s);
+                    e->h_blk_tile.push_back(t | (p << 24));
+                }
+            nsplit[s] = S;
+            if (S > 1) {
+                part_off[s] = part;
+                part += (uint64_t)S * sims[s]->n;
+                bp.max_split = std::max(bp.max_split, S);
             }
         }
         bp.nblocks = (uint32_t)e->h_blk_seg.size();
+        nodes = (part * 24 + sizeof(NodeHot) - 1) / sizeof(NodeHot);
+        if (nodes >= (1ull << 32)) return pg_set_error(PG_ERR_ARG, "direct-sum partials too large");
     }
     e->h_ext_seg.clear();
     e->h_ext_chunk.clear();
@@ -821,6 +858,8 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
             d.chunk0 = chunk0[s];
             d.nchunks = nchunk[s];
             d.tile0 = tile0[s];
+            d.nsplit = nsplit[s];
+            d.part_off = part_off[s];
         }
         off += sm->n;
     }

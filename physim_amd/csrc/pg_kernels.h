@@ -2546,19 +2546,73 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     }
 }
 
+// The end of a direct-sum evaluation for body i of segment d (position x, y,
+// z, mass m): its acceleration from the summed forces f (FP64: sums of
+// m_i m_j terms, divided by m here; mixed: accelerations), then the Stormer
+// update (verlet.rs:24-82) into dst, or the acceleration into acc_out.
+template <bool FP64, bool INTEGRATE>
+__device__ __forceinline__ void direct_store(const SegDesc& d, uint64_t i, double x, double y,
+                                             double z, double m, bool fixed, double fx,
+                                             double fy, double fz, double* acc_out) {
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (!fixed) {
+        if (FP64) {
+            ax = fx / m; ay = fy / m; az = fz / m;
+        } else if (m == 0.0) {
+            ax = ay = az = __builtin_nan("");
+        } else {
+            ax = fx; ay = fy; az = fz;
+        }
+    }
+    if (INTEGRATE) {
+        // velocities are stored only by the Taylor first step
+        const double dt = d.dt;
+        const double dt2 = dt * dt;
+        if (d.st->history == 0) {
+            const double ux = d.svx[i], uy = d.svy[i], uz = d.svz[i];
+            d.dx[i] = x + ux * dt + 0.5 * ax * dt2;
+            d.dy[i] = y + uy * dt + 0.5 * ay * dt2;
+            d.dz[i] = z + uz * dt + 0.5 * az * dt2;
+            d.dvx[i] = ux + ax * dt;
+            d.dvy[i] = uy + ay * dt;
+            d.dvz[i] = uz + az * dt;
+        } else {
+            d.dx[i] = 2.0 * x - d.spx[i] + ax * dt2;
+            d.dy[i] = 2.0 * y - d.spy[i] + ay * dt2;
+            d.dz[i] = 2.0 * z - d.spz[i] + az * dt2;
+        }
+        d.dpx[i] = x; d.dpy[i] = y; d.dpz[i] = z;
+        d.dm[i] = m;
+        d.dfix[i] = d.sfix[i];
+        d.did[i] = d.sid[i];
+    } else {
+        double* o = acc_out + 3 * (d.off + d.sid[i]);
+        o[0] = ax; o[1] = ay; o[2] = az;
+    }
+}
+
 // simple_astro (transformers.rs:219-244): LDS-tiled all-pairs sum. Each block
-// holds 256 targets of one segment and streams the segment's sources through
-// LDS. Mixed precision sums each 256-source tile in FP32 and adds the tile
-// partial into a compensated FP32 sum. Reads src, writes dst (same order).
+// holds 256 targets of one segment and streams a range of the segment's
+// sources through LDS: all of them, or one of d.nsplit contiguous ranges of
+// source tiles (the tile word's top 8 bits), whose partial sums k_direct_finish
+// adds in range order. Mixed precision sums each 256-source tile in FP32 and
+// adds the tile partial into a compensated FP32 sum. Reads src, writes dst
+// (same order).
 template <bool FP64, bool INTEGRATE>
 __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ desc,
                                                    const uint32_t* __restrict__ blk_seg,
                                                    const uint32_t* __restrict__ blk_tile,
+                                                   double* __restrict__ part,
                                                    double* __restrict__ acc_out) {
     __shared__ double4 sj[kBlock];
     const uint32_t s = blk_seg[blockIdx.x];
     const SegDesc& d = desc[s];
-    const uint64_t i = (uint64_t)blk_tile[blockIdx.x] * kBlock + threadIdx.x;
+    const uint32_t tw = blk_tile[blockIdx.x];
+    const uint64_t i = (uint64_t)(tw & 0xFFFFFFu) * kBlock + threadIdx.x;
+    const uint32_t sp = tw >> 24, ns = d.nsplit;
+    const uint64_t src_tiles = (d.n + kBlock - 1) / kBlock;
+    const uint64_t j0 = src_tiles * sp / ns * kBlock;
+    const uint64_t j1 = min(src_tiles * (sp + 1) / ns * kBlock, d.n);
     const bool valid = i < d.n;
     double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
     bool fixed = true;
@@ -2572,14 +2626,14 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
     Kahan3 ka;
     ka.zero();
     double f64[3] = {0.0, 0.0, 0.0};
-    for (uint64_t t0 = 0; t0 < d.n; t0 += kBlock) {
+    for (uint64_t t0 = j0; t0 < j1; t0 += kBlock) {
         const uint64_t jj = t0 + threadIdx.x;
         // mixed precision stages the mass as FP32 in the fourth word
-        if (jj < d.n)
+        if (jj < j1)
             sj[threadIdx.x] = make_double4(d.sx[jj], d.sy[jj], d.sz[jj],
                                            FP64 ? d.sm[jj] : bits_f64((uint64_t)f32_bits((float)d.sm[jj])));
         __syncthreads();
-        const int cnt = (int)min((uint64_t)kBlock, d.n - t0);
+        const int cnt = (int)min((uint64_t)kBlock, j1 - t0);
         if (FP64) {
             for (int k = 0; k < cnt; ++k) {
                 const double4 b = sj[k];
@@ -2609,43 +2663,43 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
         __syncthreads();
     }
     if (!valid) return;
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    if (!fixed) {
-        if (FP64) {
-            ax = f64[0] / m; ay = f64[1] / m; az = f64[2] / m;
-        } else if (m == 0.0) {
-            ax = ay = az = __builtin_nan("");
-        } else {
-            ax = (double)ka.sx - (double)ka.cx;
-            ay = (double)ka.sy - (double)ka.cy;
-            az = (double)ka.sz - (double)ka.cz;
-        }
-    }
-    if (INTEGRATE) {
-        // verlet.rs:24-82; velocities are stored only by the Taylor first step
-        const double dt = d.dt;
-        const double dt2 = dt * dt;
-        if (d.st->history == 0) {
-            const double ux = d.svx[i], uy = d.svy[i], uz = d.svz[i];
-            d.dx[i] = x + ux * dt + 0.5 * ax * dt2;
-            d.dy[i] = y + uy * dt + 0.5 * ay * dt2;
-            d.dz[i] = z + uz * dt + 0.5 * az * dt2;
-            d.dvx[i] = ux + ax * dt;
-            d.dvy[i] = uy + ay * dt;
-            d.dvz[i] = uz + az * dt;
-        } else {
-            d.dx[i] = 2.0 * x - d.spx[i] + ax * dt2;
-            d.dy[i] = 2.0 * y - d.spy[i] + ay * dt2;
-            d.dz[i] = 2.0 * z - d.spz[i] + az * dt2;
-        }
-        d.dpx[i] = x; d.dpy[i] = y; d.dpz[i] = z;
-        d.dm[i] = m;
-        d.dfix[i] = d.sfix[i];
-        d.did[i] = d.sid[i];
+    double fx, fy, fz;
+    if (FP64) {
+        fx = f64[0]; fy = f64[1]; fz = f64[2];
     } else {
-        double* o = acc_out + 3 * (d.off + d.sid[i]);
-        o[0] = ax; o[1] = ay; o[2] = az;
+        fx = (double)ka.sx - (double)ka.cx;
+        fy = (double)ka.sy - (double)ka.cy;
+        fz = (double)ka.sz - (double)ka.cz;
     }
+    if (ns > 1) {
+        double* o = part + 3 * (d.part_off + (uint64_t)sp * d.n + i);
+        o[0] = fx; o[1] = fy; o[2] = fz;
+        return;
+    }
+    direct_store<FP64, INTEGRATE>(d, i, x, y, z, m, fixed, fx, fy, fz, acc_out);
+}
+
+// Bodies of segments whose sources k_direct split: the partial sums added in
+// range order, then the store.
+template <bool FP64, bool INTEGRATE>
+__global__ __launch_bounds__(kBlock) void k_direct_finish(const SegDesc* __restrict__ desc,
+                                                          const uint64_t* __restrict__ seg_off,
+                                                          uint32_t nseg, uint64_t total,
+                                                          const double* __restrict__ part,
+                                                          double* __restrict__ acc_out) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const SegDesc& d = desc[seg_of(seg_off, nseg, g)];
+    const uint32_t ns = d.nsplit;
+    if (ns <= 1) return;
+    const uint64_t i = g - d.off;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (uint32_t sp = 0; sp < ns; ++sp) {
+        const double* o = part + 3 * (d.part_off + (uint64_t)sp * d.n + i);
+        fx += o[0]; fy += o[1]; fz += o[2];
+    }
+    direct_store<FP64, INTEGRATE>(d, i, d.sx[i], d.sy[i], d.sz[i], d.sm[i], d.sfix[i] != 0, fx,
+                                  fy, fz, acc_out);
 }
 
 // Velocity of body j: stored (before and right after the Taylor first step)
