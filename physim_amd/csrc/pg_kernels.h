@@ -2356,11 +2356,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                                                  uint32_t nseg, double* __restrict__ acc_out,
                                                  uint64_t* __restrict__ ext_partial,
                                                  StepCounters* __restrict__ sc, uint64_t tag) {
-    // Workgroups go to the 8 XCDs in turn: the blocks of one XCD take one
-    // contiguous eighth of the sorted bodies, so the node records their
-    // walks share stay in that XCD's L2 (a bijection for any grid size)
-    const uint32_t nbk = gridDim.x, xcd = blockIdx.x & 7u, per = nbk >> 3, rem = nbk & 7u;
-    const uint32_t blk = xcd * per + min(xcd, rem) + (blockIdx.x >> 3);
+    // Workgroups go to the 8 XCDs in turn. Each XCD takes runs of 16
+    // consecutive blocks (4096 sorted bodies), so neighbouring walks share
+    // their node records in one L2, while the runs of a batch's costly
+    // segments still spread over all XCDs (whole groups of 8 runs; the
+    // blocks past them keep their own index: a bijection for any grid)
+    uint32_t blk = blockIdx.x;
+    if (blockIdx.x < (gridDim.x & ~127u)) {
+        const uint32_t x = blockIdx.x & 7u, k = blockIdx.x >> 3;
+        blk = (k >> 4) * 128u + x * 16u + (k & 15u);
+    }
     const uint64_t g = (uint64_t)blk * blockDim.x + threadIdx.x;
     bool valid = g < total;
     uint32_t s = 0;
