@@ -67,6 +67,18 @@ typedef struct Acceleration {
     double x, y, z;
 } Acceleration;
 
+/* The layouts the reference's repr(C) structs fix (checked at compile time
+ * wherever this header is included). */
+#if defined(__cplusplus)
+static_assert(sizeof(Entity) == 80 && offsetof(Entity, id) == 64 && offsetof(Entity, fixed) == 72,
+              "Entity must match physim-core's repr(C) layout");
+static_assert(sizeof(Acceleration) == 24, "Acceleration is three doubles");
+#elif defined(__STDC_VERSION__) && __STDC_VERSION__ >= 201112L
+_Static_assert(sizeof(Entity) == 80 && offsetof(Entity, id) == 64 && offsetof(Entity, fixed) == 72,
+               "Entity must match physim-core's repr(C) layout");
+_Static_assert(sizeof(Acceleration) == 24, "Acceleration is three doubles");
+#endif
+
 /* physim-core/src/plugin/meta.rs:317 */
 typedef char* (*RustStringAllocFn)(const char*);
 

@@ -77,6 +77,11 @@ def lib():
             "or_pipeline_steps": (I, [P, P, SZ, I, D, D, D, I]),
             "or_pipeline_steps_timed": (I, [P, P, SZ, D, D, D, I]),
             "or_energy": (None, [P, SZ, D, P]),
+            "or_bh_transform_rows": (I, [I, P, SZ, D, D, P, SZ, P, P]),
+            "or_pipeline_steps_timed_mt": (I, [P, P, SZ, D, D, D, I, I]),
+            "or_max_threads": (I, []),
+            "or_euler_step": (I, [P, SZ, I, D, D, D]),
+            "or_rk4_step": (I, [P, SZ, I, D, D, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -262,6 +267,18 @@ def simple_transform_rows(state, e, rows):
     return a.view(np.float64).reshape(len(rows), 3)
 
 
+def bh_transform_rows(state, theta, e, rows, dim=3, counts=False):
+    """astro2 / astro accelerations of the bodies `rows` only (the whole tree
+    is built). Returns (acc [len(rows), 3], counts or None, fault)."""
+    state = np.ascontiguousarray(state, ENTITY)
+    rows = np.ascontiguousarray(rows, np.int64)
+    a = np.zeros(len(rows), ACCEL)
+    c = np.zeros(len(rows), np.int64) if counts else None
+    fault = lib().or_bh_transform_rows(dim, _p(state), len(state), theta, e, _p(rows), len(rows),
+                                       _p(a), _p(c))
+    return a.view(np.float64).reshape(len(rows), 3), c, fault
+
+
 def gravity(a, b, e):
     ea = np.ascontiguousarray(np.asarray(a, ENTITY).reshape(1))
     eb = np.ascontiguousarray(np.asarray(b, ENTITY).reshape(1))
@@ -305,6 +322,31 @@ def pipeline_steps_timed(state, theta, e, dt, steps, verlet=None):
     v = verlet or Verlet()
     fault = lib().or_pipeline_steps_timed(v.h, _p(state), len(state), theta, e, dt, steps)
     return v, fault
+
+
+def pipeline_steps_timed_mt(state, theta, e, dt, steps, threads, verlet=None):
+    """The timed astro2 + verlet baseline with the walks and the update split
+    over `threads` host threads (serial insertion build)."""
+    assert state.dtype == ENTITY and state.flags["C_CONTIGUOUS"]
+    v = verlet or Verlet()
+    fault = lib().or_pipeline_steps_timed_mt(v.h, _p(state), len(state), theta, e, dt, steps,
+                                             threads)
+    return v, fault
+
+
+def max_threads():
+    return lib().or_max_threads()
+
+
+def integrate_steps(state, integrator, kind, theta, e, dt, steps):
+    """`steps` pipeline iterations in place with the euler or rk4 integrator
+    (integrators/src/euler.rs, rk4.rs) and one gravity transform."""
+    assert state.dtype == ENTITY and state.flags["C_CONTIGUOUS"]
+    fn = {"euler": lib().or_euler_step, "rk4": lib().or_rk4_step}[integrator]
+    fault = 0
+    for _ in range(steps):
+        fault |= fn(_p(state), len(state), KIND[kind], theta, e, dt)
+    return fault
 
 
 def energy(state, e):

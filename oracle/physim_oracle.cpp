@@ -28,6 +28,10 @@
 #include <cstring>
 #include <vector>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 extern "C" {
 
 // physim-core/src/lib.rs:16-29 (repr(C) Entity, 80 bytes) and :31-37.
@@ -629,6 +633,40 @@ extern "C" void or_simple_transform_rows(const Entity* s, size_t n, double e, co
     }
 }
 
+// The astro2 / astro transform (or_bh_transform's arithmetic) for a subset of
+// target bodies (rows) against the whole tree, so that large-N checks can
+// sample. acc[r] += the acceleration of body rows[r]; counts[r] (optional)
+// receives its interaction-list length. Returns the tree fault flags.
+extern "C" int or_bh_transform_rows(int dim, const Entity* s, size_t n, double theta, double e,
+                                    const int64_t* rows, size_t nrows, Acceleration* acc,
+                                    int64_t* counts) {
+    OrTree* t = or_build_tree(dim, s, n);
+    int fault = t->fault;
+    std::vector<const TNode*> list;
+    for (size_t r = 0; r < nrows; ++r) {
+        if (counts) counts[r] = 0;
+        const Entity& a = s[rows[r]];
+        if (a.fixed) continue;
+        double f[3] = {0.0, 0.0, 0.0};
+        const double loc[3] = {a.x, a.y, a.z};
+        list.clear();
+        tree_walk(t, loc, theta, list);
+        if (counts) counts[r] = (int64_t)list.size();
+        for (const TNode* nb : list) {
+            const Entity& b = nb->e;
+            if (a.x == b.x && a.y == b.y && a.z == b.z) continue;
+            double fij[3];
+            gravity(a, b, e, fij);
+            f[0] += fij[0]; f[1] += fij[1]; f[2] += fij[2];
+        }
+        acc[r].x += f[0] / a.mass;
+        acc[r].y += f[1] / a.mass;
+        acc[r].z += f[2] / a.mass;
+    }
+    or_tree_free(t);
+    return fault;
+}
+
 // ---------------------------------------------------------------------------
 // verlet: integrators/src/verlet.rs:19-108
 // ---------------------------------------------------------------------------
@@ -710,6 +748,167 @@ extern "C" int or_pipeline_steps_timed(OrVerlet* v, Entity* state, size_t n, dou
         fault |= or_bh_transform_timed(3, state, n, theta, e, acc.data());
         or_verlet_integrate(v, state, new_state.data(), n, acc.data(), dt);
         std::copy(new_state.begin(), new_state.end(), state);
+    }
+    return fault;
+}
+
+// The timed baseline on `threads` host threads (OpenMP): the reference's
+// serial insertion build (transformers.rs:132-135, unchanged: its COM folds
+// follow insertion order), then the per-body walks and pair sums of
+// transformers.rs:137-159 split over the threads (each body's own sum keeps
+// the reference's order, so the accelerations equal the serial ones bit for
+// bit), then the Stormer update (verlet.rs:52-82) split the same way.
+extern "C" int or_pipeline_steps_timed_mt(OrVerlet* v, Entity* state, size_t n, double theta,
+                                          double e, double dt, int steps, int threads) {
+    std::vector<Acceleration> acc(n);
+    std::vector<Entity> new_state(state, state + n);
+    int fault = 0;
+    const double dt2 = dt * dt;
+    for (int st = 0; st < steps; ++st) {
+        std::fill(acc.begin(), acc.end(), Acceleration{0.0, 0.0, 0.0});
+        OrTree* t = or_build_tree(3, state, n);
+        fault |= t->fault;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+        for (size_t i = 0; i < n; ++i) {
+            const Entity& a = state[i];
+            if (a.fixed) continue;
+            double f[3] = {0.0, 0.0, 0.0};
+            const double loc[3] = {a.x, a.y, a.z};
+            std::vector<const TNode*> list;
+            list.reserve(100);
+            tree_walk(t, loc, theta, list);
+            std::vector<Entity> bs;
+            bs.reserve(list.size());
+            for (const TNode* nb : list) bs.push_back(nb->e);
+            for (const Entity& b : bs) {
+                if (a.x == b.x && a.y == b.y && a.z == b.z) continue;
+                double fij[3];
+                gravity(a, b, e, fij);
+                f[0] += fij[0]; f[1] += fij[1]; f[2] += fij[2];
+            }
+            acc[i].x += f[0] / a.mass;
+            acc[i].y += f[1] / a.mass;
+            acc[i].z += f[2] / a.mass;
+        }
+        or_tree_free(t);
+        if (v->previous_state.size() != n) {
+            or_verlet_integrate(v, state, new_state.data(), n, acc.data(), dt);
+        } else {
+            const Entity* p = v->previous_state.data();
+#pragma omp parallel for schedule(static) num_threads(threads)
+            for (size_t i = 0; i < n; ++i) {
+                const Entity& en = state[i];
+                const Acceleration& a = acc[i];
+                double x = 2.0 * en.x - p[i].x + a.x * dt2;
+                double y = 2.0 * en.y - p[i].y + a.y * dt2;
+                double z = 2.0 * en.z - p[i].z + a.z * dt2;
+                Entity ne = en;
+                ne.x = x; ne.y = y; ne.z = z;
+                ne.vx = (x - en.x) / dt;
+                ne.vy = (y - en.y) / dt;
+                ne.vz = (z - en.z) / dt;
+                new_state[i] = ne;
+            }
+            v->previous_state.assign(state, state + n);
+        }
+        std::copy(new_state.begin(), new_state.end(), state);
+    }
+    return fault;
+}
+
+extern "C" int or_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+// acc_fn of the pipeline (pipeline.rs:137-141) with one gravity transform:
+// accelerations zeroed by the integrator, then the transform adds into them.
+static int acc_fn(int kind, const Entity* s, size_t n, double theta, double e,
+                  Acceleration* acc) {
+    std::fill(acc, acc + n, Acceleration{0.0, 0.0, 0.0});
+    if (kind == 0) {
+        or_simple_transform(s, n, e, acc);
+        return 0;
+    }
+    return or_bh_transform(kind, s, n, theta, e, acc, nullptr);
+}
+
+// integrators/src/euler.rs:17-49: x' = x + v dt + 0.5 a dt^2 (dt.powi(2) =
+// dt * dt), v' = v + a dt, every body (no case for `fixed`: a fixed body has
+// a = 0 and moves with its velocity); other fields copied.
+extern "C" int or_euler_step(Entity* state, size_t n, int kind, double theta, double e,
+                             double dt) {
+    std::vector<Acceleration> acc(n);
+    int fault = acc_fn(kind, state, n, theta, e, acc.data());
+    const double dt2 = dt * dt;
+    for (size_t i = 0; i < n; ++i) {
+        Entity& en = state[i];
+        const Acceleration& a = acc[i];
+        const double x = en.x + en.vx * dt + 0.5 * a.x * dt2;
+        const double y = en.y + en.vy * dt + 0.5 * a.y * dt2;
+        const double z = en.z + en.vz * dt + 0.5 * a.z * dt2;
+        const double vx = en.vx + a.x * dt;
+        const double vy = en.vy + a.y * dt;
+        const double vz = en.vz + a.z * dt;
+        en.x = x; en.y = y; en.z = z;
+        en.vx = vx; en.vy = vy; en.vz = vz;
+    }
+    return fault;
+}
+
+// integrators/src/rk4.rs:24-182. Four acc_fn calls (:36, :70, :104, :138) on
+// the base state and on base + 0.5 k1, base + 0.5 k2, base + k3 (positions
+// and velocities of every body, fixed ones included, other fields copied);
+// k = (dt * v_stage, dt * a_stage); the result base + (k1 + 2 k2 + 2 k3 + k4)
+// / 6.0 summed left to right, and for a fixed body its position with zero
+// velocity (:160-174).
+extern "C" int or_rk4_step(Entity* state, size_t n, int kind, double theta, double e, double dt) {
+    struct K6 { double x, y, z, vx, vy, vz; };
+    std::vector<Acceleration> f(n);
+    std::vector<Entity> tmp(state, state + n);
+    std::vector<K6> k[4];
+    int fault = 0;
+    for (int s = 0; s < 4; ++s) {
+        const Entity* cur = s == 0 ? state : tmp.data();
+        fault |= acc_fn(kind, cur, n, theta, e, f.data());
+        k[s].resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            const Entity& c = cur[i];
+            k[s][i] = K6{dt * c.vx, dt * c.vy, dt * c.vz, dt * f[i].x, dt * f[i].y, dt * f[i].z};
+        }
+        if (s == 3) break;
+        const double w = s < 2 ? 0.5 : 1.0;
+        for (size_t i = 0; i < n; ++i) {
+            const Entity& b = state[i];
+            const K6& q = k[s][i];
+            Entity t = b;
+            t.x = b.x + w * q.x;
+            t.y = b.y + w * q.y;
+            t.z = b.z + w * q.z;
+            t.vx = b.vx + w * q.vx;
+            t.vy = b.vy + w * q.vy;
+            t.vz = b.vz + w * q.vz;
+            tmp[i] = t;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        Entity& b = state[i];
+        const K6 &a = k[0][i], &c = k[1][i], &d = k[2][i], &g = k[3][i];
+        if (!b.fixed) {
+            const double x = b.x + (a.x + 2.0 * c.x + 2.0 * d.x + g.x) / 6.0;
+            const double y = b.y + (a.y + 2.0 * c.y + 2.0 * d.y + g.y) / 6.0;
+            const double z = b.z + (a.z + 2.0 * c.z + 2.0 * d.z + g.z) / 6.0;
+            const double vx = b.vx + (a.vx + 2.0 * c.vx + 2.0 * d.vx + g.vx) / 6.0;
+            const double vy = b.vy + (a.vy + 2.0 * c.vy + 2.0 * d.vy + g.vy) / 6.0;
+            const double vz = b.vz + (a.vz + 2.0 * c.vz + 2.0 * d.vz + g.vz) / 6.0;
+            b.x = x; b.y = y; b.z = z;
+            b.vx = vx; b.vy = vy; b.vz = vz;
+        } else {
+            b.vx = 0.0; b.vy = 0.0; b.vz = 0.0;
+        }
     }
     return fault;
 }

@@ -2573,19 +2573,24 @@ __device__ __forceinline__ void direct_store(const SegDesc& d, uint64_t i, doubl
         // velocities are stored only by the Taylor first step
         const double dt = d.dt;
         const double dt2 = dt * dt;
+        double nx, ny, nz;
         if (d.st->history == 0) {
             const double ux = d.svx[i], uy = d.svy[i], uz = d.svz[i];
-            d.dx[i] = x + ux * dt + 0.5 * ax * dt2;
-            d.dy[i] = y + uy * dt + 0.5 * ay * dt2;
-            d.dz[i] = z + uz * dt + 0.5 * az * dt2;
+            nx = x + ux * dt + 0.5 * ax * dt2;
+            ny = y + uy * dt + 0.5 * ay * dt2;
+            nz = z + uz * dt + 0.5 * az * dt2;
             d.dvx[i] = ux + ax * dt;
             d.dvy[i] = uy + ay * dt;
             d.dvz[i] = uz + az * dt;
         } else {
-            d.dx[i] = 2.0 * x - d.spx[i] + ax * dt2;
-            d.dy[i] = 2.0 * y - d.spy[i] + ay * dt2;
-            d.dz[i] = 2.0 * z - d.spz[i] + az * dt2;
+            nx = 2.0 * x - d.spx[i] + ax * dt2;
+            ny = 2.0 * y - d.spy[i] + ay * dt2;
+            nz = 2.0 * z - d.spz[i] + az * dt2;
         }
+        d.dx[i] = nx; d.dy[i] = ny; d.dz[i] = nz;
+        // a non-finite new position fails the simulation (as the tree
+        // step's walk reports it)
+        if (!isfinite(nx) || !isfinite(ny) || !isfinite(nz)) atomicOr(&d.st->err, kErrNonFinite);
         d.dpx[i] = x; d.dpy[i] = y; d.dpz[i] = z;
         d.dm[i] = m;
         d.dfix[i] = d.sfix[i];
@@ -2652,13 +2657,15 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
                 const double4 b = sj[k];
                 const float fx = (float)(b.x - x), fy = (float)(b.y - y), fz = (float)(b.z - z);
                 const float s2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-                const float t = s2 + ef;
-                // a body's own entry (and any equal position) has fx = fy =
-                // fz = 0: the floor (the least normal FP32) keeps its weight
-                // finite, so it adds 0; it changes no other weight unless
-                // r^2 (r^2 + e)^2 is below 2^-126
-                const float w = bits_f32((uint32_t)f64_bits(b.w)) *
-                                __builtin_amdgcn_rsqf(fmaxf(s2 * t * t, 1.17549435e-38f));
+                // |F|/m_a = m_b / (r (r^2 + e)) as 1/r and 1/(r^2 + e)
+                // separately (their product r^2 (r^2 + e)^2 leaves FP32's
+                // range near r = 2.6e6 and below r = 5e-7 with e = 0); a
+                // body's own entry (and any equal position, which the
+                // reference skips) is selected out, not weighted
+                const float w = s2 > 0.0f ? bits_f32((uint32_t)f64_bits(b.w)) *
+                                                __builtin_amdgcn_rsqf(s2) *
+                                                __builtin_amdgcn_rcpf(s2 + ef)
+                                          : 0.0f;
                 px = fmaf(fx, w, px);
                 py = fmaf(fy, w, py);
                 pz = fmaf(fz, w, pz);
@@ -2820,3 +2827,4 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 }
 
 }  // namespace pg
+

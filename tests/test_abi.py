@@ -20,8 +20,10 @@ def declared_functions():
     for h in ("physim_abi.h", "physim_gpu.h"):
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        src = re.sub(r"^\s*#.*$", "", src, flags=re.M)
         src = re.sub(r"typedef\s+(struct|enum)[^{;]*\{.*?\}[^;]*;", "", src, flags=re.S)
         src = re.sub(r"typedef[^;]*;", "", src, flags=re.S)
+        src = re.sub(r"\b_?[sS]tatic_assert\s*\(.*?\)\s*;", "", src, flags=re.S)
         for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{]*)\)\s*;", src):
             names.add(m.group(1))
     return sorted(names)

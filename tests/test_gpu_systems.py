@@ -105,6 +105,32 @@ def test_server_isolates_failures_and_waits_for_memory():
         eng.close()
 
 
+def test_server_isolates_nonfinite_direct_sum(engine):
+    """A simple_astro request with a NaN coordinate fails alone with
+    PG_ERR_NONFINITE, as an astro2 one does; the direct-sum requests
+    co-batched with it complete and equal their alone results."""
+    kw = dict(kind="simple_astro", theta=1.0, e=0.01, dt=1e-4)
+    good = [plummer(600 + 200 * k, 60 + k) for k in range(3)]
+    bad = plummer(500, 70)
+    bad["y"][3] = np.nan
+    srv = P.Server(engine)
+    ids = [srv.submit(good[0], 12, **kw), srv.submit(bad, 12, **kw)]
+    ids += [srv.submit(g, 12, **kw) for g in good[1:]]
+    for _ in range(1000):
+        active, queued = srv.tick(5)
+        if active == 0 and queued == 0:
+            break
+    info, ok = srv.info(ids[1])
+    assert ok and info.status == 7
+    with pytest.raises(P.PgError):
+        srv.take(ids[1], len(bad))
+    for g, rid in zip(good, [ids[0]] + ids[2:]):
+        out, info = srv.take(rid, len(g))
+        assert info.status == 0
+        assert out.tobytes() == alone(engine, g, 12, **kw).tobytes()
+    srv.close()
+
+
 # ----------------------------------------------------------------- batching
 def test_plummer_ensembles_batched_equal_alone(engine):
     """Ensembles of log-uniform size stepped together equal each alone."""

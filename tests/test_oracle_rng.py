@@ -3,8 +3,12 @@
 The block function is pinned by RFC 8439 section 2.3.2 / A.1 and by keystreams
 that OpenSSL produced (tests/golden/make_chacha20_fixture.sh). The rand 0.9 /
 rand_core 0.9 layer above it (seed_from_u64 via PCG32, u64 assembly, float
-conversion) has no reference output in this environment: it is "parity
-unpinned", and only its documented structure is checked here.
+conversion) has no reference output in this environment. What the
+mathematics fixes is checked: which lattice each call site's draws lie on
+(2^-52 steps for random_range's [1, 2) construction, 2^-53 for StandardUniform)
+with every mantissa bit in use. seed_from_u64's PCG32 expansion and the order
+in which two keystream words form a u64 remain "parity unpinned": no output
+of the reference's rand 0.9 stack exists here to compare against.
 """
 import os
 
@@ -60,38 +64,59 @@ def test_chacha8_differs_only_by_rounds():
     assert b0[12] == 14
 
 
-def test_rng_stream_is_the_chacha8_keystream():
-    """next_u64 joins two keystream words (low first) of ChaCha8 blocks at
-    consecutive counters under the PCG32-expanded key."""
-    r = O.Rng(1)
-    u = [r.next_u64() for _ in range(40)]
-    # recompute the key the documented way: PCG32 outputs as key words
-    state = 1
-    mul, inc = 6364136223846793005, 11634580027462260723
-    key = []
-    for _ in range(8):
-        state = (state * mul + inc) % 2**64
-        xs = (((state >> 18) ^ state) >> 27) & 0xFFFFFFFF
-        rot = state >> 59
-        key.append(((xs >> rot) | (xs << ((32 - rot) & 31))) & 0xFFFFFFFF)
-    words = np.concatenate([O.chacha_block(key, c, 0, rounds=8) for c in range(5)])
-    expect = [int(words[2 * i]) | (int(words[2 * i + 1]) << 32) for i in range(40)]
-    assert u == expect
+def _lattice(values, lo, step):
+    """(v - lo) / step for every value, and whether all are integers."""
+    k = (np.asarray(values, dtype=np.float64) - lo) / step
+    return k, bool(np.all(k == np.floor(k)))
 
 
-def test_float_draws_structure():
-    r1, r2 = O.Rng(5), O.Rng(5)
-    for _ in range(100):
-        u = r2.next_u64()
-        f = r1.random_f64()
-        assert f == (u >> 11) * 2.0**-53
-    r1, r2 = O.Rng(9), O.Rng(9)
-    for _ in range(100):
-        u = r2.next_u64()
-        f = r1.random_range(-1.0, 1.0)
-        m = (u >> 12) * 2.0**-52
-        assert f == m * 2.0 - 1.0
-        assert -1.0 <= f < 1.0
+def test_range_draws_lie_on_the_2_52_lattice():
+    """rand's random_range(lo..hi) for f64 builds a value in [1, 2) from the
+    top 52 bits of a u64 (its mantissa), subtracts 1 and scales: every draw
+    of random_range(-1.0..1.0) is -1 + k * 2^-51 and every draw of
+    random_range(0.0..1.0) is k * 2^-52 for an integer k, with all 52 bits
+    in use (the lowest lattice bit set in about half of the draws). A draw
+    built as StandardUniform's (u >> 11) * 2^-53 (a different call), or from
+    fewer bits, fails one of these."""
+    r = O.Rng(9)
+    xs = [r.random_range(-1.0, 1.0) for _ in range(4000)]
+    zs = [r.random_range(0.0, 1.0) for _ in range(4000)]
+    for vals, lo, step in ((xs, -1.0, 2.0**-51), (zs, 0.0, 2.0**-52)):
+        k, integral = _lattice(vals, lo, step)
+        assert integral
+        odd = np.mean(np.mod(k, 2) == 1)
+        assert 0.45 < odd < 0.55, odd
+        assert min(vals) >= lo and max(vals) < lo + 2.0**52 * step
+
+
+def test_standard_draws_lie_on_the_2_53_lattice():
+    """StandardUniform f64 (what Plummer's random_iter draws, initialisers.rs:
+    296-299) is (u >> 11) * 2^-53: every draw is k * 2^-53 in [0, 1), with
+    the lowest bit set in about half of them; the [1, 2) construction of
+    random_range (2^-52 steps) would never set it."""
+    r = O.Rng(5)
+    vals = [r.random_f64() for _ in range(4000)]
+    k, integral = _lattice(vals, 0.0, 2.0**-53)
+    assert integral
+    assert 0.45 < np.mean(np.mod(k, 2) == 1) < 0.55
+    assert min(vals) >= 0.0 and max(vals) < 1.0
+
+
+def test_cube_positions_come_from_range_draws():
+    """The cube generator's coordinates (size 1, centre 0) are the three
+    random_range draws of Entity::random (physim-core/src/lib.rs:115-118):
+    x, y on the -1 + k 2^-51 lattice, z on the k 2^-52 lattice, each with
+    its lowest bit in use; so is a shifted, scaled cube's (size 0.5, centre
+    (0.25, -0.5, 2)) once the exact power-of-two scaling is undone."""
+    s = O.gen_cube(3000, seed=1, spin=1000.0)
+    for ax, lo, step in (("x", -1.0, 2.0**-51), ("y", -1.0, 2.0**-51), ("z", 0.0, 2.0**-52)):
+        k, integral = _lattice(s[ax], lo, step)
+        assert integral, ax
+        assert 0.4 < np.mean(np.mod(k, 2) == 1) < 0.6, ax
+    t = O.gen_cube(3000, seed=1, spin=1000.0, size=0.5, centre=(0.25, -0.5, 2.0))
+    # x * 0.5 + 0.25 and y * 0.5 - 0.5 are exact on these lattices; z * 0.5 + 2.0 rounds
+    np.testing.assert_array_equal((t["x"] - 0.25) * 2.0, s["x"])
+    np.testing.assert_array_equal((t["y"] + 0.5) * 2.0, s["y"])
 
 
 def test_cube_generator_layout():
