@@ -28,8 +28,10 @@ SORT_TILE = 2048   # keys per radix-sort tile
 
 
 def algorithmic_bytes(kernel, n, nodes):
-    """Compulsory bytes one launch of `kernel` moves for n bodies and the
-    given tree size: each input read once, each output written once."""
+    """This build's own table: the bytes one launch of `kernel` reads and
+    writes for n bodies and the given tree size, each input read once and
+    each output written once, in the layouts the kernels use (FP64 state,
+    64-byte node records)."""
     tiles = (n + SORT_TILE - 1) // SORT_TILE
     table = {
         "extent": 24.0 * n,                          # read x, y, z
@@ -56,28 +58,89 @@ def algorithmic_bytes(kernel, n, nodes):
     return table.get(kernel, 0.0)
 
 
+# Compulsory HBM bytes of the tree step per body (the method's minimum, FP32
+# tree data and FP64 state): Morton keys (read x 24, write key 8 + index 4),
+# 24 per key-sort pass over (key, index), the gather into sorted order (read
+# 32, write 16), 64 per internal tree node (write 32, read 32), the walk's own
+# position and acceleration (16 + 12) and the Stormer update (read x, x_prev,
+# a: 24 + 24 + 12; write x, v: 24 + 24).
+COMPULSORY = {"keys": 36.0, "sort_pass": 24.0, "gather": 48.0, "internal_node": 64.0,
+              "walk": 28.0, "stormer": 108.0}
+
+
+def compulsory_bytes(kernel, n, internal, sort_passes):
+    """Compulsory bytes of one launch of `kernel` (n bodies, `internal`
+    internal nodes, `sort_passes` passes of the key sort) on the basis of
+    COMPULSORY; None for kernels outside that accounting."""
+    c = COMPULSORY
+    table = {
+        "keys": c["keys"] * n,
+        "bucket_scatter": c["sort_pass"] * n, "bucket_sort": c["sort_pass"] * n,
+        "sort_scatter": c["sort_pass"] * n,
+        "scan_gather": c["gather"] * n,
+        "nodes": c["internal_node"] * internal,
+        "walk": (c["walk"] + c["stormer"]) * n,  # the update is fused into the walk
+    }
+    return table.get(kernel)
+
+
+def compulsory_step_bytes(n, internal, sort_passes):
+    c = COMPULSORY
+    return (c["keys"] + c["sort_pass"] * sort_passes + c["gather"] + c["walk"]
+            + c["stormer"]) * n + c["internal_node"] * internal
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n_cube, theta, e, dt, budget_s):
-    """The oracle's single-thread astro2 + verlet step on the same workload,
-    timed for whole steps until about budget_s seconds."""
+    """The oracle's astro2 + verlet step on the same workload, timed for
+    whole steps until about budget_s seconds: on one thread (the reference's
+    transform is single-threaded), and with the walks and updates over all
+    host cores available to this process (at most 16, this GPU's share of
+    the host)."""
     from oracle import oracle as O
 
-    state = O.cube_and_stars(n_cube, seed=1, spin=1000.0)
-    v = O.Verlet()
-    t0 = time.perf_counter()
-    O.pipeline_steps_timed(state, theta, e, dt, 1, v)  # first (Taylor) step, untimed
-    first = time.perf_counter() - t0
-    steps = max(1, min(30, int(budget_s / max(first, 1e-3))))
-    t0 = time.perf_counter()
-    O.pipeline_steps_timed(state, theta, e, dt, steps, v)
-    dt_s = time.perf_counter() - t0
-    n = len(state)
+    def timed(threads, budget):
+        state = O.cube_and_stars(n_cube, seed=1, spin=1000.0)
+        v = O.Verlet()
+        run = (lambda k: O.pipeline_steps_timed(state, theta, e, dt, k, v)) if threads == 1 else \
+              (lambda k: O.pipeline_steps_timed_mt(state, theta, e, dt, k, threads, v))
+        t0 = time.perf_counter()
+        run(1)  # first (Taylor) step, untimed
+        first = time.perf_counter() - t0
+        steps = max(2, min(30, int(budget / max(first, 1e-3))))
+        t0 = time.perf_counter()
+        run(steps)
+        return len(state), steps, time.perf_counter() - t0
+
+    n, steps, secs = timed(1, budget_s)
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    _, steps_mt, secs_mt = timed(cores, budget_s / 2)
     return {
-        "value": n * steps / dt_s,
+        "value": n * steps / secs,
         "unit": "body-steps/s",
         "cores": 1,
         "kind": "port",
         "sample": f"{steps} timed steps (after 1 untimed) of the same {n}-body workload, "
-                  f"oracle astro2 + verlet, 1 thread, {dt_s:.1f} s",
+                  f"oracle astro2 + verlet (C++ restatement, -O2 -ffp-contract=off), 1 thread, "
+                  f"{secs:.1f} s",
+        "all_cores": {
+            "value": n * steps_mt / secs_mt,
+            "cores": cores,
+            "sample": f"{steps_mt} timed steps after 1 untimed; serial insertion tree build, "
+                      f"walks and updates over {cores} OpenMP threads, {secs_mt:.1f} s",
+        },
+        "cpu_model": _cpu_model(),
+        "host_cpus_visible": os.cpu_count(),
+        "compiler": "g++ " + O.compiler(),
     }
 
 
@@ -381,8 +444,12 @@ def main():
     per_step_ms = {k: v[0] / args.profile_steps for k, v in prof.items()}
     dom = max(per_step_ms, key=per_step_ms.get)
     avg_ms = prof[dom][0] / prof[dom][1]
+    internal = st["nodes"] - n
+    sort_passes = 2 if "bucket_sort" in prof else 8  # sample sort: scatter + bucket sort
+    cbytes = compulsory_bytes(dom, n, internal, sort_passes)
     abytes = algorithmic_bytes(dom, n, st["nodes"])
-    achieved = abytes / (avg_ms * 1e-3) / 1e9
+    achieved = cbytes / (avg_ms * 1e-3) / 1e9
+    step_bytes = compulsory_step_bytes(n, internal, sort_passes)
 
     # HBM bytes per launch of that kernel, measured with rocprofv3 FETCH_SIZE /
     # WRITE_SIZE on this workload (scaled by a copy kernel of known size)
@@ -433,7 +500,18 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "avg_launch_ms": avg_ms,
-            "algorithmic_bytes_per_launch": abytes,
+            "algorithmic_bytes_per_launch": cbytes,
+            "basis": "compulsory bytes (keys 36, 24 per sort pass, gather 48, walk 28 + Stormer "
+                     "108 per body; 64 per internal node)",
+            "own_table": {"algorithmic_bytes_per_launch": abytes,
+                          "achieved": abytes / (avg_ms * 1e-3) / 1e9,
+                          "frac": abytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "basis": "bytes each kernel reads and writes in its own layouts "
+                                   "(FP64 state, 64-byte records for every node)"},
+            "step": {"compulsory_bytes_per_body": step_bytes / n,
+                     "sort_passes": sort_passes, "internal_nodes": internal,
+                     "achieved": step_bytes / (elapsed / args.steps) / 1e9,
+                     "frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS},
         },
         "extra": {
             "interactions_per_step": st["interactions"],

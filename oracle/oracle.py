@@ -67,6 +67,7 @@ def lib():
             "or_tree_serialize": (SZ, [P, P, P, SZ]),
             "or_extent": (D, [P, SZ]),
             "or_build_tree": (P, [I, P, SZ]),
+            "or_build_tree_mt": (P, [I, P, SZ, I]),
             "or_bh_transform": (I, [I, P, SZ, D, D, P, P]),
             "or_bh_transform_timed": (I, [I, P, SZ, D, D, P]),
             "or_simple_transform": (None, [P, SZ, D, P]),
@@ -80,6 +81,7 @@ def lib():
             "or_bh_transform_rows": (I, [I, P, SZ, D, D, P, SZ, P, P]),
             "or_pipeline_steps_timed_mt": (I, [P, P, SZ, D, D, D, I, I]),
             "or_max_threads": (I, []),
+            "or_compiler": (ctypes.c_char_p, []),
             "or_euler_step": (I, [P, SZ, I, D, D, D]),
             "or_rk4_step": (I, [P, SZ, I, D, D, D]),
         }
@@ -195,6 +197,13 @@ class Tree:
     def from_state(cls, state, dim=3):
         state = np.ascontiguousarray(state, ENTITY)
         return cls(dim=dim, handle=lib().or_build_tree(dim, _p(state), len(state)))
+
+    @classmethod
+    def from_state_mt(cls, state, dim=3, threads=4):
+        """The same tree built on `threads` host threads (subtrees of the
+        depth-3 cells in parallel)."""
+        state = np.ascontiguousarray(state, ENTITY)
+        return cls(dim=dim, handle=lib().or_build_tree_mt(dim, _p(state), len(state), threads))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -336,6 +345,10 @@ def pipeline_steps_timed_mt(state, theta, e, dt, steps, threads, verlet=None):
 
 def max_threads():
     return lib().or_max_threads()
+
+
+def compiler():
+    return lib().or_compiler().decode()
 
 
 def integrate_steps(state, integrator, kind, theta, e, dt, steps):

@@ -333,10 +333,10 @@ static inline bool no_children(const OrTree* t, const TNode& n) {
     return true;
 }
 
-// octree.rs:59-128 (OctreeNode::push), iterative form of the tail recursion.
-static void tree_push(OrTree* t, const Entity& item, int64_t body) {
-    int32_t ni = 0;
-    for (int count = 0;; ++count) {
+// octree.rs:59-128 (OctreeNode::push), iterative form of the tail recursion,
+// from node `ni` at recursion depth `count0` (the root and 0 for a push).
+static void tree_push_at(OrTree* t, int32_t ni, int count0, const Entity& item, int64_t body) {
+    for (int count = count0;; ++count) {
         if (count > 64) {  // octree.rs:60-62 "Recursion too deep"
             t->fault |= 1;
             return;
@@ -394,6 +394,8 @@ static void tree_push(OrTree* t, const Entity& item, int64_t body) {
         return;
     }
 }
+
+static void tree_push(OrTree* t, const Entity& item, int64_t body) { tree_push_at(t, 0, 0, item, body); }
 
 extern "C" OrTree* or_tree_new(int dim, const double centre[3], double extent, size_t reserve) {
     OrTree* t = new OrTree();
@@ -525,6 +527,173 @@ extern "C" OrTree* or_build_tree(int dim, const Entity* s, size_t n) {
     g_fault = 0;
     for (size_t i = 0; i < n; ++i) tree_push(t, s[i], (int64_t)i);
     t->fault |= g_fault;
+    return t;
+}
+
+// The same tree as or_build_tree, built on `threads` host threads. Each
+// node's entity is the fold of COM/mass over its bodies in insertion order
+// (octree.rs:81-83 at every push through it; the merge rule of :69-80 only
+// touches leaves), and a node's subtree depends only on its own bodies in
+// insertion order. So the subtrees of the cells at depth kMtDepth are built
+// independently (each from an empty node at that depth, its bodies pushed
+// in order) and the entities of the nodes above are folds over their bodies
+// in order. Where that equivalence could fail -- a node above that depth
+// with fewer than two bodies, or whose first two bodies lie within the
+// merge distance -- the serial build runs instead.
+static bool close_pair(const Entity& a, const Entity& b) {
+    return fabs(a.x - b.x) < 1e-9 && fabs(a.y - b.y) < 1e-9 && fabs(a.z - b.z) < 1e-9;
+}
+
+static Entity fold(const Entity* s, const uint32_t* idx, size_t cnt) {
+    Entity cur = s[idx[0]];
+    for (size_t k = 1; k < cnt; ++k) {
+        double com[3];
+        centre_of_mass(cur, s[idx[k]], com);
+        cur = fake(com, cur.mass + s[idx[k]].mass);
+    }
+    return cur;
+}
+
+extern "C" OrTree* or_build_tree_mt(int dim, const Entity* s, size_t n, int threads) {
+    constexpr int kMtDepth = 3;
+    if (n < 4096 || threads <= 1) return or_build_tree(dim, s, n);
+    const double c0[3] = {0.0, 0.0, 0.0};
+    OrTree* t = or_tree_new(dim, c0, 1.0 * or_extent(s, n), 0);
+    const int bits = dim;  // octant bits per level
+    const uint32_t ncells[kMtDepth + 1] = {1u, 1u << bits, 1u << (2 * bits), 1u << (3 * bits)};
+    // each body's cell path to depth kMtDepth, by the reference's descent
+    std::vector<uint32_t> cell(n);
+    const TNode root = t->arena[0];
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (size_t i = 0; i < n; ++i) {
+        const double p[3] = {s[i].x, s[i].y, s[i].z};
+        TNode nd = root;
+        uint32_t id = 0;
+        for (int d = 0; d < kMtDepth; ++d) {
+            const int o = octant_id(t, nd, p);
+            TNode ch;
+            octant_centre(t, nd, p, ch.c);
+            ch.ext = nd.ext / 2.0;
+            nd = ch;
+            id = (id << bits) | (uint32_t)o;
+        }
+        cell[i] = id;
+    }
+    // the bodies of every node at depth d <= kMtDepth in insertion order: a
+    // stable counting sort by the node's prefix of the cell path
+    std::vector<size_t> start[kMtDepth + 1];
+    std::vector<uint32_t> order[kMtDepth + 1];
+#pragma omp parallel for schedule(static, 1) num_threads(threads)
+    for (int d = 0; d <= kMtDepth; ++d) {
+        const int sh = bits * (kMtDepth - d);
+        start[d].assign(ncells[d] + 1, 0);
+        for (size_t i = 0; i < n; ++i) ++start[d][(cell[i] >> sh) + 1];
+        for (uint32_t c = 0; c < ncells[d]; ++c) start[d][c + 1] += start[d][c];
+        order[d].resize(n);
+        std::vector<size_t> fill(start[d].begin(), start[d].end() - 1);
+        for (size_t i = 0; i < n; ++i) order[d][fill[cell[i] >> sh]++] = (uint32_t)i;
+    }
+    auto count_of = [&](int d, uint32_t q) { return start[d][q + 1] - start[d][q]; };
+    // checks and entities of the nodes above the subtrees; the subtrees
+    std::vector<Entity> ent[kMtDepth];
+    for (int d = 0; d < kMtDepth; ++d) ent[d].assign(ncells[d], Entity{});
+    const uint32_t nleaf = ncells[kMtDepth];
+    std::vector<OrTree*> sub(nleaf, nullptr);
+    int bad = 0;
+    const uint32_t nupper = ncells[0] + ncells[1] + ncells[2];
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) reduction(| : bad)
+    for (uint32_t task = 0; task < nupper + nleaf; ++task) {
+        if (task < nupper) {
+            const int d = task < ncells[0] ? 0 : task < ncells[0] + ncells[1] ? 1 : 2;
+            const uint32_t q = task - (d == 0 ? 0u : d == 1 ? ncells[0] : ncells[0] + ncells[1]);
+            const size_t cnt = count_of(d, q);
+            if (cnt == 0) continue;
+            const uint32_t* ids = order[d].data() + start[d][q];
+            if (cnt < 2 || close_pair(s[ids[0]], s[ids[1]])) {
+                bad |= 1;
+                continue;
+            }
+            ent[d][q] = fold(s, ids, cnt);
+        } else {
+            const uint32_t c = task - nupper;
+            const size_t cnt = count_of(kMtDepth, c);
+            if (cnt == 0) continue;
+            const uint32_t* ids = order[kMtDepth].data() + start[kMtDepth][c];
+            const double p[3] = {s[ids[0]].x, s[ids[0]].y, s[ids[0]].z};
+            TNode nd = root;
+            for (int d = 0; d < kMtDepth; ++d) {
+                TNode ch;
+                octant_centre(t, nd, p, ch.c);
+                ch.ext = nd.ext / 2.0;
+                nd = ch;
+            }
+            OrTree* st = or_tree_new(dim, nd.c, nd.ext, 2 * cnt);
+            for (size_t k = 0; k < cnt; ++k) tree_push_at(st, 0, kMtDepth, s[ids[k]], (int64_t)ids[k]);
+            sub[c] = st;
+        }
+    }
+    if (bad) {
+        for (OrTree* st : sub)
+            if (st) or_tree_free(st);
+        or_tree_free(t);
+        return or_build_tree(dim, s, n);
+    }
+    // one arena: the nodes above depth kMtDepth, then each subtree (links
+    // offset), copied in parallel into place
+    std::vector<int32_t> idx_prev(1, 0), idx_cur;
+    t->arena[0].e = ent[0][0];
+    t->arena[0].has = true;
+    t->arena[0].body = -1;
+    std::vector<size_t> sub_base(nleaf, 0);
+    {
+        size_t total = t->arena.size();
+        for (int d = 1; d < kMtDepth; ++d)
+            for (uint32_t q = 0; q < ncells[d]; ++q) total += count_of(d, q) ? 1 : 0;
+        for (uint32_t c = 0; c < nleaf; ++c) total += sub[c] ? sub[c]->arena.size() : 0;
+        t->arena.reserve(total);
+    }
+    for (int d = 1; d <= kMtDepth; ++d) {
+        idx_cur.assign(ncells[d], -1);
+        for (uint32_t q = 0; q < ncells[d]; ++q) {
+            const uint32_t parent = q >> bits, o = q & ((1u << bits) - 1u);
+            if (idx_prev[parent] < 0 || count_of(d, q) == 0) continue;
+            int32_t k;
+            if (d < kMtDepth) {
+                // the centre by the reference's descent along any body's path
+                const uint32_t b0 = order[d][start[d][q]];
+                const double p[3] = {s[b0].x, s[b0].y, s[b0].z};
+                double cc[3];
+                octant_centre(t, t->arena[idx_prev[parent]], p, cc);
+                k = tree_new_node(t, cc, t->arena[idx_prev[parent]].ext / 2.0);
+                t->arena[k].e = ent[d][q];
+                t->arena[k].has = true;
+                t->arena[k].body = -1;
+            } else {
+                k = (int32_t)t->arena.size();
+                sub_base[q] = (size_t)k;
+                t->arena.insert(t->arena.end(), sub[q]->arena.size(), TNode{});
+                t->merges += sub[q]->merges;
+                t->fault |= sub[q]->fault;
+            }
+            t->arena[idx_prev[parent]].child[o] = k;
+            idx_cur[q] = k;
+        }
+        idx_prev.swap(idx_cur);
+    }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (uint32_t c = 0; c < nleaf; ++c) {
+        OrTree* st = sub[c];
+        if (!st) continue;
+        const int32_t base = (int32_t)sub_base[c];
+        TNode* dst = t->arena.data() + base;
+        for (size_t k = 0; k < st->arena.size(); ++k) {
+            TNode nd = st->arena[k];
+            for (int r = 0; r < 8; ++r)
+                if (nd.child[r] >= 0) nd.child[r] += base;
+            dst[k] = nd;
+        }
+        or_tree_free(st);
+    }
     return t;
 }
 
@@ -752,9 +921,9 @@ extern "C" int or_pipeline_steps_timed(OrVerlet* v, Entity* state, size_t n, dou
     return fault;
 }
 
-// The timed baseline on `threads` host threads (OpenMP): the reference's
-// serial insertion build (transformers.rs:132-135, unchanged: its COM folds
-// follow insertion order), then the per-body walks and pair sums of
+// The timed baseline on `threads` host threads (OpenMP): the insertion
+// tree of transformers.rs:132-135 built in parallel (or_build_tree_mt, the
+// same tree bit for bit), then the per-body walks and pair sums of
 // transformers.rs:137-159 split over the threads (each body's own sum keeps
 // the reference's order, so the accelerations equal the serial ones bit for
 // bit), then the Stormer update (verlet.rs:52-82) split the same way.
@@ -766,7 +935,7 @@ extern "C" int or_pipeline_steps_timed_mt(OrVerlet* v, Entity* state, size_t n, 
     const double dt2 = dt * dt;
     for (int st = 0; st < steps; ++st) {
         std::fill(acc.begin(), acc.end(), Acceleration{0.0, 0.0, 0.0});
-        OrTree* t = or_build_tree(3, state, n);
+        OrTree* t = or_build_tree_mt(3, state, n, threads);
         fault |= t->fault;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
         for (size_t i = 0; i < n; ++i) {
@@ -815,6 +984,9 @@ extern "C" int or_pipeline_steps_timed_mt(OrVerlet* v, Entity* state, size_t n, 
     }
     return fault;
 }
+
+// The compiler that built this file (reported beside the CPU baseline).
+extern "C" const char* or_compiler(void) { return __VERSION__; }
 
 extern "C" int or_max_threads(void) {
 #ifdef _OPENMP

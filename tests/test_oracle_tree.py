@@ -326,3 +326,42 @@ def test_mass_and_com_conservation(dim):
             b = rec[k][2]
             assert fl[k][0] == s["mass"][b]
             assert tuple(fl[k][1:4]) == (s["x"][b], s["y"][b], s["z"][b])
+
+
+def _same_tree(a, b):
+    ra, fa = a.serialize()
+    rb, fb = b.serialize()
+    return (np.array_equal(ra[:, :3], rb[:, :3]) and fa.tobytes() == fb.tobytes()
+            and a.merges == b.merges and a.fault == b.fault)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_parallel_build_equals_insertion_build(dim):
+    """The multi-threaded build (used by the all-cores CPU baseline) gives
+    the insertion tree bit for bit: topology, centres, masses and centres of
+    mass (the same insertion-order folds), merges included; where a node
+    above its split depth starts with a coincident pair it falls back to the
+    serial build."""
+    s = O.cube_and_stars(30000, seed=4)
+    assert _same_tree(O.Tree.from_state(s, dim), O.Tree.from_state_mt(s, dim, threads=4))
+    # coincident bodies deep in the tree (merged by the reference's rule)
+    t = s.copy()
+    t["x"][500], t["y"][500], t["z"][500] = t["x"][900], t["y"][900], t["z"][900] + 1e-12
+    a, b = O.Tree.from_state(t, dim), O.Tree.from_state_mt(t, dim, threads=4)
+    assert a.merges == 1 and _same_tree(a, b)
+    # the first two bodies coincide: the root would be a merged leaf
+    u = s.copy()
+    u["x"][1], u["y"][1], u["z"][1] = u["x"][0], u["y"][0], u["z"][0]
+    assert _same_tree(O.Tree.from_state(u, dim), O.Tree.from_state_mt(u, dim, threads=4))
+    # a Plummer sphere (deep, uneven cells)
+    p = O.gen_plummer_ref(20000, seed=3, mass=1.0, a=0.2)
+    assert _same_tree(O.Tree.from_state(p, dim), O.Tree.from_state_mt(p, dim, threads=4))
+
+
+def test_parallel_baseline_steps_equal_serial():
+    s = O.cube_and_stars(20000, seed=2)
+    a, b = s.copy(), s.copy()
+    O.pipeline_steps_timed(a, 1.5, 0.5, 1e-5, 3)
+    O.pipeline_steps_timed_mt(b, 1.5, 0.5, 1e-5, 3, 4)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "mass", "id"):
+        np.testing.assert_array_equal(a[f], b[f])
