@@ -164,6 +164,16 @@ int pg_batch_step(pg_engine* eng, pg_sim* const* sims, uint32_t count, uint32_t 
  * reusable scratch. */
 int pg_transform_aos(pg_engine* eng, const pg_params* params, const Entity* state, uint64_t n,
                      Acceleration* acc);
+/* pg_transform_aos without device allocation: no counterpart in the
+ * reference. For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): every device buffer
+ * of the call lies in the library's static device array of 48 MiB per GPU,
+ * so n is at most 65,536 bodies (PG_ERR_ARG beyond) and a tree at most 4
+ * nodes per body (PG_ERR_INTERNAL beyond). Same arguments, results and error
+ * behaviour as pg_transform_aos, which it equals bit for bit. Calls on one
+ * GPU share the array and are serialised; it keeps the last call's contents. */
+int pg_transform_aos_noalloc(pg_engine* eng, const pg_params* params, const Entity* state,
+                             uint64_t n, Acceleration* acc);
 
 /* ---- serving (no reference counterpart) -------------------------------- */
 typedef struct pg_server pg_server;

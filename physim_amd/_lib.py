@@ -132,6 +132,7 @@ def lib():
         "pg_sim_morton": (I, [P, P, P, U64]),
         "pg_batch_step": (I, [P, P, U32, U32]),
         "pg_transform_aos": (I, [P, ctypes.POINTER(Params), P, U64, P]),
+        "pg_transform_aos_noalloc": (I, [P, ctypes.POINTER(Params), P, U64, P]),
         "pg_server_create": (I, [P, U32, PP]),
         "pg_server_destroy": (None, [P]),
         "pg_server_submit": (I, [P, P, U64, ctypes.POINTER(Params), U32, ctypes.POINTER(U64)]),
@@ -289,6 +290,17 @@ class Engine:
         a = np.zeros(len(state), ACCEL) if acc is None else acc
         p = params(kind, theta, e, 0.0, precision)
         check(lib().pg_transform_aos(self.h, ctypes.byref(p), _p(state), len(state), _p(a)))
+        return a
+
+    def transform_noalloc(self, state, kind="astro2", theta=1.0, e=1.0, precision="mixed",
+                          acc=None):
+        """transform() through the allocation-free entry point (at most
+        65,536 bodies; its device buffers are the library's static array)."""
+        state = np.ascontiguousarray(state, ENTITY)
+        a = np.zeros(len(state), ACCEL) if acc is None else acc
+        p = params(kind, theta, e, 0.0, precision)
+        check(lib().pg_transform_aos_noalloc(self.h, ctypes.byref(p), _p(state), len(state),
+                                             _p(a)))
         return a
 
     def batch_step(self, sims, steps):

@@ -138,6 +138,15 @@ struct pg_engine {
 
     // transient simulation of the transform path
     pg_sim* xform = nullptr;
+
+    // the scratch is the library's static device array (allocation-free
+    // transform): never grown or freed
+    bool fixed_scratch = false;
+    // per-call host tables of run_batch, kept so that a call whose sizes fit
+    // what earlier calls held allocates nothing on the host either
+    std::vector<pg_sim*> h_sims;
+    std::vector<uint32_t> h_nsplit, h_chunk0, h_nchunk, h_tile0;
+    std::vector<uint64_t> h_part_off;
 };
 
 struct pg_sim {
@@ -225,7 +234,7 @@ static void free_scratch(pg_engine* e) {
     for (auto& g : e->graph_cache)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     e->graph_cache.clear();
-    if (e->scratch) (void)hipFree(e->scratch);
+    if (e->scratch && !e->fixed_scratch) (void)hipFree(e->scratch);
     e->scratch = nullptr;
     e->scratch_bytes = 0;
     e->cap_bodies = e->cap_nodes = 0;
@@ -357,21 +366,11 @@ extern "C" int pg_engine_copy_probe(pg_engine* e, uint64_t bytes, uint32_t reps,
     return PG_OK;
 }
 
-// Grows the batch scratch to hold `bodies` bodies, `nodes` tree nodes, `segs`
-// segments and `blocks` direct-sum blocks. One allocation, carved in place.
-static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_t segs,
-                          uint32_t blocks) {
-    if (bodies <= e->cap_bodies && nodes <= e->cap_nodes && segs <= e->cap_segs &&
-        blocks <= e->cap_blocks && e->scratch)
-        return PG_OK;
-    bodies = std::max<uint64_t>(bodies, e->cap_bodies);
-    nodes = std::max<uint64_t>(nodes, e->cap_nodes);
-    segs = std::max<uint32_t>(segs, e->cap_segs);
-    blocks = std::max<uint32_t>(blocks, e->cap_blocks);
-    bodies = std::max<uint64_t>(bodies, 1024);
-    segs = std::max<uint32_t>(segs, 4);
-    blocks = std::max<uint32_t>(blocks, 16);
-    nodes = std::max<uint64_t>(nodes, 1024);
+// The batch scratch layout for `bodies` bodies, `nodes` tree nodes, `segs`
+// segments and `blocks` direct-sum blocks, carved from one block at `base`
+// (sizes only when base is NULL). Returns the block's size in bytes.
+static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t nodes,
+                              uint32_t segs, uint32_t blocks) {
     const uint64_t sort_tiles = (bodies + kSortTile - 1) / kSortTile;
     // node tiles start at every segment's first body: at most bodies/T + segs
     const uint64_t node_tiles = bodies / kNodeTile + segs + 1;
@@ -407,22 +406,7 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     // a tile boundary is crossed by at most one node per level (levels 0..32)
     const uint64_t top = (node_tiles + 2) * kTopSlots;
     const uint64_t o_tl = take(top * 8), o_tla = take(top * 8);
-    const uint64_t total = off;
-    if (e->sims_bytes + total > e->budget)
-        return pg_set_error(PG_ERR_BUDGET, "scratch of %llu bytes exceeds the HBM budget",
-                            (unsigned long long)total);
-    (void)hipStreamSynchronize(e->stream);
-    free_scratch(e);
-    void* p = nullptr;
-    hipError_t he = hipMalloc(&p, total);
-    if (he != hipSuccess)
-        return pg_set_error(PG_ERR_NOMEM, "hipMalloc(%llu) for scratch: %s",
-                            (unsigned long long)total, hipGetErrorString(he));
-    char* b = (char*)p;
-    e->scratch = p;
-    e->ext_valid = 0;
-    e->sorted_valid = 0;
-    e->scratch_bytes = total;
+    if (!b) return off;
     e->keys[0] = (uint64_t*)(b + o_k0);
     e->keys[1] = (uint64_t*)(b + o_k1);
     e->vals[0] = (uint32_t*)(b + o_v0);
@@ -463,17 +447,63 @@ static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_
     e->cap_chunks = (uint32_t)chunks;
     e->cap_tiles = (uint32_t)node_tiles;
     e->cap_top = top;
-    // Status words are versioned; zero them once so no stale tag can match.
-    HIP_TRY(hipMemsetAsync(e->sc, 0, sizeof(StepCounters), e->stream));
-    HIP_TRY(hipMemsetAsync(e->err_global, 0, 256, e->stream));
-    HIP_TRY(hipMemsetAsync(e->top_slot, 0, top * 8, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
     e->cap_bodies = bodies;
     e->cap_nodes = nodes;
     e->cap_segs = segs;
     e->cap_blocks = blocks;
+    e->ext_valid = 0;
+    e->sorted_valid = 0;
     e->last_sim = nullptr;
+    return off;
+}
+
+// Zeroes the scratch's versioned status words once, so no stale tag can
+// match (the fill happens on the engine's stream).
+static int clear_scratch_status(pg_engine* e) {
+    HIP_TRY(hipMemsetAsync(e->sc, 0, sizeof(StepCounters), e->stream));
+    HIP_TRY(hipMemsetAsync(e->err_global, 0, 256, e->stream));
+    HIP_TRY(hipMemsetAsync(e->top_slot, 0, e->cap_top * 8, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return PG_OK;
+}
+
+// Grows the batch scratch to hold `bodies` bodies, `nodes` tree nodes, `segs`
+// segments and `blocks` direct-sum blocks. One allocation, carved in place.
+// An engine on a fixed scratch (the allocation-free path) never grows: a
+// batch beyond its capacities is an error.
+static int ensure_scratch(pg_engine* e, uint64_t bodies, uint64_t nodes, uint32_t segs,
+                          uint32_t blocks) {
+    if (bodies <= e->cap_bodies && nodes <= e->cap_nodes && segs <= e->cap_segs &&
+        blocks <= e->cap_blocks && e->scratch)
+        return PG_OK;
+    if (e->fixed_scratch)
+        return pg_set_error(PG_ERR_ARG,
+                            "batch exceeds the fixed scratch (%llu bodies, %llu nodes, %u blocks)",
+                            (unsigned long long)e->cap_bodies, (unsigned long long)e->cap_nodes,
+                            e->cap_blocks);
+    bodies = std::max<uint64_t>(bodies, e->cap_bodies);
+    nodes = std::max<uint64_t>(nodes, e->cap_nodes);
+    segs = std::max<uint32_t>(segs, e->cap_segs);
+    blocks = std::max<uint32_t>(blocks, e->cap_blocks);
+    bodies = std::max<uint64_t>(bodies, 1024);
+    segs = std::max<uint32_t>(segs, 4);
+    blocks = std::max<uint32_t>(blocks, 16);
+    nodes = std::max<uint64_t>(nodes, 1024);
+    const uint64_t total = carve_scratch(e, nullptr, bodies, nodes, segs, blocks);
+    if (e->sims_bytes + total > e->budget)
+        return pg_set_error(PG_ERR_BUDGET, "scratch of %llu bytes exceeds the HBM budget",
+                            (unsigned long long)total);
+    (void)hipStreamSynchronize(e->stream);
+    free_scratch(e);
+    void* p = nullptr;
+    hipError_t he = hipMalloc(&p, total);
+    if (he != hipSuccess)
+        return pg_set_error(PG_ERR_NOMEM, "hipMalloc(%llu) for scratch: %s",
+                            (unsigned long long)total, hipGetErrorString(he));
+    e->scratch = p;
+    e->scratch_bytes = total;
+    carve_scratch(e, (char*)p, bodies, nodes, segs, blocks);
+    return clear_scratch_status(e);
 }
 
 // ----------------------------------------------------------- kernel timing
@@ -734,7 +764,8 @@ static int err_text(uint32_t bits) {
 // (integrate) or one force evaluation into e->acc_out (no integrate).
 static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint32_t steps,
                      bool integrate) {
-    std::vector<pg_sim*> sims;
+    std::vector<pg_sim*>& sims = e->h_sims;
+    sims.clear();
     for (uint32_t i = 0; i < count; ++i) {
         pg_sim* s = sims_in[i];
         if (!s) return pg_set_error(PG_ERR_ARG, "sim %u is NULL", i);
@@ -766,8 +797,10 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     if (nodes >= (1ull << 32)) return pg_set_error(PG_ERR_ARG, "tree too large for 32-bit nodes");
     e->h_blk_seg.clear();
     e->h_blk_tile.clear();
-    std::vector<uint32_t> nsplit(bp.nseg, 1u);
-    std::vector<uint64_t> part_off(bp.nseg, 0ull);
+    std::vector<uint32_t>& nsplit = e->h_nsplit;
+    std::vector<uint64_t>& part_off = e->h_part_off;
+    nsplit.assign(bp.nseg, 1u);
+    part_off.assign(bp.nseg, 0ull);
     if (bp.kind == PG_KIND_DIRECT) {
         // Blocks: per segment, target tile and source split (split in the
         // tile word's top 8 bits). A segment's partial sums (3 doubles per
@@ -795,7 +828,10 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     }
     e->h_ext_seg.clear();
     e->h_ext_chunk.clear();
-    std::vector<uint32_t> chunk0(bp.nseg), nchunk(bp.nseg);
+    std::vector<uint32_t>& chunk0 = e->h_chunk0;
+    std::vector<uint32_t>& nchunk = e->h_nchunk;
+    chunk0.assign(bp.nseg, 0u);
+    nchunk.assign(bp.nseg, 0u);
     for (uint32_t s = 0; s < bp.nseg; ++s) {
         const uint32_t nc = (uint32_t)((sims[s]->n + kExtentChunk - 1) / kExtentChunk);
         chunk0[s] = (uint32_t)e->h_ext_seg.size();
@@ -809,7 +845,8 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     // node tiles of kNodeTile bodies from each segment's first body, so that a
     // segment's tree arithmetic does not depend on where the batch puts it
     e->h_tile_start.clear();
-    std::vector<uint32_t> tile0(bp.nseg);
+    std::vector<uint32_t>& tile0 = e->h_tile0;
+    tile0.assign(bp.nseg, 0u);
     {
         uint64_t o = 0;
         for (uint32_t s = 0; s < bp.nseg; ++s) {
@@ -821,6 +858,10 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         bp.ntiles = (uint32_t)e->h_tile_start.size();
         e->h_tile_start.push_back((uint32_t)o);
     }
+    // A fixed scratch holds fewer nodes than the worst-case bound; the node
+    // passes check every node index against the capacity and report a tree
+    // that would exceed it (PG_ERR_INTERNAL) instead of writing past it.
+    if (e->fixed_scratch && bp.kind != PG_KIND_DIRECT) nodes = std::min(nodes, e->cap_nodes);
     PG_TRY(ensure_scratch(e, bp.total, nodes, bp.nseg, bp.nblocks));
     if (bp.nchunks > e->cap_chunks)
         return pg_set_error(PG_ERR_INTERNAL, "extent chunk table overflow");
@@ -997,6 +1038,28 @@ static int validate_params(const pg_params* p) {
     return PG_OK;
 }
 
+// A simulation's device arrays for n bodies, carved from one block of
+// state_bytes(n) at b.
+static void carve_state(pg_sim* s, char* b, uint64_t n) {
+    uint64_t off = 0;
+    auto take = [&](uint64_t sz) {
+        char* q = b + off;
+        off += align256(sz);
+        return q;
+    };
+    for (int k = 0; k < 2; ++k) {
+        s->x[k] = (double*)take(n * 8); s->y[k] = (double*)take(n * 8); s->z[k] = (double*)take(n * 8);
+        s->px[k] = (double*)take(n * 8); s->py[k] = (double*)take(n * 8); s->pz[k] = (double*)take(n * 8);
+        s->vx[k] = (double*)take(n * 8); s->vy[k] = (double*)take(n * 8); s->vz[k] = (double*)take(n * 8);
+        s->m[k] = (double*)take(n * 8);
+        s->fix[k] = (uint8_t*)take(n);
+        s->id[k] = (uint32_t*)take(n * 4);
+    }
+    s->aos = (Entity*)take(n * 80);
+    s->counts = (int64_t*)take(n * 8);
+    s->st = (SegState*)take(sizeof(SegState));
+}
+
 extern "C" int pg_sim_create(pg_engine* e, uint64_t n, const pg_params* p, pg_sim** out) {
     if (!e || !out) return pg_set_error(PG_ERR_ARG, "engine or out is NULL");
     PG_TRY(validate_params(p));
@@ -1018,24 +1081,7 @@ extern "C" int pg_sim_create(pg_engine* e, uint64_t n, const pg_params* p, pg_si
                             hipGetErrorString(he));
     }
     s->bytes = bytes;
-    char* b = (char*)s->block;
-    uint64_t off = 0;
-    auto take = [&](uint64_t sz) {
-        char* q = b + off;
-        off += align256(sz);
-        return q;
-    };
-    for (int k = 0; k < 2; ++k) {
-        s->x[k] = (double*)take(n * 8); s->y[k] = (double*)take(n * 8); s->z[k] = (double*)take(n * 8);
-        s->px[k] = (double*)take(n * 8); s->py[k] = (double*)take(n * 8); s->pz[k] = (double*)take(n * 8);
-        s->vx[k] = (double*)take(n * 8); s->vy[k] = (double*)take(n * 8); s->vz[k] = (double*)take(n * 8);
-        s->m[k] = (double*)take(n * 8);
-        s->fix[k] = (uint8_t*)take(n);
-        s->id[k] = (uint32_t*)take(n * 4);
-    }
-    s->aos = (Entity*)take(n * 80);
-    s->counts = (int64_t*)take(n * 8);
-    s->st = (SegState*)take(sizeof(SegState));
+    carve_state(s, (char*)s->block, n);
     HIP_TRY(hipMemset(s->st, 0, sizeof(SegState)));
     e->sims_bytes += bytes;
     e->sims.push_back(s);
@@ -1358,5 +1404,123 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
             fl[8 * k + 6] = c[2];
         }
     }
+    return PG_OK;
+}
+
+// ------------------------------------------- allocation-free transform
+// pg_transform_aos for callers that cannot allocate device memory. Every
+// device buffer of the call -- the staged Entity AoS, the SoA state, keys,
+// sort buffers, tree nodes, per-step tables, accelerations and the staged
+// Acceleration AoS -- lies in one static device array of kNoAllocBytes,
+// `scratch`, carved for at most kNoAllocMaxBodies bodies and
+// kNoAllocNodesPerBody tree nodes per body. Each GPU holds its own copy of
+// the array (until the process ends), found once per device with
+// hipGetSymbolAddress; host code uses only that address. Calls on one GPU
+// share the array and are serialised by a per-device lock (calls that
+// overlap on one GPU would race on it); the array keeps the last call's
+// contents, and every call rewrites or clears what it reads.
+constexpr uint64_t kNoAllocMaxBodies = 65536;
+constexpr uint64_t kNoAllocNodesPerBody = 4;
+constexpr uint64_t kNoAllocBytes = 48ull << 20;
+constexpr int kNoAllocDevices = 16;
+static_assert(sizeof(SegState) <= 256, "state_bytes reserves 256 bytes for SegState");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+struct NoAllocCtx {
+    pg_engine eng;        // engine view whose scratch is `scratch`
+    pg_sim sim;           // the transient simulation, carved from it too
+    char* base = nullptr; // this device's address of `scratch`
+    double* stage = nullptr;  // staged Acceleration AoS (n * 3 doubles)
+    std::mutex mu;
+    bool ready = false;
+};
+NoAllocCtx g_noalloc[kNoAllocDevices];
+}  // namespace
+
+// Once per device, with that device current: the array's address, the
+// carve-up (state, staging, then the batch scratch) and the cleared status
+// words; host tables reserved at their largest, so no later call allocates.
+static int noalloc_setup(NoAllocCtx& c, int device, hipStream_t stream) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    const uint64_t n = kNoAllocMaxBodies;
+    const uint64_t state_sz = align256(state_bytes(n));
+    const uint64_t stage_sz = align256(n * 24);
+    const uint64_t nodes = n * kNoAllocNodesPerBody;
+    const uint32_t blocks = (uint32_t)((n + kBlock - 1) / kBlock) * kMaxDirectSplit;
+    pg_engine& e = c.eng;
+    const uint64_t need = state_sz + stage_sz + carve_scratch(&e, nullptr, n, nodes, 1, blocks);
+    if (need > kNoAllocBytes)
+        return pg_set_error(PG_ERR_INTERNAL, "allocation-free layout needs %llu of %llu bytes",
+                            (unsigned long long)need, (unsigned long long)kNoAllocBytes);
+    c.base = (char*)ptr;
+    c.stage = (double*)(c.base + state_sz);
+    e.device = device;
+    e.stream = stream;
+    e.fixed_scratch = true;
+    e.graphs = false;
+    e.budget = kNoAllocBytes;
+    e.scratch = c.base;
+    e.scratch_bytes = kNoAllocBytes;
+    carve_scratch(&e, c.base + state_sz + stage_sz, n, nodes, 1, blocks);
+    PG_TRY(clear_scratch_status(&e));
+    const uint64_t chunks = n / kExtentChunk + 2, tiles = n / kNodeTile + 2;
+    e.h_sims.reserve(1);
+    e.h_nsplit.reserve(1);
+    e.h_part_off.reserve(1);
+    e.h_chunk0.reserve(1);
+    e.h_nchunk.reserve(1);
+    e.h_tile0.reserve(1);
+    e.h_desc[0].reserve(1);
+    e.h_desc[1].reserve(1);
+    e.h_seg_off.reserve(2);
+    e.h_blk_seg.reserve(blocks);
+    e.h_blk_tile.reserve(blocks);
+    e.h_ext_seg.reserve(chunks);
+    e.h_ext_chunk.reserve(chunks);
+    e.h_tile_start.reserve(tiles);
+    c.ready = true;
+    return PG_OK;
+}
+
+extern "C" int pg_transform_aos_noalloc(pg_engine* eng, const pg_params* p, const Entity* state,
+                                        uint64_t n, Acceleration* acc) {
+    if (!eng) return pg_set_error(PG_ERR_ARG, "engine is NULL");
+    PG_TRY(validate_params(p));
+    if (n == 0) return PG_OK;
+    if (!state || !acc) return pg_set_error(PG_ERR_ARG, "state or acc is NULL");
+    if (n > kNoAllocMaxBodies)
+        return pg_set_error(PG_ERR_ARG,
+                            "n = %llu exceeds the allocation-free transform's maximum of %llu bodies",
+                            (unsigned long long)n, (unsigned long long)kNoAllocMaxBodies);
+    if (eng->device < 0 || eng->device >= kNoAllocDevices)
+        return pg_set_error(PG_ERR_ARG, "device %d beyond the allocation-free path's %d devices",
+                            eng->device, kNoAllocDevices);
+    std::lock_guard<std::recursive_mutex> lk(eng->mu);
+    HIP_TRY(hipSetDevice(eng->device));
+    NoAllocCtx& c = g_noalloc[eng->device];
+    std::lock_guard<std::mutex> lc(c.mu);
+    c.eng.stream = eng->stream;
+    if (!c.ready) PG_TRY(noalloc_setup(c, eng->device, eng->stream));
+    pg_sim* s = &c.sim;
+    s->eng = &c.eng;
+    s->n = n;
+    s->p = *p;
+    s->counting = false;
+    s->uploaded = false;
+    carve_state(s, c.base, n);
+    PG_TRY(pg_sim_upload(s, state, n));
+    PG_TRY(run_batch(&c.eng, &s, 1, 1, false));
+    // acc[i] += the device's accelerations, added on the device in FP64 (the
+    // same sums as pg_transform_aos forms on the host)
+    const uint64_t words = n * 3;
+    HIP_TRY(hipMemcpyAsync(c.stage, acc, words * 8, hipMemcpyHostToDevice, eng->stream));
+    k_add_f64<<<grid_for(words, 1u << 30), kBlock, 0, eng->stream>>>(c.stage, c.eng.acc_out,
+                                                                      words);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(acc, c.stage, words * 8, hipMemcpyDeviceToHost, eng->stream));
+    HIP_TRY(hipStreamSynchronize(eng->stream));
     return PG_OK;
 }

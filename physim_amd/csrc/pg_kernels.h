@@ -2826,5 +2826,12 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+// dst[k] += src[k], FP64.
+__global__ void k_add_f64(double* __restrict__ dst, const double* __restrict__ src, uint64_t n) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (uint64_t)gridDim.x * blockDim.x)
+        dst[k] += src[k];
+}
+
 }  // namespace pg
 
