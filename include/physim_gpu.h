@@ -116,11 +116,31 @@ int pg_sim_set_params(pg_sim* sim, const pg_params* params);
 /* Host AoS -> device SoA. Clears the Stormer history, so the next step is
  * the Taylor first step (verlet.rs:102-103). */
 int pg_sim_upload(pg_sim* sim, const Entity* state, uint64_t n);
+/* Host AoS -> device SoA with the Stormer history of the reference's
+ * verlet: `previous` (may be NULL: as pg_sim_upload) is the state the
+ * integrator was given on its previous call, whose positions the next step
+ * takes as x_prev (verlet.rs:52-82,104-107: the history survives an edited
+ * state of the same length). */
+int pg_sim_upload_state(pg_sim* sim, const Entity* state, const Entity* previous, uint64_t n);
 /* Device -> host AoS in upload order; radius, id and fixed are those
  * uploaded, as verlet copies them (verlet.rs:41,72). */
 int pg_sim_download(pg_sim* sim, Entity* out, uint64_t n);
 /* Advance `steps` pipeline iterations on the device. */
 int pg_sim_step(pg_sim* sim, uint32_t steps);
+
+typedef enum pg_integrator {
+    PG_INTEG_VERLET = 0, /* Stormer verlet (integrators/src/verlet.rs:24-108), the default */
+    PG_INTEG_EULER = 1,  /* Euler::integrate (integrators/src/euler.rs:17-49) */
+    PG_INTEG_RK4 = 2,    /* Rk4::integrate (integrators/src/rk4.rs:24-182): four
+                            acceleration passes per step */
+} pg_integrator;
+/* The integrator pg_sim_step applies: the reference's `verlet`, `euler` or
+ * `rk4` element (IntegratorElement::integrate, physim-core/src/plugin/
+ * integrator.rs:5-13) with acc_fn = the sim's transform. A change starts
+ * the new integrator fresh from the current positions and velocities (a new
+ * verlet begins with its Taylor step). pg_batch_step and the server take
+ * verlet simulations. */
+int pg_sim_set_integrator(pg_sim* sim, int32_t integrator);
 /* Accelerations of the current device state (no integration), FP64, n*3
  * values in upload order. */
 int pg_sim_accel(pg_sim* sim, double* acc_xyz);
@@ -201,6 +221,54 @@ int pg_server_take(pg_server* srv, uint64_t request_id, Entity* out, uint64_t n,
                    pg_request_info* info);
 int pg_server_info(pg_server* srv, uint64_t request_id, pg_request_info* info);
 double pg_server_clock(pg_server* srv);
+
+/* ---- the verlet element on the device ---------------------------------- */
+/* The C side of a `verlet` integrator element that steps on the device:
+ * Verlet::integrate (integrators/src/verlet.rs:85-108) behind the
+ * IntegratorElement interface (physim-core/src/plugin/integrator.rs:5-13),
+ * with acc_fn = the bound GPU gravity element's transform. A Rust shim keeps
+ * the element name `verlet`; the GPU element announces itself on the bus
+ * (topic "physim_gpu", message "transform", sender_id = the element), and
+ * the shim binds it. Each integrate call steps the device state once and
+ * fills new_state (every field). The device state is reused while the input
+ * equals, field for field, what the last call returned; an input changed by
+ * a synth or transmute is uploaded with the last call's input as the Stormer
+ * history (a Taylor restart only when the length changes, as the reference).
+ * Without a bound element integrate returns PG_ERR_STATE (the shim then runs
+ * the stock verlet with the pipeline's acc_fn). */
+typedef struct pg_verlet pg_verlet;
+int pg_verlet_create(pg_verlet** out);
+void pg_verlet_destroy(pg_verlet* v);
+/* element: a transform element object of this library (its init's return). */
+int pg_verlet_bind(pg_verlet* v, const void* element);
+int pg_verlet_integrate(pg_verlet* v, const Entity* entities, Entity* new_state, uint64_t n,
+                        double dt);
+
+/* ---- device-resident pipeline runner ---------------------------------- */
+/* Pipeline::run's simulation loop (physim-core/src/pipeline.rs:134-192) for
+ * a pipeline of one GPU gravity transform (params) and an integrator
+ * (pg_integrator), the state resident on the device, with csvsink renderers
+ * (utilities/src/csvsink.rs:50-79: the first state and every print_n-th,
+ * "x,y,z," per entity in Rust's shortest f64 Display, one line per frame).
+ * A state crosses PCIe only when a sink prints it; the renderer formats on
+ * its own thread behind a two-frame queue (pipeline.rs:127's
+ * sync_channel(2)). Synths and transmutes are not run on the device. */
+typedef struct pg_runner pg_runner;
+int pg_runner_create(pg_engine* eng, const pg_params* params, int32_t integrator,
+                     const Entity* initial, uint64_t n, pg_runner** out);
+/* csvsink properties file (default "csvsink.csv", truncated now) and print_n
+ * (default 1 when 0). Sinks are added before the first run. */
+int pg_runner_add_csvsink(pg_runner* r, const char* file, uint32_t print_n);
+/* `iterations` more pipeline iterations (the initial state is handed to the
+ * sinks on the first call). */
+int pg_runner_run(pg_runner* r, uint32_t iterations);
+/* Waits until every handed-over frame is written. */
+int pg_runner_flush(pg_runner* r);
+int pg_runner_state(pg_runner* r, Entity* out, uint64_t n);
+void pg_runner_destroy(pg_runner* r);
+/* Rust's `{}` (Display) of an f64 into out (NUL-terminated, cap bytes);
+ * returns its length. csvsink's number format. */
+size_t pg_format_f64(double v, char* out, size_t cap);
 
 /* ---- initial conditions ------------------------------------------------ */
 /* RandomCube::create_entities (astro/src/initialisers.rs:81-107). */

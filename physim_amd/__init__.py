@@ -16,6 +16,8 @@ from ._lib import (  # noqa: F401
     PRECISION,
     PgError,
     Engine,
+    Runner,
+    format_f64,
     Server,
     Sim,
     gen_cube,

@@ -123,6 +123,19 @@ def lib():
         "pg_sim_upload": (I, [P, P, U64]),
         "pg_sim_download": (I, [P, P, U64]),
         "pg_sim_step": (I, [P, U32]),
+        "pg_sim_upload_state": (I, [P, P, P, U64]),
+        "pg_verlet_create": (I, [PP]),
+        "pg_verlet_destroy": (None, [P]),
+        "pg_verlet_bind": (I, [P, P]),
+        "pg_verlet_integrate": (I, [P, P, P, U64, D]),
+        "pg_runner_create": (I, [P, ctypes.POINTER(Params), ctypes.c_int32, P, U64, PP]),
+        "pg_runner_add_csvsink": (I, [P, ctypes.c_char_p, U32]),
+        "pg_runner_run": (I, [P, U32]),
+        "pg_runner_flush": (I, [P]),
+        "pg_runner_state": (I, [P, P, U64]),
+        "pg_runner_destroy": (None, [P]),
+        "pg_format_f64": (SZ, [D, ctypes.c_char_p, SZ]),
+        "pg_sim_set_integrator": (I, [P, ctypes.c_int32]),
         "pg_sim_accel": (I, [P, P]),
         "pg_sim_energy": (I, [P, P]),
         "pg_sim_stats": (I, [P, ctypes.POINTER(Stats)]),
@@ -308,9 +321,12 @@ class Engine:
         check(lib().pg_batch_step(self.h, arr, len(sims), steps))
 
 
+INTEGRATOR = {"verlet": 0, "euler": 1, "rk4": 2}
+
+
 class Sim:
     def __init__(self, engine, state=None, n=None, kind="astro2", theta=1.0, e=1.0, dt=1e-6,
-                 precision="mixed"):
+                 precision="mixed", integrator="verlet"):
         self.engine = engine
         self.n = len(state) if state is not None else int(n)
         self.params = params(kind, theta, e, dt, precision)
@@ -318,8 +334,14 @@ class Sim:
         check(lib().pg_sim_create(engine.h, self.n, ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
         engine._children.add(self)
+        if integrator != "verlet":
+            self.set_integrator(integrator)
         if state is not None:
             self.upload(state)
+
+    def set_integrator(self, name):
+        """verlet (default), euler or rk4: the reference's integrator elements."""
+        check(lib().pg_sim_set_integrator(self.h, INTEGRATOR[name]))
 
     def close(self):
         if getattr(self, "h", None):
@@ -442,3 +464,47 @@ def copy_probe(engine, nbytes, reps=20):
     g = ctypes.c_double()
     check(lib().pg_engine_copy_probe(engine.h, nbytes, reps, ctypes.byref(g)))
     return g.value
+
+
+def format_f64(v):
+    """Rust's `{}` of an f64 (csvsink's number format), from the library."""
+    buf = ctypes.create_string_buffer(400)
+    lib().pg_format_f64(float(v), buf, 400)
+    return buf.value.decode()
+
+
+class Runner:
+    """A device-resident pipeline (one GPU gravity transform + integrator)
+    with csvsink renderers."""
+
+    def __init__(self, engine, state, kind="astro2", theta=1.0, e=1.0, dt=1e-6,
+                 precision="mixed", integrator="verlet"):
+        state = np.ascontiguousarray(state, ENTITY)
+        self.n = len(state)
+        p = params(kind, theta, e, dt, precision)
+        h = ctypes.c_void_p()
+        check(lib().pg_runner_create(engine.h, ctypes.byref(p), INTEGRATOR[integrator],
+                                     _p(state), self.n, ctypes.byref(h)))
+        self.h = h
+
+    def add_csvsink(self, file, print_n=1):
+        check(lib().pg_runner_add_csvsink(self.h, str(file).encode(), print_n))
+
+    def run(self, iterations):
+        check(lib().pg_runner_run(self.h, iterations))
+
+    def flush(self):
+        check(lib().pg_runner_flush(self.h))
+
+    def state(self):
+        out = np.zeros(self.n, ENTITY)
+        check(lib().pg_runner_state(self.h, _p(out), self.n))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pg_runner_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "libphysim_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-SOURCES = ["engine.hip", "plugin.cpp", "generators.cpp", "server.cpp"]
+SOURCES = ["engine.hip", "plugin.cpp", "generators.cpp", "server.cpp", "integrator.cpp", "runner.cpp"]
 
 # Host-only sources are compiled by g++ with the oracle's flags, so host FP64
 # (initial conditions) rounds like the reference's baseline x86-64 build.
@@ -53,11 +53,20 @@ def build(verbose=False):
         subprocess.check_call(cmd)
         objs.append(obj)
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs + ["-ldl"]
+    cmd = [HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs + ["-ldl", "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, OUT)
+    # the C harness of the pipeline loop (a test program over the C-ABI)
+    harness = os.path.join(ROOT, "tests", "harness")
+    cmd = [CXX.replace("g++", "gcc") if CXX.endswith("g++") else "gcc", "-O2", "-std=c11",
+           "-Wall", "-I" + INCLUDE, os.path.join(harness, "pipeline_harness.c"),
+           "-L" + HERE, "-lphysim_gpu", "-Wl,-rpath,$ORIGIN/../../physim_amd",
+           "-o", os.path.join(harness, "pipeline_harness")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
     return OUT
 
 

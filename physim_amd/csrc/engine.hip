@@ -165,6 +165,9 @@ struct pg_sim {
     int cur = 0;
     bool uploaded = false;
     double dt_last = 0.0;  // dt of the last integrating step
+    int32_t integ = PG_INTEG_VERLET;
+    double* stage = nullptr;   // euler / rk4: base state and rk4 sums (12 doubles per body)
+    uint64_t stage_steps = 0;  // steps taken by the euler / rk4 stage loop
 };
 
 // ---------------------------------------------------------------- sizing
@@ -1110,6 +1113,7 @@ extern "C" void pg_sim_destroy(pg_sim* s) {
     if (e->last_sim == s) e->last_sim = nullptr;
     if (e->xform == s) e->xform = nullptr;
     e->sims.erase(std::remove(e->sims.begin(), e->sims.end(), s), e->sims.end());
+    if (s->stage) (void)hipFree(s->stage);
     (void)hipFree(s->block);
     e->sims_bytes -= s->bytes;
     delete s;
@@ -1145,12 +1149,37 @@ extern "C" int pg_sim_upload(pg_sim* s, const Entity* state, uint64_t n) {
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemsetAsync(s->st, 0, sizeof(SegState), e->stream));
+    s->stage_steps = 0;
     // new positions: no extent partials are current
     if (e->sc) HIP_TRY(hipMemsetAsync(&e->sc->ext_tag, 0, sizeof(uint64_t), e->stream));
     e->ext_valid = 0;
     HIP_TRY(hipStreamSynchronize(e->stream));
     s->uploaded = true;
     if (e->last_sim == s) e->last_sim = nullptr;
+    return PG_OK;
+}
+
+extern "C" int pg_sim_upload_state(pg_sim* s, const Entity* state, const Entity* previous,
+                                   uint64_t n) {
+    PG_TRY(pg_sim_upload(s, state, n));
+    if (!previous || !n) return PG_OK;
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    // the previous input is staged in the other state buffer (its ten FP64
+    // arrays hold at least n * 80 contiguous bytes; upload made buffer 0
+    // current), its positions become the history of buffer 0
+    Entity* stage = (Entity*)s->x[1];
+    HIP_TRY(hipMemcpyAsync(stage, previous, n * sizeof(Entity), hipMemcpyHostToDevice, e->stream));
+    k_unpack_prev<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(stage, n, s->px[0], s->py[0],
+                                                                   s->pz[0]);
+    HIP_TRY(hipGetLastError());
+    // one step taken: the next is a Stormer step (verlet.rs:104-106)
+    SegState st;
+    memset(&st, 0, sizeof st);
+    st.steps_done = 1;
+    HIP_TRY(hipMemcpyAsync(s->st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return PG_OK;
 }
 
@@ -1162,7 +1191,7 @@ static int velocity_source(pg_sim* s, VelocitySource* vs) {
     const int c = s->cur;
     vs->vx = s->vx[c]; vs->vy = s->vy[c]; vs->vz = s->vz[c];
     vs->px = s->px[c]; vs->py = s->py[c]; vs->pz = s->pz[c];
-    vs->derive = st.steps_done >= 2 ? 1 : 0;
+    vs->derive = s->integ == PG_INTEG_VERLET && st.steps_done >= 2 ? 1 : 0;
     vs->dt = s->dt_last;
     return PG_OK;
 }
@@ -1187,18 +1216,94 @@ extern "C" int pg_sim_download(pg_sim* s, Entity* out, uint64_t n) {
     return PG_OK;
 }
 
+static StageState stage_state(pg_sim* s) {
+    const int c = s->cur;
+    return StageState{s->x[c], s->y[c], s->z[c], s->vx[c], s->vy[c], s->vz[c],
+                      s->fix[c], s->id[c], s->n};
+}
+
+// euler.rs:17-49 / rk4.rs:24-182 on the device: each acceleration pass is
+// the force-only tree or direct pass over the current (stage) state, as the
+// reference's acc_fn runs the transform on it; the stage kernels form the
+// updates.
+static int step_stages(pg_engine* e, pg_sim* s, uint32_t steps) {
+    if (!s->uploaded) return pg_set_error(PG_ERR_STATE, "no state uploaded");
+    if (!s->n) return PG_OK;
+    const unsigned g = grid_for(s->n, 1u << 30);
+    double* base = s->stage;
+    double* ksum = s->stage + 6 * s->n;
+    for (uint32_t k = 0; k < steps; ++k) {
+        if (s->integ == PG_INTEG_EULER) {
+            PG_TRY(run_batch(e, &s, 1, 1, false));
+            k_euler_update<<<g, kBlock, 0, e->stream>>>(stage_state(s), e->acc_out, s->p.dt);
+            HIP_TRY(hipGetLastError());
+        } else {
+            k_rk4_begin<<<g, kBlock, 0, e->stream>>>(stage_state(s), base);
+            HIP_TRY(hipGetLastError());
+            for (int st = 0; st < 4; ++st) {
+                PG_TRY(run_batch(e, &s, 1, 1, false));
+                k_rk4_stage<<<g, kBlock, 0, e->stream>>>(stage_state(s), e->acc_out, base, ksum,
+                                                         s->p.dt, st);
+                HIP_TRY(hipGetLastError());
+            }
+        }
+        s->stage_steps += 1;
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->ext_valid = 0;  // the stage kernels moved the bodies
+    return PG_OK;
+}
+
 extern "C" int pg_sim_step(pg_sim* s, uint32_t steps) {
     if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
     pg_engine* e = s->eng;
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     HIP_TRY(hipSetDevice(e->device));
+    if (s->integ != PG_INTEG_VERLET) return step_stages(e, s, steps);
     return run_batch(e, &s, 1, steps, true);
+}
+
+extern "C" int pg_sim_set_integrator(pg_sim* s, int32_t integrator) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    if (integrator != PG_INTEG_VERLET && integrator != PG_INTEG_EULER &&
+        integrator != PG_INTEG_RK4)
+        return pg_set_error(PG_ERR_ARG, "unknown integrator %d", integrator);
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    if (integrator == s->integ) return PG_OK;
+    if (integrator != PG_INTEG_VERLET && !s->stage && s->n) {
+        hipError_t he = hipMalloc(&s->stage, s->n * 12 * sizeof(double));
+        if (he != hipSuccess) {
+            s->stage = nullptr;
+            return pg_set_error(PG_ERR_NOMEM, "hipMalloc for the integrator stages: %s",
+                                hipGetErrorString(he));
+        }
+    }
+    if (s->uploaded && s->n) {
+        // the current velocities into the stored arrays (the Stormer ones are
+        // derived), then a fresh start: a new verlet element begins with its
+        // Taylor step (verlet.rs:102-103)
+        VelocitySource vs;
+        PG_TRY(velocity_source(s, &vs));
+        if (vs.derive)
+            k_store_velocity<<<grid_for(s->n, 1u << 30), kBlock, 0, e->stream>>>(
+                stage_state(s), vs.px, vs.py, vs.pz, vs.dt);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemsetAsync(s->st, 0, sizeof(SegState), e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    s->integ = integrator;
+    return PG_OK;
 }
 
 extern "C" int pg_batch_step(pg_engine* e, pg_sim* const* sims, uint32_t count, uint32_t steps) {
     if (!e || (count && !sims)) return pg_set_error(PG_ERR_ARG, "engine or sims is NULL");
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     HIP_TRY(hipSetDevice(e->device));
+    for (uint32_t i = 0; i < count; ++i)
+        if (sims[i] && sims[i]->integ != PG_INTEG_VERLET)
+            return pg_set_error(PG_ERR_ARG, "batched steps take verlet simulations (sim %u)", i);
     return run_batch(e, sims, count, steps, true);
 }
 
@@ -1291,7 +1396,7 @@ extern "C" int pg_sim_stats(pg_sim* s, pg_stats* out) {
     SegState st;
     HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
     memset(out, 0, sizeof *out);
-    out->steps_done = st.steps_done;
+    out->steps_done = st.steps_done + s->stage_steps;
     out->interactions = st.interactions;
     out->max_level = st.max_level;
     out->error = map_err(st.err);

@@ -2804,6 +2804,15 @@ __global__ void k_unpack(const Entity* __restrict__ aos, uint64_t n, double* x, 
     id[i] = (uint32_t)i;
 }
 
+// The previous positions (Stormer history) of an uploaded state.
+__global__ void k_unpack_prev(const Entity* __restrict__ aos, uint64_t n, double* px, double* py,
+                              double* pz) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Entity e = aos[i];
+    px[i] = e.x; py[i] = e.y; pz[i] = e.z;
+}
+
 __global__ void k_pack(Entity* __restrict__ aos, uint64_t n, const double* x, const double* y,
                        const double* z, VelocitySource v, const uint32_t* id) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2824,6 +2833,95 @@ __global__ __launch_bounds__(kBlock) void k_copy(const uint4* __restrict__ src,
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------ euler / rk4 stages
+// The euler and rk4 integrators (integrators/src/euler.rs:17-49,
+// rk4.rs:24-182) run their acceleration passes through the force-only
+// tree or direct pass, which leaves the state in some body order (ids give
+// each body's upload index) and the accelerations in upload order. These
+// kernels work in the state's current order and keep the base state and the
+// rk4 sums in upload order.
+struct StageState {
+    double *x, *y, *z, *vx, *vy, *vz;
+    const uint8_t* fix;
+    const uint32_t* id;
+    uint64_t n;
+};
+
+// euler.rs:26-45: x' = x + v dt + 0.5 a dt^2 (dt.powi(2) = dt * dt),
+// v' = v + a dt, every body (fixed ones have a = 0 and keep moving).
+__global__ void k_euler_update(StageState st, const double* __restrict__ acc, double dt) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= st.n) return;
+    const double* a = acc + 3 * (uint64_t)st.id[j];
+    const double dt2 = dt * dt;
+    const double x = st.x[j], y = st.y[j], z = st.z[j];
+    const double vx = st.vx[j], vy = st.vy[j], vz = st.vz[j];
+    st.x[j] = x + vx * dt + 0.5 * a[0] * dt2;
+    st.y[j] = y + vy * dt + 0.5 * a[1] * dt2;
+    st.z[j] = z + vz * dt + 0.5 * a[2] * dt2;
+    st.vx[j] = vx + a[0] * dt;
+    st.vy[j] = vy + a[1] * dt;
+    st.vz[j] = vz + a[2] * dt;
+}
+
+// The base state (x, v) of an rk4 step, saved in upload order.
+__global__ void k_rk4_begin(StageState st, double* __restrict__ base) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= st.n) return;
+    double* b = base + 6 * (uint64_t)st.id[j];
+    b[0] = st.x[j]; b[1] = st.y[j]; b[2] = st.z[j];
+    b[3] = st.vx[j]; b[4] = st.vy[j]; b[5] = st.vz[j];
+}
+
+// Stage s (0..3) of rk4 after its acceleration pass on the stage state:
+// k_s = (dt * v_stage, dt * a_stage) (rk4.rs:40-51 and alike), the running
+// sum k1 + 2.0 k2 + 2.0 k3 + k4 in the reference's left-to-right order
+// (:162-167); then the next stage state base + w k_s (w = 0.5, 0.5, 1.0;
+// :55-66, :89-100, :123-134, every body, fixed ones included), or after k4
+// the result base + sum / 6.0, a fixed body keeping its position with zero
+// velocity (:168-174).
+__global__ void k_rk4_stage(StageState st, const double* __restrict__ acc,
+                            const double* __restrict__ base, double* __restrict__ ksum, double dt,
+                            int stage) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= st.n) return;
+    const uint64_t u = st.id[j];
+    const double* a = acc + 3 * u;
+    const double* b = base + 6 * u;
+    double* ks = ksum + 6 * u;
+    const double k[6] = {dt * st.vx[j], dt * st.vy[j], dt * st.vz[j],
+                         dt * a[0], dt * a[1], dt * a[2]};
+    double sum[6];
+    for (int c = 0; c < 6; ++c) {
+        if (stage == 0) sum[c] = k[c];
+        else if (stage == 3) sum[c] = ks[c] + k[c];
+        else sum[c] = ks[c] + 2.0 * k[c];
+        ks[c] = sum[c];
+    }
+    double* out[6] = {st.x, st.y, st.z, st.vx, st.vy, st.vz};
+    if (stage < 3) {
+        const double w = stage < 2 ? 0.5 : 1.0;
+        for (int c = 0; c < 6; ++c) out[c][j] = b[c] + w * k[c];
+    } else if (st.fix[j] == 0) {
+        for (int c = 0; c < 6; ++c) out[c][j] = b[c] + sum[c] / 6.0;
+    } else {
+        for (int c = 0; c < 3; ++c) out[c][j] = b[c];
+        for (int c = 3; c < 6; ++c) out[c][j] = 0.0;
+    }
+}
+
+// v = (x - x_prev) / dt (the Stormer velocity, verlet.rs:68-70) written into
+// the stored velocity arrays, for a change of integrator.
+__global__ void k_store_velocity(StageState st, const double* __restrict__ px,
+                                 const double* __restrict__ py, const double* __restrict__ pz,
+                                 double dt) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= st.n) return;
+    st.vx[j] = (st.x[j] - px[j]) / dt;
+    st.vy[j] = (st.y[j] - py[j]) / dt;
+    st.vz[j] = (st.z[j] - pz[j]) / dt;
 }
 
 // dst[k] += src[k], FP64.

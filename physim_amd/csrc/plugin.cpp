@@ -13,6 +13,9 @@
 #include <mutex>
 #include <string>
 
+#include <set>
+
+#include "pg_engine.h"
 #include "physim_abi.h"
 #include "physim_gpu.h"
 
@@ -199,6 +202,7 @@ struct Element {
 std::mutex g_mu;
 pg_engine* g_engine = nullptr;
 void* g_bus = nullptr;
+std::set<const void*> g_elements;  // live gravity elements (for the verlet binding)
 
 pg_engine* engine() {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -227,6 +231,8 @@ void* element_init(int kind, const char* name, const uint8_t* cfg, uintptr_t len
     el->params.e = pr.has_e ? std::fabs(pr.e) : 1.0;
     el->params.dt = 0.0;
     el->params.precision = pr.precision == "fp64" ? PG_PREC_FP64 : PG_PREC_MIXED;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_elements.insert(el);
     return el;
 }
 
@@ -251,7 +257,13 @@ void element_transform(const void* obj, const Entity* state, uintptr_t n, Accele
     }
 }
 
-void element_destroy(void* obj) { delete (Element*)obj; }
+void element_destroy(void* obj) {
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_elements.erase(obj);
+    }
+    delete (Element*)obj;
+}
 
 char* describe(void* obj, RustStringAllocFn alloc, bool with_theta) {
     if (!obj || !alloc) return nullptr;
@@ -274,7 +286,11 @@ typedef void (*PostBusFn)(void*, CMessage);
 
 // transformers.rs:198-203: every gravity element announces itself to the
 // energy sink. post_bus_callback is exported by the host (physim.h:81); a
-// host without it only loses that announcement.
+// host without it only loses the announcements. A GPU element also tells a
+// device-resident integrator (the verlet shim, which receives bus messages
+// as every element does: physim-core/src/messages.rs) that its transform
+// can run on the device: topic "physim_gpu", message "transform", the
+// element as sender_id, which the shim hands to pg_verlet_bind.
 void element_post_config(void* obj) {
     if (!obj || !g_bus) return;
     PostBusFn post = (PostBusFn)dlsym(RTLD_DEFAULT, "post_bus_callback");
@@ -285,6 +301,9 @@ void element_post_config(void* obj) {
     m.message = "gravity";
     m.sender_id = (uintptr_t)obj;
     m.origin = 1;
+    post(g_bus, m);
+    m.topic = "physim_gpu";
+    m.message = "transform";
     post(g_bus, m);
 }
 
@@ -315,6 +334,15 @@ PG_ELEMENT(astro, PG_KIND_QUADTREE, true, "")
 PG_ELEMENT(simple_astro, PG_KIND_DIRECT, false, "")
 
 }  // namespace
+
+pg_engine* pg_plugin_engine(void) { return engine(); }
+
+bool pg_plugin_element_params(const void* element, pg_params* out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!element || !g_elements.count(element)) return false;
+    *out = ((const Element*)element)->params;
+    return true;
+}
 
 extern "C" {
 

@@ -147,3 +147,57 @@ def test_pipeline_matches_manual_steps():
         v.integrate(b, new, acc, 1e-5)
         b = new.copy()
     assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------ euler / rk4
+def _two_body_orbit():
+    """Two unit masses 1 apart, unsoftened (e = 0): each accelerates by 1
+    toward the other, so a speed of sqrt(0.5) about their centre keeps the
+    circular orbit of period pi / sqrt(0.5)."""
+    v = np.sqrt(0.5)
+    return O.concat([O.gen_star(x=0.5, vy=v, mass=1.0), O.gen_star(x=-0.5, vy=-v, mass=1.0)])
+
+
+def _orbit_error(integrator, steps):
+    s = _two_body_orbit()
+    s0 = s.copy()
+    dt = np.pi / np.sqrt(0.5) / steps
+    assert O.integrate_steps(s, integrator, "simple_astro", 1.0, 0.0, dt, steps) == 0
+    return max(abs(s["x"] - s0["x"]).max(), abs(s["y"] - s0["y"]).max())
+
+
+def test_rk4_is_fourth_order_and_euler_first_order():
+    """Global error after one period on the closed orbit: halving dt divides
+    rk4's by ~16 and euler's by ~2. A wrong stage weight or a stage taken at
+    the wrong state lowers rk4's order and fails the ratio."""
+    r_rk4 = _orbit_error("rk4", 200) / _orbit_error("rk4", 400)
+    r_eul = _orbit_error("euler", 4000) / _orbit_error("euler", 8000)
+    assert 14.0 < r_rk4 < 18.0, r_rk4
+    assert 1.8 < r_eul < 2.2, r_eul
+    assert _orbit_error("rk4", 400) < 1e-6
+
+
+def test_euler_step_is_verlets_taylor_step():
+    """euler.rs:26-45 and verlet.rs:24-48 are the same formulas: one step of
+    each from the same state agrees bit for bit (x and v)."""
+    s = O.cube_and_stars(500, seed=3)
+    a, b = s.copy(), s.copy()
+    O.integrate_steps(a, "euler", "astro2", 1.0, 0.5, 1e-4, 1)
+    O.pipeline_steps(b, "astro2", 1.0, 0.5, 1e-4, 1)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(a[f], b[f])
+
+
+def test_rk4_fixed_bodies_stay_with_zero_velocity():
+    """rk4.rs:168-174: a fixed body keeps its position and gets v = 0, while
+    its stage positions (which move with its velocity) still pull the
+    others; euler has no such case (the fixed body drifts with v)."""
+    s = O.cube_and_stars(300, seed=5)
+    s["fixed"][-1] = 1
+    s["vx"][-1] = 3.0
+    r = s.copy()
+    O.integrate_steps(r, "rk4", "astro2", 1.0, 0.5, 1e-3, 2)
+    assert r["x"][-1] == s["x"][-1] and r["y"][-1] == s["y"][-1] and r["vx"][-1] == 0.0
+    e = s.copy()
+    O.integrate_steps(e, "euler", "astro2", 1.0, 0.5, 1e-3, 1)
+    assert e["x"][-1] == s["x"][-1] + 3.0 * 1e-3 and e["vx"][-1] == 3.0
