@@ -254,6 +254,9 @@ def ensembles_workload(P, args):
     eng.batch_step(sims, 2)
     eng.synchronize()
     print(f"ensembles: {len(ns)} simulations, {total} bodies uploaded", file=sys.stderr, flush=True)
+    default = eng.tuning()
+    best, tried = eng.autotune(sims, trial_steps=5)
+    print(f"ensembles: autotuner chose {best}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     eng.batch_step(sims, args.ensemble_steps)
     eng.synchronize()
@@ -277,6 +280,8 @@ def ensembles_workload(P, args):
         "batched_body_steps_per_s": total * args.ensemble_steps / batched,
         "alone_body_steps_per_s": total * args.ensemble_steps / solo_time,
         "alone_sample": f"each ensemble alone for {sample} steps after 2, scaled to {args.ensemble_steps}",
+        "tuning": {"default": default, "chosen": best,
+                   "trials_ms_per_step": [[t, ms] for t, ms in tried]},
     }
     out["batched_over_alone"] = out["batched_body_steps_per_s"] / out["alone_body_steps_per_s"]
     for s in sims:
@@ -287,9 +292,11 @@ def ensembles_workload(P, args):
 
 def precision_workload(P, args):
     """Two colliding virial Plummer spheres (mass ratio 10:1, a 0.1, centres
-    1.0 apart along x, closing speed 0.5, seeds 7 and 8), theta 0.5, e 1e-3,
-    dt 1e-5: relative energy (force-consistent softened potential) and
-    momentum drift of mixed precision and of FP64 mode over the same steps."""
+    1.0 apart along x, closing speed 0.5, seeds 7 and 8), theta 0.5 with the
+    direct-sum near field inside args.near_field leaf widths, e 1e-3, dt
+    1e-5: relative energy (force-consistent softened potential, FP64 O(N^2)
+    on the device) and momentum drift of mixed precision and of FP64 mode
+    over the same steps."""
     import numpy as np
 
     n = args.precision_bodies
@@ -305,11 +312,15 @@ def precision_workload(P, args):
     res = {}
     for prec in ("mixed", "fp64"):
         sim = P.Sim(eng, state, kind="astro2", theta=0.5, e=1e-3, dt=1e-5, precision=prec)
+        sim.set_near_field(args.near_field)
+        t0 = time.perf_counter()
         E0 = sim.energy()
+        energy_s = time.perf_counter() - t0
+        print(f"precision {prec}: initial energy in {energy_s:.1f} s", file=sys.stderr, flush=True)
         pscale = float(np.sum(state["mass"] * np.sqrt(state["vx"] ** 2 + state["vy"] ** 2
                                                       + state["vz"] ** 2)))
         drift, elapsed = [], 0.0
-        chunk = max(1, args.precision_steps // 4)
+        chunk = max(1, args.precision_steps // args.precision_checkpoints)
         done = 0
         while done < args.precision_steps:
             k = min(chunk, args.precision_steps - done)
@@ -333,7 +344,9 @@ def precision_workload(P, args):
         "n_gpus": 1,
         "config": {"workload": f"two virial Plummer spheres, {n1} + {n2} bodies (mass 1.0 : 0.1, "
                                "a 0.1), centres 1.0 apart on x, closing speed 0.5, seeds 7/8; "
-                               "astro2 theta 0.5 e 1e-3; verlet dt 1e-5",
+                               f"astro2 theta 0.5, near field {args.near_field} leaf widths, "
+                               "e 1e-3; verlet dt 1e-5",
+                   "near_field_leaf_widths": args.near_field,
                    "bodies": n, "steps": args.precision_steps},
         "mixed": res["mixed"], "fp64": res["fp64"],
         "energy_drift_difference": abs(res["mixed"]["energy_drift"] - res["fp64"]["energy_drift"]),
@@ -391,8 +404,11 @@ def main():
     ap.add_argument("--tick-steps", type=int, default=50)
     ap.add_argument("--ensembles", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=500)
-    ap.add_argument("--precision-bodies", type=int, default=1 << 18)
+    ap.add_argument("--precision-bodies", type=int, default=1 << 22)
     ap.add_argument("--precision-steps", type=int, default=1000)
+    ap.add_argument("--precision-checkpoints", type=int, default=2)
+    ap.add_argument("--near-field", type=float, default=2.0,
+                    help="precision workload: near field in leaf widths (0: the reference's walk)")
     args = ap.parse_args()
 
     if args.gpus != 1:

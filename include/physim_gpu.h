@@ -94,6 +94,26 @@ int pg_engine_set_graphs(pg_engine* eng, int enable);
  * (<= 4096) and in HBM otherwise. Both give the same order. Defaults 1024,
  * 4096. A tuning and testing control; no reference counterpart. */
 int pg_engine_set_sort(pg_engine* eng, uint32_t max_buckets, uint32_t bucket_cap);
+/* Launch choices that change the time of a step but not its results (the
+ * sort's order, the tree and every sum are the same under all of them). No
+ * reference counterpart. */
+typedef struct pg_tuning {
+    uint32_t max_buckets;   /* sample sort up to this many buckets (<= 1024; 0: radix sort) */
+    uint32_t bucket_keys;   /* mean keys per sample-sort bucket (256..4096; default 1664) */
+    uint32_t walk_run_log2; /* the walk's blocks go to each XCD in runs of 2^this (default 4) */
+    uint32_t reserved;
+} pg_tuning;
+int pg_engine_set_tuning(pg_engine* eng, const pg_tuning* tuning);
+int pg_engine_get_tuning(pg_engine* eng, pg_tuning* tuning);
+/* The autotuner: runs the batch of `sims` (verlet) under each candidate
+ * tuning from the same saved state -- four warm-up steps, then trial_steps
+ * timed steps -- keeps the fastest in the engine and restores the sims'
+ * state (their results are unchanged). *best receives the choice; tried[k]
+ * and ms_per_step[k] (up to cap entries; may be NULL) each candidate and its
+ * time; *ntried the number of candidates. */
+int pg_engine_autotune(pg_engine* eng, pg_sim* const* sims, uint32_t count, uint32_t trial_steps,
+                       pg_tuning* best, pg_tuning* tried, double* ms_per_step, uint32_t cap,
+                       uint32_t* ntried);
 /* Per-kernel HIP-event timing of every launch (eager mode, no graphs). */
 int pg_engine_set_profiling(pg_engine* eng, int enable);
 /* Accumulated per-kernel times. names: '\n'-separated kernel names
@@ -159,6 +179,13 @@ typedef struct pg_stats {
     uint32_t reserved;
 } pg_stats;
 int pg_sim_stats(pg_sim* sim, pg_stats* out);
+/* Direct-sum near field (no reference counterpart; 0, the default, is the
+ * reference's walk): a tree cell is opened rather than taken whole when the
+ * body lies within leaf_widths widths of its own leaf cell of the cell's box,
+ * so the bodies that close contribute their pair forces directly. A
+ * deviation from astro2's interaction lists, for the colliding-spheres
+ * precision study (2 leaf widths). */
+int pg_sim_set_near_field(pg_sim* sim, double leaf_widths);
 /* Count interactions per body in the force pass (debug; off by default). */
 int pg_sim_set_counting(pg_sim* sim, int enable);
 /* Per-body interaction-list length of the last force pass (upload order). */

@@ -79,6 +79,7 @@ def lib():
             "or_pipeline_steps_timed": (I, [P, P, SZ, D, D, D, I]),
             "or_energy": (None, [P, SZ, D, P]),
             "or_bh_transform_rows": (I, [I, P, SZ, D, D, P, SZ, P, P]),
+            "or_bh_transform_near": (I, [I, P, SZ, D, D, D, P, SZ, P, P]),
             "or_pipeline_steps_timed_mt": (I, [P, P, SZ, D, D, D, I, I]),
             "or_max_threads": (I, []),
             "or_compiler": (ctypes.c_char_p, []),
@@ -286,6 +287,20 @@ def bh_transform_rows(state, theta, e, rows, dim=3, counts=False):
     fault = lib().or_bh_transform_rows(dim, _p(state), len(state), theta, e, _p(rows), len(rows),
                                        _p(a), _p(c))
     return a.view(np.float64).reshape(len(rows), 3), c, fault
+
+
+def bh_transform_near(state, theta, e, near, rows=None, dim=3, counts=False):
+    """astro2 / astro with the build's near field (`near` widths of each
+    body's own leaf cell); rows None: every body. Returns (acc, counts or
+    None, fault)."""
+    state = np.ascontiguousarray(state, ENTITY)
+    m = len(state) if rows is None else len(rows)
+    r = None if rows is None else np.ascontiguousarray(rows, np.int64)
+    a = np.zeros(m, ACCEL)
+    c = np.zeros(m, np.int64) if counts else None
+    fault = lib().or_bh_transform_near(dim, _p(state), len(state), theta, e, near, _p(r), m,
+                                       _p(a), _p(c))
+    return a.view(np.float64).reshape(m, 3), c, fault
 
 
 def gravity(a, b, e):

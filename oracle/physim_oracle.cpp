@@ -423,8 +423,12 @@ extern "C" size_t or_tree_node_count(const OrTree* t) { return t->arena[0].has ?
 // octree.rs:130-158 (get_leaves_with_resolution). Appends the interaction
 // list (positions and masses of the entities pushed, in the reference's
 // order) and optionally the arena index of each contributing node.
+//
+// margin > 0 adds the build's near field (no reference counterpart): a cell
+// is not taken whole while loc lies within `margin` of its box (half-width
+// ext + margin on every axis the tree splits); it is opened instead.
 static void tree_walk(const OrTree* t, const double loc[3], double theta,
-                      std::vector<const TNode*>& out) {
+                      std::vector<const TNode*>& out, double margin = 0.0) {
     std::vector<const TNode*> stack;
     stack.reserve(100);
     stack.push_back(&t->arena[0]);
@@ -434,7 +438,12 @@ static void tree_walk(const OrTree* t, const double loc[3], double theta,
         if (n->has) {
             double d0 = loc[0] - n->c[0], d1 = loc[1] - n->c[1], d2 = loc[2] - n->c[2];
             double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-            if (n->ext / r < theta) {
+            bool take = n->ext / r < theta;
+            if (take && margin > 0.0) {
+                const double reach = n->ext + margin;
+                take = !(fabs(d0) < reach && fabs(d1) < reach && (t->dim == 2 || fabs(d2) < reach));
+            }
+            if (take) {
                 out.push_back(n);
                 continue;
             }
@@ -820,6 +829,51 @@ extern "C" int or_bh_transform_rows(int dim, const Entity* s, size_t n, double t
         const double loc[3] = {a.x, a.y, a.z};
         list.clear();
         tree_walk(t, loc, theta, list);
+        if (counts) counts[r] = (int64_t)list.size();
+        for (const TNode* nb : list) {
+            const Entity& b = nb->e;
+            if (a.x == b.x && a.y == b.y && a.z == b.z) continue;
+            double fij[3];
+            gravity(a, b, e, fij);
+            f[0] += fij[0]; f[1] += fij[1]; f[2] += fij[2];
+        }
+        acc[r].x += f[0] / a.mass;
+        acc[r].y += f[1] / a.mass;
+        acc[r].z += f[2] / a.mass;
+    }
+    or_tree_free(t);
+    return fault;
+}
+
+// Half-width of the leaf cell that holds loc (the descent of octree.rs:
+// 160-165 from the root until a node without children).
+static double leaf_ext(const OrTree* t, const double loc[3]) {
+    const TNode* n = &t->arena[0];
+    for (;;) {
+        if (no_children(t, *n)) return n->ext;
+        const int32_t c = n->child[octant_id(t, *n, loc)];
+        if (c < 0) return n->ext;
+        n = &t->arena[c];
+    }
+}
+
+// or_bh_transform_rows with the near field: body a's margin is `near` widths
+// of its own leaf cell (2 * its half-width). rows = NULL: every body.
+extern "C" int or_bh_transform_near(int dim, const Entity* s, size_t n, double theta, double e,
+                                    double near, const int64_t* rows, size_t nrows,
+                                    Acceleration* acc, int64_t* counts) {
+    OrTree* t = or_build_tree(dim, s, n);
+    int fault = t->fault;
+    std::vector<const TNode*> list;
+    const size_t m = rows ? nrows : n;
+    for (size_t r = 0; r < m; ++r) {
+        if (counts) counts[r] = 0;
+        const Entity& a = s[rows ? rows[r] : r];
+        if (a.fixed) continue;
+        double f[3] = {0.0, 0.0, 0.0};
+        const double loc[3] = {a.x, a.y, a.z};
+        list.clear();
+        tree_walk(t, loc, theta, list, near * (2.0 * leaf_ext(t, loc)));
         if (counts) counts[r] = (int64_t)list.size();
         for (const TNode* nb : list) {
             const Entity& b = nb->e;

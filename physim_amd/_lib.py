@@ -65,6 +65,14 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("max_buckets", ctypes.c_uint32), ("bucket_keys", ctypes.c_uint32),
+                ("walk_run_log2", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_[:3]}
+
+
 class RequestInfo(ctypes.Structure):
     _fields_ = [
         ("id", ctypes.c_uint64),
@@ -116,6 +124,10 @@ def lib():
         "pg_engine_profile_read": (I, [P, ctypes.c_char_p, SZ, P, P, U32, ctypes.POINTER(U32)]),
         "pg_engine_profile_reset": (I, [P]),
         "pg_engine_synchronize": (I, [P]),
+        "pg_engine_set_tuning": (I, [P, ctypes.POINTER(Tuning)]),
+        "pg_engine_get_tuning": (I, [P, ctypes.POINTER(Tuning)]),
+        "pg_engine_autotune": (I, [P, P, U32, U32, ctypes.POINTER(Tuning), P, P, U32,
+                                   ctypes.POINTER(U32)]),
         "pg_engine_copy_probe": (I, [P, U64, U32, ctypes.POINTER(D)]),
         "pg_sim_create": (I, [P, U64, ctypes.POINTER(Params), PP]),
         "pg_sim_destroy": (None, [P]),
@@ -136,6 +148,7 @@ def lib():
         "pg_runner_destroy": (None, [P]),
         "pg_format_f64": (SZ, [D, ctypes.c_char_p, SZ]),
         "pg_sim_set_integrator": (I, [P, ctypes.c_int32]),
+        "pg_sim_set_near_field": (I, [P, D]),
         "pg_sim_accel": (I, [P, P]),
         "pg_sim_energy": (I, [P, P]),
         "pg_sim_stats": (I, [P, ctypes.POINTER(Stats)]),
@@ -316,6 +329,28 @@ class Engine:
                                              _p(a)))
         return a
 
+    def tuning(self):
+        t = Tuning()
+        check(lib().pg_engine_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, max_buckets=1024, bucket_keys=1664, walk_run_log2=4):
+        t = Tuning(max_buckets, bucket_keys, walk_run_log2, 0)
+        check(lib().pg_engine_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, sims, trial_steps=4):
+        """Times each candidate tuning on the batch (state saved and restored),
+        keeps the fastest; returns (best, [(tuning, ms_per_step), ...])."""
+        arr = (ctypes.c_void_p * len(sims))(*[s.h for s in sims])
+        best, cap = Tuning(), 64
+        tried = (Tuning * cap)()
+        ms = np.zeros(cap)
+        k = ctypes.c_uint32()
+        check(lib().pg_engine_autotune(self.h, arr, len(sims), trial_steps, ctypes.byref(best),
+                                       ctypes.cast(tried, ctypes.c_void_p), _p(ms), cap,
+                                       ctypes.byref(k)))
+        return best.as_dict(), [(tried[i].as_dict(), float(ms[i])) for i in range(k.value)]
+
     def batch_step(self, sims, steps):
         arr = (ctypes.c_void_p * len(sims))(*[s.h for s in sims])
         check(lib().pg_batch_step(self.h, arr, len(sims), steps))
@@ -338,6 +373,11 @@ class Sim:
             self.set_integrator(integrator)
         if state is not None:
             self.upload(state)
+
+    def set_near_field(self, leaf_widths):
+        """Direct-sum near field: cells within `leaf_widths` widths of each
+        body's own leaf cell are opened (0: the reference's walk)."""
+        check(lib().pg_sim_set_near_field(self.h, float(leaf_widths)))
 
     def set_integrator(self, name):
         """verlet (default), euler or rk4: the reference's integrator elements."""

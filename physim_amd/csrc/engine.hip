@@ -74,6 +74,7 @@ struct pg_engine {
     uint32_t* vals[2] = {nullptr, nullptr};
     uint32_t* first_node = nullptr;
     uint8_t* lo_level = nullptr;
+    uint8_t* leaf_lv = nullptr;    // leaf level of each sorted body (near-field walks)
     uint32_t* sort_cnt = nullptr;  // per-tile digit counts, digit-major
     uint32_t* sort_tot = nullptr;  // per-digit totals of one pass
     uint16_t* bkt = nullptr;       // sample sort: bucket of each key
@@ -83,6 +84,8 @@ struct pg_engine {
     uint32_t* bfill = nullptr;     // sample sort: reserved keys per bucket
     uint32_t* bbase = nullptr;     // sample sort: first key of each bucket
     uint32_t max_buckets = kMaxBuckets;  // larger batches use the radix sort
+    uint32_t bucket_keys = kBucketKeys;  // keys per sample-sort bucket (mean)
+    uint32_t walk_run_log2 = 4;          // walk blocks per XCD run: 2^walk_run_log2
     uint64_t ext_valid = 0;  // batch whose extent partials the last step left (0: none)
     uint64_t sorted_valid = 0;  // batch whose sample-sorted keys keys[1] holds (0: none)
     uint32_t bucket_cap = kBucketCap;    // larger buckets are sorted in HBM
@@ -166,6 +169,7 @@ struct pg_sim {
     bool uploaded = false;
     double dt_last = 0.0;  // dt of the last integrating step
     int32_t integ = PG_INTEG_VERLET;
+    double near = 0.0;  // near field in leaf widths (0: the reference's walk)
     double* stage = nullptr;   // euler / rk4: base state and rk4 sums (12 doubles per body)
     uint64_t stage_steps = 0;  // steps taken by the euler / rk4 stage loop
 };
@@ -294,6 +298,42 @@ extern "C" int pg_engine_set_sort(pg_engine* e, uint32_t max_buckets, uint32_t b
     return PG_OK;
 }
 
+// Drops captured steps and the sort/extent tags after a change that alters
+// the launch sequence (kernel arguments are fixed at capture).
+static void invalidate_launches(pg_engine* e) {
+    (void)hipStreamSynchronize(e->stream);
+    for (auto& g : e->graph_cache)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    e->graph_cache.clear();
+    e->sorted_valid = 0;
+    e->ext_valid = 0;
+}
+
+extern "C" int pg_engine_set_tuning(pg_engine* e, const pg_tuning* t) {
+    if (!e || !t) return pg_set_error(PG_ERR_ARG, "engine or tuning is NULL");
+    if (t->max_buckets > kMaxBuckets || t->bucket_keys < 256 || t->bucket_keys > kBucketCap ||
+        t->walk_run_log2 > 8)
+        return pg_set_error(PG_ERR_ARG,
+                            "tuning out of range (max_buckets <= %d, bucket_keys in [256, %d], "
+                            "walk_run_log2 <= 8)", kMaxBuckets, kBucketCap);
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    invalidate_launches(e);
+    e->max_buckets = t->max_buckets;
+    e->bucket_keys = t->bucket_keys;
+    e->walk_run_log2 = t->walk_run_log2;
+    return PG_OK;
+}
+
+extern "C" int pg_engine_get_tuning(pg_engine* e, pg_tuning* t) {
+    if (!e || !t) return pg_set_error(PG_ERR_ARG, "engine or tuning is NULL");
+    t->max_buckets = e->max_buckets;
+    t->bucket_keys = e->bucket_keys;
+    t->walk_run_log2 = e->walk_run_log2;
+    t->reserved = 0;
+    return PG_OK;
+}
+
 extern "C" int pg_engine_set_profiling(pg_engine* e, int enable) {
     if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
     e->profiling = enable != 0;
@@ -385,7 +425,7 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     };
     const uint64_t o_k0 = take(bodies * 8), o_k1 = take(bodies * 8);
     const uint64_t o_v0 = take(bodies * 4), o_v1 = take(bodies * 4);
-    const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies);
+    const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies), o_ll = take(bodies);
     const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
     const uint64_t o_bk = take(bodies * 2), o_sp = take(kMaxBuckets * 8);
     const uint64_t o_bc = take(kMaxBuckets * 4), o_bf = take(kMaxBuckets * 4);
@@ -416,6 +456,7 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     e->vals[1] = (uint32_t*)(b + o_v1);
     e->first_node = (uint32_t*)(b + o_fn);
     e->lo_level = (uint8_t*)(b + o_lo);
+    e->leaf_lv = (uint8_t*)(b + o_ll);
     e->sort_cnt = (uint32_t*)(b + o_ss);
     e->sort_tot = (uint32_t*)(b + o_st);
     e->bkt = (uint16_t*)(b + o_bk);
@@ -659,7 +700,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "scan_gather", [&] {
         k_scan_gather<DIM><<<tiles, kBlock, 0, st>>>(
             desc, e->tile_start, e->ext_partial, bp.nchunks, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local,
-            e->tile_tot, e->lo_level, e->lvl_first, e->k2, e->err_global);
+            e->tile_tot, e->lo_level, e->lvl_first, e->k2, e->err_global, e->leaf_lv);
     });
     launch(e, cap, "tree_scan", [&] {
         k_tree_scan<<<1 + kTopSlots, kScanThreads, 0, st>>>(e->tile_tot, e->tile_base, tiles,
@@ -680,7 +721,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "walk", [&] {
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
             desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
-            e->acc_out, e->ext_partial, e->sc, bp.sort_tag);
+            e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2);
     });
 }
 
@@ -793,7 +834,7 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     if (bp.total >= (1ull << 32))
         return pg_set_error(PG_ERR_ARG, "a batch holds at most 2^32-1 bodies");
     {
-        const uint64_t nb = (bp.total + kBucketKeys - 1) / kBucketKeys;
+        const uint64_t nb = (bp.total + e->bucket_keys - 1) / e->bucket_keys;
         bp.buckets = nb >= 1 && nb <= e->max_buckets ? (uint32_t)nb : 0u;
     }
     uint64_t nodes = bp.kind == PG_KIND_DIRECT ? 0 : nodes_bound(bp.total, bp.dim, bp.lmax) + bp.nseg;
@@ -904,6 +945,7 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
             d.tile0 = tile0[s];
             d.nsplit = nsplit[s];
             d.part_off = part_off[s];
+            d.near = sm->near;
         }
         off += sm->n;
     }
@@ -1126,6 +1168,14 @@ extern "C" int pg_sim_set_params(pg_sim* s, const pg_params* p) {
     return PG_OK;
 }
 
+extern "C" int pg_sim_set_near_field(pg_sim* s, double leaf_widths) {
+    if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
+    if (!(leaf_widths >= 0.0) || leaf_widths > 1e6)
+        return pg_set_error(PG_ERR_ARG, "near field of %g leaf widths", leaf_widths);
+    s->near = leaf_widths;
+    return PG_OK;
+}
+
 extern "C" int pg_sim_set_counting(pg_sim* s, int enable) {
     if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
     s->counting = enable != 0;
@@ -1305,6 +1355,130 @@ extern "C" int pg_batch_step(pg_engine* e, pg_sim* const* sims, uint32_t count, 
         if (sims[i] && sims[i]->integ != PG_INTEG_VERLET)
             return pg_set_error(PG_ERR_ARG, "batched steps take verlet simulations (sim %u)", i);
     return run_batch(e, sims, count, steps, true);
+}
+
+// The autotuner. Every candidate tuning leaves the results bit-identical
+// (the sort's order, tree and arithmetic do not depend on them); only the
+// time changes. Each candidate runs the batch from the same saved state: four
+// warm-up steps (first-step passes, graph capture), then `trial_steps` timed
+// steps; the fastest is kept, and the simulations get their saved state back.
+extern "C" int pg_engine_autotune(pg_engine* e, pg_sim* const* sims, uint32_t count,
+                                  uint32_t trial_steps, pg_tuning* best, pg_tuning* tried,
+                                  double* ms_per_step, uint32_t cap, uint32_t* ntried) {
+    if (!e || !sims || !count) return pg_set_error(PG_ERR_ARG, "engine or sims is NULL");
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!sims[i] || sims[i]->eng != e || !sims[i]->uploaded)
+            return pg_set_error(PG_ERR_ARG, "sim %u is not an uploaded sim of this engine", i);
+        if (sims[i]->integ != PG_INTEG_VERLET)
+            return pg_set_error(PG_ERR_ARG, "the autotuner times verlet steps (sim %u)", i);
+        total += sims[i]->n;
+    }
+    if (trial_steps == 0) trial_steps = 4;
+    // candidates: the walk's XCD runs, and for a batch the sample sort can
+    // take, its bucket size (or the radix sort)
+    std::vector<pg_tuning> cand;
+    const bool tree = sims[0]->p.kind != PG_KIND_DIRECT;
+    std::vector<uint32_t> runs = tree ? std::vector<uint32_t>{0u, 2u, 4u, 6u}
+                                      : std::vector<uint32_t>{e->walk_run_log2};
+    std::vector<std::pair<uint32_t, uint32_t>> sorts;  // (max_buckets, bucket_keys)
+    if (tree) {
+        for (uint32_t bk : {1280u, 1664u, 2048u})
+            if ((total + bk - 1) / bk <= (uint64_t)kMaxBuckets) sorts.push_back({kMaxBuckets, bk});
+        sorts.push_back({0u, kBucketKeys});
+    } else {
+        sorts.push_back({e->max_buckets, e->bucket_keys});
+    }
+    for (auto so : sorts)
+        for (uint32_t r : runs) cand.push_back(pg_tuning{so.first, so.second, r, 0u});
+    pg_tuning keep;
+    PG_TRY(pg_engine_get_tuning(e, &keep));
+    // the saved state: each simulation's device block and host step fields
+    struct Saved {
+        void* block = nullptr;
+        int cur = 0;
+        uint64_t stage_steps = 0;
+        double dt_last = 0.0;
+    };
+    std::vector<Saved> saved(count);
+    int rc = PG_OK;
+    for (uint32_t i = 0; i < count && rc == PG_OK; ++i) {
+        if (hipMalloc(&saved[i].block, sims[i]->bytes) != hipSuccess) {
+            saved[i].block = nullptr;
+            rc = pg_set_error(PG_ERR_NOMEM, "autotune: no memory to save sim %u", i);
+            break;
+        }
+        if (hipMemcpyAsync(saved[i].block, sims[i]->block, sims[i]->bytes,
+                           hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+            rc = pg_set_error(PG_ERR_HIP, "autotune: saving sim %u failed", i);
+        saved[i].cur = sims[i]->cur;
+        saved[i].stage_steps = sims[i]->stage_steps;
+        saved[i].dt_last = sims[i]->dt_last;
+    }
+    auto restore = [&]() -> int {
+        for (uint32_t i = 0; i < count; ++i) {
+            if (!saved[i].block) continue;
+            HIP_TRY(hipMemcpyAsync(sims[i]->block, saved[i].block, sims[i]->bytes,
+                                   hipMemcpyDeviceToDevice, e->stream));
+            sims[i]->cur = saved[i].cur;
+            sims[i]->stage_steps = saved[i].stage_steps;
+            sims[i]->dt_last = saved[i].dt_last;
+        }
+        // the positions changed under the extent partials the last walk left:
+        // the device-side tag that lets the extent pass skip goes too (as on
+        // an upload)
+        if (e->sc) HIP_TRY(hipMemsetAsync(&e->sc->ext_tag, 0, sizeof(uint64_t), e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        e->ext_valid = 0;
+        e->sorted_valid = 0;
+        e->last_sim = nullptr;
+        return PG_OK;
+    };
+    double best_ms = 1e300;
+    pg_tuning best_t = keep;
+    uint32_t k = 0;
+    for (const pg_tuning& t : cand) {
+        if (rc != PG_OK) break;
+        rc = pg_engine_set_tuning(e, &t);
+        if (rc == PG_OK) rc = restore();
+        // warm-up: the first-step passes, then a captured graph
+        if (rc == PG_OK) rc = run_batch(e, sims, count, 4, true);
+        double ms = 0.0;
+        if (rc == PG_OK) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = run_batch(e, sims, count, trial_steps, true);
+            ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                     .count() / trial_steps;
+        }
+        if (rc != PG_OK) {
+            const std::string why = pg_last_error();
+            pg_set_error(rc, "autotune candidate %u (max_buckets %u, bucket_keys %u, walk_run_log2 "
+                         "%u): %s", k, t.max_buckets, t.bucket_keys, t.walk_run_log2, why.c_str());
+            break;
+        }
+        if (k < cap) {
+            if (tried) tried[k] = t;
+            if (ms_per_step) ms_per_step[k] = ms;
+        }
+        ++k;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_t = t;
+        }
+    }
+    // the fastest tuning stays set; the simulations get their state back
+    const std::string err = pg_last_error();
+    int rc2 = pg_engine_set_tuning(e, rc == PG_OK ? &best_t : &keep);
+    int rc3 = restore();
+    if (rc != PG_OK) pg_set_error(rc, "%s", err.c_str());
+    for (Saved& sv : saved)
+        if (sv.block) (void)hipFree(sv.block);
+    if (ntried) *ntried = k;
+    if (best) *best = rc == PG_OK ? best_t : keep;
+    if (rc != PG_OK) return rc;
+    return rc2 != PG_OK ? rc2 : rc3;
 }
 
 extern "C" int pg_sim_accel(pg_sim* s, double* acc) {

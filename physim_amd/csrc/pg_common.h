@@ -72,6 +72,7 @@ struct SegDesc {
     uint32_t tile0;   // first node tile of this segment (node tiles start at its first body)
     uint32_t nsplit;  // direct sum: blocks summing one target tile, each over a range of sources
     uint64_t part_off;  // direct sum with nsplit > 1: first partial sum (x3 doubles) of the segment
+    double near;      // near field in widths of the body's own leaf cell (0: off)
 };
 
 // Per-segment device scalars.

@@ -1476,7 +1476,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
     uint32_t* __restrict__ node_local, uint32_t* __restrict__ tile_tot,
     uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_first,
-    const uint64_t* __restrict__ k2, uint32_t* err) {
+    const uint64_t* __restrict__ k2, uint32_t* err, uint8_t* __restrict__ leaf_lv) {
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_cnt[kScanTile];
     __shared__ uint32_t s_firstlo[kTopSlots];
@@ -1502,7 +1502,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
             const uint64_t key = skeys[i];
             const uint32_t s = seg_of_key(key, segbits);
             const uint64_t src = s < nseg ? (uint64_t)svals[i] - desc[s].off : ~0ull;
-            int lo = 0;
+            int lo = 0, leaf = 0;
             if (s >= nseg || src >= desc[s].n || i < desc[s].off || i >= desc[s].off + desc[s].n) {
                 atomicOr(err, kErrCapacity);
                 c = 1;
@@ -1519,6 +1519,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 if (lcpL >= depth) lcpL = depth - 1;
                 c = 1u + (uint32_t)max(0, lcpR - lcpL);
                 lo = lcpL + 1;
+                leaf = max(lcpL, lcpR) + 1;  // the level of the body's own leaf
                 const uint64_t j = i - d.off;
                 d.dx[j] = d.sx[src]; d.dy[j] = d.sy[src]; d.dz[j] = d.sz[src];
                 // the previous positions only where the Stormer history is
@@ -1535,6 +1536,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 d.did[j] = d.sid[src];
             }
             lo_level[i] = (uint8_t)lo;
+            // by sorted position, read by a walk with a near field
+            if (s < nseg && desc[s].near > 0.0) leaf_lv[i] = (uint8_t)leaf;
             atomicMin(&s_firstlo[lo], (uint32_t)i);
         }
         s_cnt[li] = c;
@@ -2355,16 +2358,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                                                  const NodeHot* __restrict__ hot, uint64_t cap,
                                                  uint32_t nseg, double* __restrict__ acc_out,
                                                  uint64_t* __restrict__ ext_partial,
-                                                 StepCounters* __restrict__ sc, uint64_t tag) {
-    // Workgroups go to the 8 XCDs in turn. Each XCD takes runs of 16
-    // consecutive blocks (4096 sorted bodies), so neighbouring walks share
-    // their node records in one L2, while the runs of a batch's costly
-    // segments still spread over all XCDs (whole groups of 8 runs; the
-    // blocks past them keep their own index: a bijection for any grid)
+                                                 StepCounters* __restrict__ sc, uint64_t tag,
+                                                 const uint8_t* __restrict__ leaf_lv,
+                                                 uint32_t run_log2) {
+    // Workgroups go to the 8 XCDs in turn. Each XCD takes runs of 2^run_log2
+    // consecutive blocks (16 by default: 4096 sorted bodies), so neighbouring
+    // walks share their node records in one L2, while the runs of a batch's
+    // costly segments still spread over all XCDs (whole groups of 8 runs; the
+    // blocks past them keep their own index: a bijection for any grid; 0 is
+    // the plain round-robin order)
     uint32_t blk = blockIdx.x;
-    if (blockIdx.x < (gridDim.x & ~127u)) {
+    const uint32_t group = 8u << run_log2;
+    if (blockIdx.x < gridDim.x / group * group) {
         const uint32_t x = blockIdx.x & 7u, k = blockIdx.x >> 3;
-        blk = (k >> 4) * 128u + x * 16u + (k & 15u);
+        blk = (k >> run_log2) * group + (x << run_log2) + (k & ((1u << run_log2) - 1u));
     }
     const uint64_t g = (uint64_t)blk * blockDim.x + threadIdx.x;
     bool valid = g < total;
@@ -2400,6 +2407,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         const double e = d0.e;
         const float ef = (float)e;
         const bool walker = mine && !fixed;
+        // near field (no reference counterpart; off at 0): a cell is opened,
+        // never taken whole, when the body lies within `near` widths of its
+        // own leaf cell of the cell's box, so every body that close adds its
+        // own pair force
+        const bool nearf = d0.near > 0.0;
+        const double margin = nearf && walker ? d0.near * ldexp(E, 1 - (int)leaf_lv[g]) : 0.0;
         uint32_t skip_until = 0;
         uint32_t nd = root;
         // The next node is nd + 1 (some lane opened this one) or its skip:
@@ -2417,8 +2430,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                 bool take = leaf;
                 if (!leaf) {
                     const double d0x = x - h.cx, d0y = y - h.cy, d0z = z - h.cz;
-                    take = accept_cell(ldexp(E, -(int)h.level),
-                                       d0x * d0x + d0y * d0y + d0z * d0z, theta);
+                    const double hw = ldexp(E, -(int)h.level);
+                    take = accept_cell(hw, d0x * d0x + d0y * d0y + d0z * d0z, theta);
+                    if (take && nearf) {
+                        const double reach = hw + margin;
+                        take = !(fabs(d0x) < reach && fabs(d0y) < reach &&
+                                 (DIM == 2 || fabs(d0z) < reach));
+                    }
                 }
                 if (take) {
                     ++count;

@@ -76,6 +76,30 @@ def test_bh_mixed_matches_oracle(engine, kind, dim, theta):
 
 
 @pytest.mark.parametrize("kind,dim", [("astro2", 3), ("astro", 2)])
+@pytest.mark.parametrize("near", [2.0, 5.0])
+def test_near_field_matches_oracle(engine, kind, dim, near):
+    """The near field (cells within `near` widths of the body's own leaf
+    opened, so close bodies add pair forces): interaction counts bit-exact
+    against the oracle's walk with the same rule, accelerations to the FP64
+    and mixed bars; stepping with it works alike in a batch."""
+    s = small_cube(6000, seed=4)
+    ref, counts, fault = O.bh_transform_near(s, 0.5, 0.5, near, dim=dim, counts=True)
+    assert fault == 0
+    for prec in ("fp64", "mixed"):
+        sim = P.Sim(engine, s, kind=kind, precision=prec, theta=0.5, e=0.5)
+        sim.set_near_field(near)
+        sim.set_counting(True)
+        a = sim.accel()
+        np.testing.assert_array_equal(sim.interaction_counts(), counts)
+        r = rel_err(a, ref)
+        if prec == "fp64":
+            assert fp64_ok(r)
+        else:
+            assert np.quantile(r, 0.999) <= 1e-4 and r.max() <= 1e-3
+        sim.close()
+
+
+@pytest.mark.parametrize("kind,dim", [("astro2", 3), ("astro", 2)])
 def test_tree_topology_bit_exact(engine, kind, dim):
     for seed, n in ((1, 10000), (5, 3000), (9, 257)):
         s = small_cube(n, seed=seed)

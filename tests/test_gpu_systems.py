@@ -212,3 +212,34 @@ def test_mixed_precision_energy_drift_close_to_fp64(engine, kind, theta):
     assert diff.max() <= 1e-6
     if kind == "simple_astro":
         assert np.abs(drift["fp64"]).max() < 1e-3
+
+
+# ---------------------------------------------------------------- autotuner
+def test_autotune_keeps_results_and_state(engine):
+    """Every tuning the autotuner tries gives bit-identical steps; the
+    autotuner leaves the simulations' state as it found them and keeps its
+    choice in the engine."""
+    states = [plummer(n, seed=80 + k) for k, n in enumerate((3000, 9000, 1500, 20000))]
+    kw = dict(kind="astro2", theta=0.7, e=0.01, dt=1e-4)
+    default = engine.tuning()
+    refs = [alone(engine, s, 6, **kw) for s in states]
+    sims = [P.Sim(engine, s, **kw) for s in states]
+    engine.batch_step(sims, 2)
+    try:
+        best, tried = engine.autotune(sims, trial_steps=3)
+        assert len(tried) >= 4 and best in [t for t, _ in tried]
+        assert all(ms > 0 for _, ms in tried)
+        assert engine.tuning() == best
+        print("autotune:", best, [(t, round(ms, 3)) for t, ms in tried])
+        engine.batch_step(sims, 4)
+        for sim, ref in zip(sims, refs):
+            assert sim.download().tobytes() == ref.tobytes()
+        # each candidate alone gives the same steps
+        for t, _ in tried:
+            engine.set_tuning(**t)
+            out = [alone(engine, s, 6, **kw) for s in states[:2]]
+            assert all(o.tobytes() == r.tobytes() for o, r in zip(out, refs[:2])), t
+    finally:
+        engine.set_tuning(**default)
+        for sim in sims:
+            sim.close()

@@ -365,3 +365,27 @@ def test_parallel_baseline_steps_equal_serial():
     O.pipeline_steps_timed_mt(b, 1.5, 0.5, 1e-5, 3, 4)
     for f in ("x", "y", "z", "vx", "vy", "vz", "mass", "id"):
         np.testing.assert_array_equal(a[f], b[f])
+
+
+def test_near_field_rule():
+    """The build's near field in the oracle's walk: 0 leaf widths is the
+    reference's walk exactly; widening it only opens cells (lists grow); a
+    near field wider than the tree takes every body directly, which is the
+    direct sum (up to summation order)."""
+    s = O.cube_and_stars(3000, seed=2)
+    a0, c0, _ = O.bh_transform(s, 0.5, 0.5, counts=True)
+    a1, c1, _ = O.bh_transform_near(s, 0.5, 0.5, 0.0, counts=True)
+    np.testing.assert_array_equal(a0, a1)
+    np.testing.assert_array_equal(c0, c1)
+    _, c2, _ = O.bh_transform_near(s, 0.5, 0.5, 2.0, counts=True)
+    _, c3, _ = O.bh_transform_near(s, 0.5, 0.5, 6.0, counts=True)
+    assert np.all(c2 >= c0) and np.all(c3 >= c2) and c2.sum() > c0.sum()
+    ad, cd, _ = O.bh_transform_near(s, 0.5, 0.5, 1e6, counts=True)
+    assert np.all(cd == len(s))
+    ref = O.simple_transform(s, 0.5)
+    r = np.linalg.norm(ad - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert r.max() <= 1e-12
+    # rows agree with the full pass
+    rows = np.array([0, 17, 2999, 3001])
+    ar, cr, _ = O.bh_transform_near(s, 0.5, 0.5, 2.0, rows=rows, counts=True)
+    np.testing.assert_array_equal(cr, c2[rows])
