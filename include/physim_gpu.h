@@ -142,6 +142,14 @@ int pg_sim_upload(pg_sim* sim, const Entity* state, uint64_t n);
  * takes as x_prev (verlet.rs:52-82,104-107: the history survives an edited
  * state of the same length). */
 int pg_sim_upload_state(pg_sim* sim, const Entity* state, const Entity* previous, uint64_t n);
+/* A checkpoint of a verlet simulation: its state (as pg_sim_download) and
+ * its Stormer history as `previous` (the records with the previous
+ * positions; may be NULL), *has_history = 0 before the first step (previous
+ * then equals state). pg_sim_upload_state(sim, state, previous) resumes it:
+ * the resumed run continues exactly as the checkpointed one (verlet.rs:
+ * 52-82 needs x_prev, which a plain download drops). */
+int pg_sim_checkpoint(pg_sim* sim, Entity* state, Entity* previous, uint64_t n,
+                      int* has_history);
 /* Device -> host AoS in upload order; radius, id and fixed are those
  * uploaded, as verlet copies them (verlet.rs:41,72). */
 int pg_sim_download(pg_sim* sim, Entity* out, uint64_t n);

@@ -136,6 +136,7 @@ def lib():
         "pg_sim_download": (I, [P, P, U64]),
         "pg_sim_step": (I, [P, U32]),
         "pg_sim_upload_state": (I, [P, P, P, U64]),
+        "pg_sim_checkpoint": (I, [P, P, P, U64, ctypes.POINTER(I)]),
         "pg_verlet_create": (I, [PP]),
         "pg_verlet_destroy": (None, [P]),
         "pg_verlet_bind": (I, [P, P]),
@@ -373,6 +374,18 @@ class Sim:
             self.set_integrator(integrator)
         if state is not None:
             self.upload(state)
+
+    def checkpoint(self):
+        """(state, previous or None): the state and its Stormer history."""
+        st, prev, h = np.zeros(self.n, ENTITY), np.zeros(self.n, ENTITY), ctypes.c_int()
+        check(lib().pg_sim_checkpoint(self.h, _p(st), _p(prev), self.n, ctypes.byref(h)))
+        return st, (prev if h.value else None)
+
+    def resume(self, state, previous=None):
+        """Uploads a checkpoint: the next step continues its history."""
+        state = np.ascontiguousarray(state, ENTITY)
+        prev = None if previous is None else np.ascontiguousarray(previous, ENTITY)
+        check(lib().pg_sim_upload_state(self.h, _p(state), _p(prev), len(state)))
 
     def set_near_field(self, leaf_widths):
         """Direct-sum near field: cells within `leaf_widths` widths of each

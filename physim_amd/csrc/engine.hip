@@ -75,6 +75,8 @@ struct pg_engine {
     uint32_t* first_node = nullptr;
     uint8_t* lo_level = nullptr;
     uint8_t* leaf_lv = nullptr;    // leaf level of each sorted body (near-field walks)
+    double4* leafpos = nullptr;    // position and mass of each sorted body (the leaves)
+    uint2* node_span = nullptr;    // per internal node: first body, body after its range
     uint32_t* sort_cnt = nullptr;  // per-tile digit counts, digit-major
     uint32_t* sort_tot = nullptr;  // per-digit totals of one pass
     uint16_t* bkt = nullptr;       // sample sort: bucket of each key
@@ -438,6 +440,7 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     const uint64_t o_lf = take((node_tiles + 2) * kTopSlots * 4);
     const uint64_t o_k2 = take(bodies * 8);
     const uint64_t o_hot = take(nodes * sizeof(NodeHot));
+    const uint64_t o_span = take(nodes * sizeof(uint2)), o_lp = take(bodies * sizeof(double4));
     const uint64_t o_acc = take(bodies * 24);
     const uint64_t o_d0 = take(segs * sizeof(SegDesc)), o_d1 = take(segs * sizeof(SegDesc));
     const uint64_t o_so = take((segs + 1) * 8);
@@ -472,6 +475,8 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     e->lvl_first = (uint32_t*)(b + o_lf);
     e->k2 = (uint64_t*)(b + o_k2);
     e->hot = (NodeHot*)(b + o_hot);
+    e->node_span = (uint2*)(b + o_span);
+    e->leafpos = (double4*)(b + o_lp);
     e->acc_out = (double*)(b + o_acc);
     e->desc[0] = (SegDesc*)(b + o_d0);
     e->desc[1] = (SegDesc*)(b + o_d1);
@@ -700,7 +705,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "scan_gather", [&] {
         k_scan_gather<DIM><<<tiles, kBlock, 0, st>>>(
             desc, e->tile_start, e->ext_partial, bp.nchunks, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local,
-            e->tile_tot, e->lo_level, e->lvl_first, e->k2, e->err_global, e->leaf_lv);
+            e->tile_tot, e->lo_level, e->lvl_first, e->k2, e->err_global, e->leaf_lv, e->leafpos);
     });
     launch(e, cap, "tree_scan", [&] {
         k_tree_scan<<<1 + kTopSlots, kScanThreads, 0, st>>>(e->tile_tot, e->tile_base, tiles,
@@ -711,7 +716,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
             desc, e->seg_off, e->tile_start, total, skeys, bp.nseg, bp.segbits, bp.lmax,
             e->node_local, e->tile_base, e->lvl_first, e->first_node, e->lo_level, e->hot,
             e->cap_nodes, e->top_slot, e->top_last, e->out_part, e->in_part, e->tot_part, e->k2,
-            e->sc);
+            e->sc, e->leafpos, e->node_span);
     });
     launch(e, cap, "com_span", [&] {
         k_com_span<<<dim3(tiles, kComSpanGroups), kBlock, 0, st>>>(
@@ -721,7 +726,8 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     launch(e, cap, "walk", [&] {
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
             desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
-            e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2);
+            e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2,
+            e->leafpos, e->node_span);
     });
 }
 
@@ -1304,6 +1310,37 @@ static int step_stages(pg_engine* e, pg_sim* s, uint32_t steps) {
     return PG_OK;
 }
 
+extern "C" int pg_sim_checkpoint(pg_sim* s, Entity* state, Entity* previous, uint64_t n,
+                                 int* has_history) {
+    if (!s || !state) return pg_set_error(PG_ERR_ARG, "sim or state is NULL");
+    if (s->integ != PG_INTEG_VERLET)
+        return pg_set_error(PG_ERR_STATE, "checkpoints hold the verlet history (euler and rk4 "
+                                          "resume from the state alone: pg_sim_download)");
+    PG_TRY(pg_sim_download(s, state, n));
+    pg_engine* e = s->eng;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    HIP_TRY(hipSetDevice(e->device));
+    SegState st;
+    HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
+    const bool hist = st.steps_done > 0;
+    if (has_history) *has_history = hist ? 1 : 0;
+    if (!previous || !n) return PG_OK;
+    if (!hist) {
+        memcpy(previous, state, n * sizeof(Entity));
+        return PG_OK;
+    }
+    // the previous positions, staged in the other state buffer (dead
+    // between steps; its ten FP64 arrays hold n * 80 bytes)
+    const int c = s->cur;
+    Entity* stage = (Entity*)s->x[c ^ 1];
+    k_pack_prev<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(stage, s->aos, n, s->px[c],
+                                                                 s->py[c], s->pz[c], s->id[c]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(previous, stage, n * sizeof(Entity), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return PG_OK;
+}
+
 extern "C" int pg_sim_step(pg_sim* s, uint32_t steps) {
     if (!s) return pg_set_error(PG_ERR_ARG, "sim is NULL");
     pg_engine* e = s->eng;
@@ -1576,9 +1613,10 @@ extern "C" int pg_sim_stats(pg_sim* s, pg_stats* out) {
     out->error = map_err(st.err);
     out->key_levels = 2u * (uint32_t)key_lmax(key_dim(s->p.kind), 0);
     if (e->last_sim == s) {
-        uint32_t nn = 0;
-        HIP_TRY(hipMemcpy(&nn, e->first_node + e->last_total, 4, hipMemcpyDeviceToHost));
-        out->nodes = nn;
+        // internal nodes stored, plus one leaf per body
+        uint32_t ni = 0;
+        HIP_TRY(hipMemcpy(&ni, e->first_node + e->last_total, 4, hipMemcpyDeviceToHost));
+        out->nodes = (uint64_t)ni + s->n;
     }
     return PG_OK;
 }
@@ -1612,76 +1650,83 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
     if (e->last_sim != s) return pg_set_error(PG_ERR_STATE, "the last build was not of this sim");
     HIP_TRY(hipSetDevice(e->device));
     const uint64_t n = s->n;
+    // the build stores the internal nodes in preorder; every body is a leaf
     std::vector<uint32_t> fn(n + 1);
     HIP_TRY(hipMemcpy(fn.data(), e->first_node, (n + 1) * 4, hipMemcpyDeviceToHost));
-    const uint64_t nn = fn[n];
+    const uint64_t ni = fn[n];
+    const uint64_t nn = ni + n;
     if (count) *count = nn;
     if (!rec && !fl) return PG_OK;
-    std::vector<NodeHot> hot(nn);
+    std::vector<NodeHot> hot(ni);
+    std::vector<uint2> span(ni);
     std::vector<uint32_t> id(n);
-    HIP_TRY(hipMemcpy(hot.data(), e->hot, nn * sizeof(NodeHot), hipMemcpyDeviceToHost));
-    // The permuted ids live in the buffer the build wrote: the current one.
-    HIP_TRY(hipMemcpy(id.data(), s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
-    uint64_t first = 0;  // the body owning node k: first_node[first] <= k < first_node[first + 1]
-    for (uint64_t k = 0; k < nn && k < cap; ++k) {
-        const NodeHot& h = hot[k];
-        while (first + 1 < n && fn[first + 1] <= k) ++first;
-        const int L = (int)h.level;
-        const int oct = L > 0 ? (int)h.octant : -1;
-        const bool leaf = h.skip == k + 1;
-        if (rec) {
-            rec[4 * k + 0] = L;
-            rec[4 * k + 1] = oct;
-            rec[4 * k + 2] = leaf ? (int64_t)id[first] : -1;
-            rec[4 * k + 3] = (int64_t)first;
-        }
-        if (fl) {
-            fl[8 * k + 0] = h.mass;
-            fl[8 * k + 1] = h.mx;
-            fl[8 * k + 2] = h.my;
-            fl[8 * k + 3] = h.mz;
-            fl[8 * k + 4] = h.cx;
-            fl[8 * k + 5] = h.cy;
-            fl[8 * k + 6] = h.cz;
-        }
+    std::vector<uint8_t> lo(n);
+    std::vector<double4> lp(n);
+    std::vector<uint64_t> keys(n), k2;
+    if (ni) {
+        HIP_TRY(hipMemcpy(hot.data(), e->hot, ni * sizeof(NodeHot), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(span.data(), e->node_span, ni * sizeof(uint2), hipMemcpyDeviceToHost));
     }
-    if (fl) {
-        SegState st;
-        HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
-        double E;
-        memcpy(&E, &st.extent_bits, 8);
-        for (uint64_t k = 0; k < nn && k < cap; ++k) fl[8 * k + 7] = std::ldexp(E, -(int)hot[k].level);
-        // Leaves' centres, which the build does not store (the walk never
-        // reads them): the reference's descent (octree.rs:160-215 /
-        // quadtree.rs:161-194) along the leaf body's key to the leaf's level,
-        // the same operations as the device's.
-        std::vector<uint64_t> keys(n), k2;
-        HIP_TRY(hipMemcpy(keys.data(), e->keys[e->last_sorted], n * 8, hipMemcpyDeviceToHost));
-        const int lmax = e->last_lmax, dim = e->last_dim;
-        const uint32_t mask = (1u << dim) - 1u;
-        uint64_t owner = 0;
-        for (uint64_t k = 0; k < nn && k < cap; ++k) {
-            while (owner + 1 < n && fn[owner + 1] <= k) ++owner;
-            if (hot[k].skip != k + 1) continue;  // internal
-            const int L = (int)hot[k].level;
-            if (L > lmax && k2.empty()) {
-                k2.resize(n);
-                HIP_TRY(hipMemcpy(k2.data(), e->k2, n * 8, hipMemcpyDeviceToHost));
+    // the permuted ids live in the buffer the build wrote: the current one
+    HIP_TRY(hipMemcpy(id.data(), s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lo.data(), e->lo_level, n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lp.data(), e->leafpos, n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(keys.data(), e->keys[e->last_sorted], n * 8, hipMemcpyDeviceToHost));
+    SegState st;
+    HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
+    double E;
+    memcpy(&E, &st.extent_bits, 8);
+    const int lmax = e->last_lmax, dim = e->last_dim;
+    const uint32_t mask = (1u << dim) - 1u;
+    uint64_t out = 0, k = 0;
+    auto put = [&](int64_t level, int64_t oct, int64_t body, int64_t first, double mass,
+                   double mx, double my, double mz, double cx, double cy, double cz) {
+        if (out < cap) {
+            if (rec) {
+                rec[4 * out + 0] = level;
+                rec[4 * out + 1] = oct;
+                rec[4 * out + 2] = body;
+                rec[4 * out + 3] = first;
             }
-            const uint64_t key = keys[owner], kx = L > lmax ? k2[owner] : 0ull;
-            double ext = E, c[3] = {0.0, 0.0, 0.0};
-            for (int l = 0; l < L; ++l) {
-                const uint32_t o =
-                    l < lmax ? (uint32_t)((key >> (dim * (lmax - 1 - l))) & mask)
-                             : (uint32_t)((kx >> (dim * (2 * lmax - 1 - l))) & mask);
-                const double h2 = ext * 0.5;
-                for (int a = 0; a < dim; ++a) c[a] = ((o >> a) & 1u) ? c[a] + h2 : c[a] - h2;
-                ext = h2;
+            if (fl) {
+                double* f = fl + 8 * out;
+                f[0] = mass; f[1] = mx; f[2] = my; f[3] = mz;
+                f[4] = cx; f[5] = cy; f[6] = cz;
+                f[7] = std::ldexp(E, -(int)level);
             }
-            fl[8 * k + 4] = c[0];
-            fl[8 * k + 5] = c[1];
-            fl[8 * k + 6] = c[2];
         }
+        ++out;
+    };
+    for (uint64_t b = 0; b < n && out < cap; ++b) {
+        // the internal nodes that start at body b (shallowest first), then its leaf
+        for (; k < ni && span[k].x == b; ++k) {
+            const NodeHot& h = hot[k];
+            const int L = (int)h.level;
+            put(L, L > 0 ? (int64_t)h.octant : -1, -1, (int64_t)b, h.mass, h.mx, h.my, h.mz,
+                h.cx, h.cy, h.cz);
+        }
+        // the leaf's level, octant and centre: the reference's descent
+        // (octree.rs:160-215 / quadtree.rs:161-194) along the body's key, the
+        // same operations as the device's
+        const int L = (int)lo[b] + (int)(fn[b + 1] - fn[b]);
+        if (L > lmax && k2.empty()) {
+            k2.resize(n);
+            HIP_TRY(hipMemcpy(k2.data(), e->k2, n * 8, hipMemcpyDeviceToHost));
+        }
+        const uint64_t key = keys[b], kx = L > lmax ? k2[b] : 0ull;
+        auto digit = [&](int l) {
+            return l < lmax ? (uint32_t)((key >> (dim * (lmax - 1 - l))) & mask)
+                            : (uint32_t)((kx >> (dim * (2 * lmax - 1 - l))) & mask);
+        };
+        double ext = E, c[3] = {0.0, 0.0, 0.0};
+        for (int l = 0; l < L; ++l) {
+            const uint32_t o = digit(l);
+            const double h2 = ext * 0.5;
+            for (int a = 0; a < dim; ++a) c[a] = ((o >> a) & 1u) ? c[a] + h2 : c[a] - h2;
+            ext = h2;
+        }
+        put(L, L > 0 ? (int64_t)digit(L - 1) : -1, (int64_t)id[b], (int64_t)b, lp[b].w, lp[b].x,
+            lp[b].y, lp[b].z, c[0], c[1], c[2]);
     }
     return PG_OK;
 }

@@ -28,7 +28,8 @@ constexpr int kExtentChunk = 4096; // bodies per extent partial
 constexpr uint32_t kExtentBlocks = 256;  // blocks of the extent pass (each takes chunks in turn)
 constexpr int kNodeTile = 1024;    // sorted bodies per node-building block
 constexpr int kNodeThreads = 512;  // threads per node-building block (two bodies each)
-constexpr int kNodeLds = 1792;     // nodes per tile built one per thread (larger: body by body)
+constexpr int kNodeLds = 1792;     // kNodeTile bodies + up to kNodeLds - kNodeTile internal nodes: built one
+                                   // node per thread (more internal nodes: body by body)
 constexpr int kTopSlots = 65;      // tree levels 0..64: one tile-spanning node each
 constexpr int kMaxTieRun = 256;    // longest run of equal keys ordered by the second key word
 constexpr int kComSpanGroups = 5;   // level groups (of 4 waves) per tile in k_com_span

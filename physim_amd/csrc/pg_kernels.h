@@ -1457,12 +1457,14 @@ __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(
 
 // Per sorted body: its common-prefix lengths with its neighbours give the
 // tree levels it owns (lo..hi, hi = its leaf; lo = 1 + common prefix with the
-// previous body, 0 at a segment start); the exclusive scan of the per-body
-// node counts lays the tree out in preorder. This kernel writes each body's
-// offset within its tile (node_local) and the tile's total; k_tree_scan adds
-// the tiles up. Also permutes the state into the new Morton order
-// (velocities only while they are not derivable from the two positions) and
-// flags equal keys (depth).
+// previous body, 0 at a segment start). The internal nodes (levels lo..hi-1)
+// are stored, in preorder: the exclusive scan of the per-body internal node
+// counts lays them out; leaves are the bodies themselves, in sorted order.
+// This kernel writes each body's offset within its tile (node_local) and the
+// tile's total; k_tree_scan adds the tiles up. Also permutes the state into
+// the new Morton order -- positions and masses as one record per body
+// (leafpos), the rest in the state's arrays (velocities only while they are
+// not derivable from the two positions) -- and flags equal keys (depth).
 //
 // Node ranges: the node at level L containing body i starts at the last body
 // j <= i with lo[j] <= L and ends before the first body j > i with lo[j] <= L.
@@ -1476,7 +1478,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
     uint32_t* __restrict__ node_local, uint32_t* __restrict__ tile_tot,
     uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_first,
-    const uint64_t* __restrict__ k2, uint32_t* err, uint8_t* __restrict__ leaf_lv) {
+    const uint64_t* __restrict__ k2, uint32_t* err, uint8_t* __restrict__ leaf_lv,
+    double4* __restrict__ leafpos) {
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_cnt[kScanTile];
     __shared__ uint32_t s_firstlo[kTopSlots];
@@ -1505,7 +1508,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
             int lo = 0, leaf = 0;
             if (s >= nseg || src >= desc[s].n || i < desc[s].off || i >= desc[s].off + desc[s].n) {
                 atomicOr(err, kErrCapacity);
-                c = 1;
+                c = 0;
             } else {
                 const SegDesc& d = desc[s];
                 const uint64_t end = d.off + d.n;
@@ -1517,11 +1520,16 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                     lcpR = depth - 1;
                 }
                 if (lcpL >= depth) lcpL = depth - 1;
-                c = 1u + (uint32_t)max(0, lcpR - lcpL);
+                // the internal nodes the body owns: levels lcpL + 1 .. lcpR
+                // (its leaf, at level max(lcpL, lcpR) + 1, is implicit)
+                c = (uint32_t)max(0, lcpR - lcpL);
                 lo = lcpL + 1;
                 leaf = max(lcpL, lcpR) + 1;  // the level of the body's own leaf
                 const uint64_t j = i - d.off;
-                d.dx[j] = d.sx[src]; d.dy[j] = d.sy[src]; d.dz[j] = d.sz[src];
+                // position and mass as one record in sorted order: the node
+                // builder's input and the walk's leaves (the walk writes the
+                // state's own arrays)
+                leafpos[i] = make_double4(d.sx[src], d.sy[src], d.sz[src], d.sm[src]);
                 // the previous positions only where the Stormer history is
                 // valid (the Taylor step writes them; force-only calls never
                 // read them)
@@ -1531,7 +1539,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 if (d.st->vperm) {
                     d.dvx[j] = d.svx[src]; d.dvy[j] = d.svy[src]; d.dvz[j] = d.svz[src];
                 }
-                d.dm[j] = d.sm[src];
                 d.dfix[j] = d.sfix[src];
                 d.did[j] = d.sid[src];
             }
@@ -1827,10 +1834,10 @@ constexpr int kPathLevels = 12;
 // kSmallNode (one thread per body) are queued and summed by lane groups.
 constexpr uint32_t kTinyNode = 8;
 constexpr uint32_t kSmallNode = 64;
-// The queue: one thread per node, a tile has at most kNodeLds nodes of which
-// kNodeTile are leaves; an entry is the owning body (10 bits), the range's
-// last body (10 bits) and the node's place among the owner's (6 bits). One
-// thread per body: internal nodes of more than
+// The queue: one thread per node, a node-parallel tile has at most
+// kNodeQueue internal nodes; an entry is the owning body (10 bits), the
+// range's last body (10 bits) and the node's place among the owner's (6
+// bits). One thread per body: internal nodes of more than
 // kSmallNode bodies hold disjoint body sets at one level, so a tile has at
 // most kNodeTile / (kSmallNode + 1) per level; an entry (16-bit, two per
 // word) is the owning body (10 bits) and the level (6 bits: an internal node
@@ -1860,27 +1867,31 @@ __device__ __forceinline__ uint4 pack_tail(double mass, uint32_t skip, uint32_t 
     return make_uint4((uint32_t)x, (uint32_t)(x >> 32), skip, level | (octant << 16));
 }
 
-// Builds the tree and its masses and centres of mass, one block per node tile
-// (up to kNodeTile sorted bodies of one segment; the tiles of k_scan_gather).
+// Builds the tree's internal nodes and their masses and centres of mass,
+// one block per node tile (up to kNodeTile sorted bodies of one segment; the
+// tiles of k_scan_gather). The leaves are the bodies themselves: the walk
+// reads a leaf's position and mass from leafpos in sorted order, so no leaf
+// record is written.
 //
-// The tile's bodies own the preorder node range [tile_base[t],
-// tile_base[t+1]) (tile base + offsets within the tile, also written out as
-// first_node). Every node gets its geometric centre by the reference's
-// descent, its level, octant, sorted range and skip (= first node of the body
-// after the range); a leaf takes its body's exact position and mass. A node's
-// range end is found by a galloping search over the staged keys (keys carry
-// their segment, so no range crosses one); a node that runs past the tile
-// ends before the first later body starting a node at its level or above (the
-// scanned per-level table), so no search leaves the tile.
+// The tile's bodies own the preorder range [tile_base[t], tile_base[t+1]) of
+// internal nodes (tile base + offsets within the tile, also written out as
+// first_node). Every internal node gets its geometric centre by the
+// reference's descent, its level, octant and skip (= first internal node of
+// the body after its range) in its record, and its body range (first body,
+// body after the range) in node_span. A node's range end is found by a
+// galloping search over the staged keys (keys carry their segment, so no
+// range crosses one); a node that runs past the tile ends before the first
+// later body starting a node at its level or above (the scanned per-level
+// table), so no search leaves the tile.
 //
 // Bodies (keys, levels, positions, masses) are staged in LDS. A tile of at
-// most kNodeLds nodes is built in two passes, its leaves one thread per body
-// and then its internal nodes one thread per node; a larger one (clustered
-// bodies), body by body. Each record is written by
-// the thread that finds its range; an internal node whose bodies all lie in
-// the tile takes its mass and centre of mass from the sums of those bodies in
-// body order (by its thread up to kTinyNode bodies, or kSmallNode body by
-// body; else by a group of lanes after the node passes).
+// most kNodeQueue internal nodes is built one thread per node (the bodies
+// list them first); a larger one (clustered bodies), body by body. Each
+// record is written by the thread that finds its range; an internal node
+// whose bodies all lie in the tile takes its mass and centre of mass from the
+// sums of those bodies in body order (by its thread up to kTinyNode bodies,
+// or kSmallNode body by body; else by a group of lanes after the node
+// passes).
 //
 // An internal node whose range leaves the tile is left to k_com_span: it is
 // recorded (generation-tagged) at slot tile * kTopSlots + level, and
@@ -1895,7 +1906,8 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     uint32_t* __restrict__ first_node, const uint8_t* __restrict__ lo_level,
     NodeHot* __restrict__ hot, uint64_t cap, uint64_t* __restrict__ top_slot,
     uint2* __restrict__ top_last, Sum4* __restrict__ out_part, Sum4* __restrict__ in_part,
-    Sum4* __restrict__ tot_part, const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc) {
+    Sum4* __restrict__ tot_part, const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc,
+    const double4* __restrict__ leafpos, uint2* __restrict__ node_span) {
     static_assert(kNodeTile <= 65536, "owner indices are 16-bit");
     __shared__ uint64_t s_keys[kNodeTile];
     __shared__ uint32_t s_first[kNodeTile + 1];
@@ -1917,9 +1929,8 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     const uint32_t nend = tile_base[tile + 1];
     const uint32_t n1 = (uint32_t)min((uint64_t)nend, cap);
     const uint32_t nloc = n1 > n0 ? n1 - n0 : 0u;
-    // node-parallel where the internal nodes fit the list: a tile's bodies
-    // own one leaf each (a tile cut short by a segment's end has fewer)
-    const bool node_par = nloc <= nb + (uint32_t)kNodeQueue;
+    // node-parallel where the tile's internal nodes fit the list
+    const bool node_par = nloc <= (uint32_t)kNodeQueue;
     // a node tile lies in one segment
     const SegDesc& d = desc[seg_of(seg_off, nseg, b0)];
     const double ext0 = bits_f64(d.st->extent_bits);
@@ -1956,15 +1967,16 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         s_first[li] = f;
         first_node[i] = f;
         const uint64_t j = i - d.off;
-        const double x = d.dx[j], y = d.dy[j], z = d.dz[j], m = d.dm[j];
+        const double4 p = leafpos[i];
+        const double x = p.x, y = p.y, z = p.z, m = p.w;
         s_bm[li] = m;
         s_bx[li] = x;
         s_by[li] = y;
         s_bz[li] = z;
         const bool more = j + 1 < d.n;
-        const double x1 = more ? d.dx[j + 1] : 0.0, y1 = more ? d.dy[j + 1] : 0.0,
-                     z1 = more ? d.dz[j + 1] : 0.0;
-        const uint32_t hi = lo + (fn - f) - 1u;
+        const double4 p1 = more ? leafpos[i + 1] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const double x1 = p1.x, y1 = p1.y, z1 = p1.z;
+        const uint32_t hi = lo + (fn - f);  // the leaf's level
         if (fn > cap) atomicOr(&d.st->err, kErrCapacity);
         // the reference merges bodies within 1e-9 on every axis
         // (octree.rs:69-80); the device reports them instead
@@ -2006,46 +2018,42 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         }
         return s_first[last + 1 - b0];
     };
-    // The record of node nd (level L, owned by local body li, geometric
-    // centre c, octant oct). A tile-spanning node (the only one of its level
-    // that starts in the tile and leaves it) gets zero mass here and its own
-    // from k_com_span; a larger internal node is queued, its mass words
-    // written by the group pass.
-    auto emit = [&](uint32_t nd, uint32_t li, int L, bool leaf, double cx, double cy, double cz,
+    // The record of internal node nd (level L, owned by local body li,
+    // geometric centre c, octant oct), and its body range. A tile-spanning
+    // node (the only one of its level that starts in the tile and leaves it)
+    // gets zero mass here and its own from k_com_span; a larger internal
+    // node is queued, its mass words written by the group pass.
+    auto emit = [&](uint32_t nd, uint32_t li, int L, double cx, double cy, double cz,
                     uint32_t oct) {
-        uint32_t skip;
         Sum4 c{0.0, 0.0, 0.0, 0.0};
         bool queued = false;
-        if (leaf) {
-            skip = s_first[li + 1];
-            c = Sum4{s_bm[li], s_bx[li], s_by[li], s_bz[li]};
+        uint64_t last;
+        bool spans;
+        const uint32_t skip = internal_skip(b0 + li, s_keys[li], L, last, spans);
+        if (spans) {
+            top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
+            // its last body and that body's tile (the node lies in this segment)
+            top_last[(uint64_t)tile * kTopSlots + L] =
+                make_uint2((uint32_t)last, d.tile0 + (uint32_t)((last - d.off) / kNodeTile));
+            atomicMax(&s_topl, (uint32_t)L + 1u);
         } else {
-            uint64_t last;
-            bool spans;
-            skip = internal_skip(b0 + li, s_keys[li], L, last, spans);
-            if (spans) {
-                top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
-                // its last body and that body's tile (the node lies in this segment)
-                top_last[(uint64_t)tile * kTopSlots + L] =
-                    make_uint2((uint32_t)last, d.tile0 + (uint32_t)((last - d.off) / kNodeTile));
-                atomicMax(&s_topl, (uint32_t)L + 1u);
-            } else {
-                const uint32_t e = (uint32_t)(last - b0) + 1u;
-                if (e - li > (node_par ? kTinyNode : kSmallNode)) {
-                    // (the bounds above hold unless the node capacity cut
-                    // the tile short, an error; then the thread sums)
-                    const uint32_t q = atomicAdd(&s_nq, 1u);
-                    if (node_par && q < (uint32_t)kNodeQueue) {
-                        s_q[q] = li | ((e - 1u) << 10) | ((nd - s_first[li]) << 20);
-                        queued = true;
-                    } else if (!node_par && q < 2u * kNodeQueue) {
-                        reinterpret_cast<uint16_t*>(s_q)[q] = (uint16_t)(li | ((uint32_t)L << 10));
-                        queued = true;
-                    }
+            const uint32_t e = (uint32_t)(last - b0) + 1u;
+            if (e - li > (node_par ? kTinyNode : kSmallNode)) {
+                // (the bounds above hold unless the node capacity cut the
+                // tile short, an error; then the thread sums)
+                const uint32_t q = atomicAdd(&s_nq, 1u);
+                if (node_par && q < (uint32_t)kNodeQueue) {
+                    s_q[q] = li | ((e - 1u) << 10) | ((nd - s_first[li]) << 20);
+                    queued = true;
+                } else if (!node_par && q < 2u * kNodeQueue) {
+                    reinterpret_cast<uint16_t*>(s_q)[q] = (uint16_t)(li | ((uint32_t)L << 10));
+                    queued = true;
                 }
-                if (!queued) c = com_of(range_sum(s_bm, s_bx, s_by, s_bz, li, e));
             }
+            if (!queued) c = com_of(range_sum(s_bm, s_bx, s_by, s_bz, li, e));
         }
+        // first body and the body after the range (global sorted indices)
+        node_span[nd] = make_uint2((uint32_t)(b0 + li), (uint32_t)(last + 1));
         uint2* w = reinterpret_cast<uint2*>(hot + nd);
         const uint4 tail = pack_tail(c.m, skip, (uint32_t)L, oct);
         if (queued) {
@@ -2061,38 +2069,24 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         }
     };
 
-    // A leaf's record without its geometric centre: the walk always takes a
-    // leaf and never reads its centre (pg_sim_tree supplies it for the tree
-    // download), so the leaf pass skips the descent.
-    auto build_leaf = [&](uint32_t nd, uint32_t li, int L) {
-        const uint64_t key = s_keys[li];
-        const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
-        const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
-        uint2* w = reinterpret_cast<uint2*>(hot + nd);
-        w[3] = pack1(s_bx[li]);
-        reinterpret_cast<uint4*>(w)[2] = pack2(s_by[li], s_bz[li]);
-        reinterpret_cast<uint4*>(w)[3] = pack_tail(s_bm[li], s_first[li + 1], (uint32_t)L, oct);
-    };
     // node nd (level L) of local body li: its centre by the reference's
     // descent (from the tile's common path), then its record
-    auto build = [&](uint32_t nd, uint32_t li, int L, bool leaf) {
+    auto build = [&](uint32_t nd, uint32_t li, int L) {
         const uint64_t key = s_keys[li];
         const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
         const int f = min(L, lt);
         double ext = s_path[f][0], cx = s_path[f][1], cy = s_path[f][2], cz = s_path[f][3];
         descend<DIM>(key, kx, lmax, f, L, ext, cx, cy, cz);
         const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
-        emit(nd, li, L, leaf, cx, cy, cz, oct);
+        emit(nd, li, L, cx, cy, cz, oct);
     };
     if (node_par) {
-        // leaves, one thread per body, which also lists the body's internal
-        // nodes; then the internal nodes, one thread each (a node-parallel
-        // tile has at most kNodeQueue: the leaves are kNodeTile of its
-        // kNodeLds nodes, unless the capacity cut it short, an error)
+        // the bodies list their internal nodes (at most kNodeQueue in a
+        // node-parallel tile, unless the capacity cut it short, an error),
+        // then one thread builds each
         for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
             const uint32_t f = s_first[li], fn = s_first[li + 1];
-            if (fn - 1u < n1) build_leaf(fn - 1u, li, (int)s_lo[li] + (int)(fn - 1u - f));
-            const uint32_t ni = (min(fn - 1u, n1) > f) ? min(fn - 1u, n1) - f : 0u;
+            const uint32_t ni = (min(fn, n1) > f) ? min(fn, n1) - f : 0u;
             if (ni) {
                 const uint32_t p = atomicAdd(&s_ni, ni);
                 for (uint32_t j = 0; j < ni && p + j < (uint32_t)kNodeQueue; ++j)
@@ -2104,10 +2098,10 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         for (uint32_t q = threadIdx.x; q < ni; q += kNodeThreads) {
             const uint32_t ent = s_ilist[q];
             const uint32_t li = ent & 1023u, j = ent >> 10;
-            build(s_first[li] + j, li, (int)s_lo[li] + (int)j, false);
+            build(s_first[li] + j, li, (int)s_lo[li] + (int)j);
         }
     } else {
-        // one thread per body: one descent through the body's nodes
+        // one thread per body: one descent through the body's internal nodes
         for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
             const uint64_t key = s_keys[li];
             const int lo = s_lo[li];
@@ -2123,7 +2117,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
                 const uint64_t nd = (uint64_t)nbase + (uint64_t)k;
                 if (nd >= cap) break;  // reported above
                 const uint32_t oct = L > 0 ? key_digit<DIM>(key, kx, lmax, L - 1) : 0u;
-                emit((uint32_t)nd, li, L, k + 1 == cnt, cx, cy, cz, oct);
+                emit((uint32_t)nd, li, L, cx, cy, cz, oct);
                 if (k + 1 < cnt) descend<DIM>(key, kx, lmax, L, L + 1, ext, cx, cy, cz);
             }
         }
@@ -2140,7 +2134,6 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     // mass sums and level of local body li (zero / ~0 when absent)
     auto body_sum = [&](uint32_t li) -> Sum4 {
         if (li >= nb) return Sum4{0.0, 0.0, 0.0, 0.0};
-        if (s_first[li + 1] - 1u >= n1) return Sum4{0.0, 0.0, 0.0, 0.0};
         const double m = s_bm[li];
         return Sum4{m, m * s_bx[li], m * s_by[li], m * s_bz[li]};
     };
@@ -2360,7 +2353,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                                                  uint64_t* __restrict__ ext_partial,
                                                  StepCounters* __restrict__ sc, uint64_t tag,
                                                  const uint8_t* __restrict__ leaf_lv,
-                                                 uint32_t run_log2) {
+                                                 uint32_t run_log2,
+                                                 const double4* __restrict__ leafpos,
+                                                 const uint2* __restrict__ node_span) {
     // Workgroups go to the 8 XCDs in turn. Each XCD takes runs of 2^run_log2
     // consecutive blocks (16 by default: 4096 sorted bodies), so neighbouring
     // walks share their node records in one L2, while the runs of a batch's
@@ -2386,8 +2381,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     if (valid) {
         const SegDesc& d = desc[s];
         j = g - d.off;
-        x = d.dx[j]; y = d.dy[j]; z = d.dz[j];
-        m = d.dm[j];
+        const double4 p = leafpos[g];
+        x = p.x; y = p.y; z = p.z;
+        m = p.w;
         fixed = d.dfix[j] != 0;
     }
     Kahan3 ka;
@@ -2400,8 +2396,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         const uint32_t s0 = uniform_u32(__shfl(s, lead));
         const bool mine = pending && s == s0;
         const SegDesc& d0 = desc[s0];
-        const uint32_t root = uniform_u32(first_node[d0.off]);
-        const uint32_t end = uniform_u32(min((uint64_t)first_node[d0.off + d0.n], cap));
+        // the segment's bodies [b, bend) and internal nodes [k, kend) in
+        // preorder (global sorted indices)
+        const uint32_t bend = uniform_u32((uint32_t)(d0.off + d0.n));
+        const uint32_t kend = uniform_u32(min((uint64_t)first_node[bend], cap));
+        uint32_t k = uniform_u32(first_node[d0.off]);
+        uint32_t b = uniform_u32((uint32_t)d0.off);
         const double E = bits_f64(d0.st->extent_bits);
         const double theta = d0.theta;
         const double e = d0.e;
@@ -2413,47 +2413,70 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         // own pair force
         const bool nearf = d0.near > 0.0;
         const double margin = nearf && walker ? d0.near * ldexp(E, 1 - (int)leaf_lv[g]) : 0.0;
-        uint32_t skip_until = 0;
-        uint32_t nd = root;
-        // The next node is nd + 1 (some lane opened this one) or its skip:
-        // both records are loaded while this one is tested, so the walk's
-        // dependent chain does not wait a load per node (indices clamped
-        // into the node array; a record past `end` is never used).
+        uint32_t skip_until = 0;  // a lane skips the items of the bodies below this
+        // The walk visits the tree in preorder as two merged sequences: the
+        // next item is internal node k when k starts at body b (its span's
+        // first body), else the leaf of body b. After node k the next item is
+        // (b, k + 1) when some lane opened it, else (the body after its
+        // range, its skip). Both possible next records are loaded while this
+        // one is tested (indices clamped into the node array; a record past
+        // kend is never used). Each step raises b or k (guarded against a
+        // corrupt record), so the loop ends.
         const uint32_t clamp = (uint32_t)(cap - 1);
-        NodeHot h = hot[min(nd, clamp)];
-        while (nd < end) {
-            const NodeHot h_open = hot[min(nd + 1u, clamp)];
-            const NodeHot h_skip = hot[min(h.skip, clamp)];
-            const bool leaf = h.skip == nd + 1u;
-            bool open = false;
-            if (walker && nd >= skip_until) {
-                bool take = leaf;
-                if (!leaf) {
+        NodeHot h = hot[min(k, clamp)];
+        uint2 sp = node_span[min(k, clamp)];
+        while (b < bend) {
+            // this item's source (a taken node's centre of mass and mass, or
+            // a leaf's body) and whether this lane adds it
+            double bx, by, bz, bm;
+            bool add = false;
+            if (k < kend && sp.x == b) {
+                const NodeHot h_open = hot[min(k + 1u, clamp)];
+                const uint2 sp_open = node_span[min(k + 1u, clamp)];
+                const NodeHot h_skip = hot[min(h.skip, clamp)];
+                const uint2 sp_skip = node_span[min(h.skip, clamp)];
+                bx = h.mx; by = h.my; bz = h.mz; bm = h.mass;
+                bool open = false;
+                if (walker && b >= skip_until) {
                     const double d0x = x - h.cx, d0y = y - h.cy, d0z = z - h.cz;
                     const double hw = ldexp(E, -(int)h.level);
-                    take = accept_cell(hw, d0x * d0x + d0y * d0y + d0z * d0z, theta);
+                    bool take = accept_cell(hw, d0x * d0x + d0y * d0y + d0z * d0z, theta);
                     if (take && nearf) {
                         const double reach = hw + margin;
                         take = !(fabs(d0x) < reach && fabs(d0y) < reach &&
                                  (DIM == 2 || fabs(d0z) < reach));
                     }
+                    if (take) skip_until = sp.y;
+                    add = take;
+                    open = !take;
                 }
-                if (take) {
-                    ++count;
-                    skip_until = h.skip;
-                    if (!(h.mx == x && h.my == y && h.mz == z)) {
-                        if (FP64)
-                            pair_fp64(x, y, z, m, h.mx, h.my, h.mz, h.mass, e, f64);
-                        else
-                            pair_mixed(x, y, z, h.mx, h.my, h.mz, h.mass, ef, ka);
-                    }
+                if (__ballot(open) || h.skip <= k || sp.y <= b) {
+                    k = uniform_u32(k + 1u);
+                    h = h_open;
+                    sp = sp_open;
                 } else {
-                    open = true;
+                    b = uniform_u32(sp.y);
+                    k = uniform_u32(h.skip);
+                    h = h_skip;
+                    sp = sp_skip;
+                }
+            } else {
+                // the leaf of body b: its exact position and mass
+                const double4 lp = leafpos[b];
+                bx = lp.x; by = lp.y; bz = lp.z; bm = lp.w;
+                add = walker && b >= skip_until;
+                b = uniform_u32(b + 1u);
+            }
+            if (add) {
+                ++count;
+                // (the reference skips a source at the body's own position)
+                if (!(bx == x && by == y && bz == z)) {
+                    if (FP64)
+                        pair_fp64(x, y, z, m, bx, by, bz, bm, e, f64);
+                    else
+                        pair_mixed(x, y, z, bx, by, bz, bm, ef, ka);
                 }
             }
-            const bool step = __ballot(open) || h.skip <= nd;
-            nd = uniform_u32(step ? nd + 1u : h.skip);
-            h = step ? h_open : h_skip;
         }
         pending = pending && !mine;
     }
@@ -2510,6 +2533,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
             }
             d.dx[j] = nx; d.dy[j] = ny; d.dz[j] = nz;
             d.dpx[j] = x; d.dpy[j] = y; d.dpz[j] = z;
+            d.dm[j] = m;
             emax = f64_bits(fmax(fmax(fabs(nx), fabs(ny)), fabs(nz)));
             nonfinite = !isfinite(nx) || !isfinite(ny) || !isfinite(nz);
             chunk = d.chunk0 + (uint32_t)(j / kExtentChunk);
@@ -2517,6 +2541,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         } else {
             double* o = acc_out + 3 * (d.off + d.did[j]);
             o[0] = ax; o[1] = ay; o[2] = az;
+            // the permuted state keeps its positions and masses
+            d.dx[j] = x; d.dy[j] = y; d.dz[j] = z;
+            d.dm[j] = m;
         }
     }
     if (INTEGRATE) {
@@ -2829,6 +2856,19 @@ __global__ void k_unpack_prev(const Entity* __restrict__ aos, uint64_t n, double
     if (i >= n) return;
     const Entity e = aos[i];
     px[i] = e.x; py[i] = e.y; pz[i] = e.z;
+}
+
+// The Stormer history as Entity records in upload order: the current
+// record (radius, mass, id, fixed, velocity) with the previous position.
+__global__ void k_pack_prev(Entity* __restrict__ out, const Entity* __restrict__ cur, uint64_t n,
+                            const double* px, const double* py, const double* pz,
+                            const uint32_t* id) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t u = id[j];
+    Entity e = cur[u];
+    e.x = px[j]; e.y = py[j]; e.z = pz[j];
+    out[u] = e;
 }
 
 __global__ void k_pack(Entity* __restrict__ aos, uint64_t n, const double* x, const double* y,

@@ -243,3 +243,31 @@ def test_autotune_keeps_results_and_state(engine):
         engine.set_tuning(**default)
         for sim in sims:
             sim.close()
+
+
+# ---------------------------------------------------------- checkpoint/resume
+@pytest.mark.parametrize("kind", ["astro2", "simple_astro"])
+def test_checkpoint_resume_continues_the_run(engine, kind):
+    """A checkpoint (state + Stormer history) taken after 7 steps and resumed
+    in a fresh simulation gives, after 9 more steps, the run that never
+    stopped, bit for bit; a plain download + upload restarts with the Taylor
+    step and does not."""
+    s = plummer(3000, 91)
+    kw = dict(kind=kind, theta=0.7, e=0.01, dt=1e-4)
+    a = P.Sim(engine, s, **kw)
+    a.step(7)
+    st, prev = a.checkpoint()
+    assert prev is not None and not np.array_equal(prev["x"], st["x"])
+    a.step(9)
+    b = P.Sim(engine, n=len(s), **kw)
+    b.resume(st, prev)
+    b.step(9)
+    assert b.download().tobytes() == a.download().tobytes()
+    c = P.Sim(engine, st, **kw)  # no history: the Taylor step
+    c.step(9)
+    assert c.download().tobytes() != a.download().tobytes()
+    fresh = P.Sim(engine, s, **kw)
+    st0, prev0 = fresh.checkpoint()
+    assert prev0 is None and st0.tobytes() == fresh.download().tobytes()
+    for sim in (a, b, c, fresh):
+        sim.close()
