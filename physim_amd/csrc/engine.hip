@@ -840,8 +840,13 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     if (bp.total >= (1ull << 32))
         return pg_set_error(PG_ERR_ARG, "a batch holds at most 2^32-1 bodies");
     {
-        const uint64_t nb = (bp.total + e->bucket_keys - 1) / e->bucket_keys;
-        bp.buckets = nb >= 1 && nb <= e->max_buckets ? (uint32_t)nb : 0u;
+        // a batch past max_buckets of bucket_keys grows its buckets up to
+        // the slot sort's reach (kSlots keys) before it takes the radix sort
+        uint64_t keys_per = e->bucket_keys;
+        if (e->max_buckets && (bp.total + keys_per - 1) / keys_per > e->max_buckets)
+            keys_per = std::max<uint64_t>(keys_per, (bp.total + e->max_buckets - 1) / e->max_buckets);
+        const uint64_t nb = (bp.total + keys_per - 1) / keys_per;
+        bp.buckets = nb >= 1 && nb <= e->max_buckets && keys_per <= kSlots ? (uint32_t)nb : 0u;
     }
     uint64_t nodes = bp.kind == PG_KIND_DIRECT ? 0 : nodes_bound(bp.total, bp.dim, bp.lmax) + bp.nseg;
     if (nodes >= (1ull << 32)) return pg_set_error(PG_ERR_ARG, "tree too large for 32-bit nodes");
