@@ -310,6 +310,20 @@ def precision_workload(P, args):
         P.gen_plummer_virial(n2, seed=8, mass=m2, a=0.1, centre=(0.5, 0.0, 0.0), v=(v2, 0.0, 0.0)),
     ])
     eng = P.Engine(0)
+    if args.precision_checkpoints == 0:
+        # steps only (a profiling run): no energy evaluations
+        sim = P.Sim(eng, state, kind="astro2", theta=0.5, e=1e-3, dt=1e-5, precision="mixed")
+        sim.set_near_field(args.near_field)
+        sim.step(2)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        sim.step(args.precision_steps)
+        eng.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / args.precision_steps
+        sim.close()
+        eng.close()
+        return {"metric": "collision steps (no energy)", "bodies": n, "near_field": args.near_field,
+                "steps": args.precision_steps, "ms_per_step": ms}
     res = {}
     for prec in ("mixed", "fp64"):
         sim = P.Sim(eng, state, kind="astro2", theta=0.5, e=1e-3, dt=1e-5, precision=prec)
@@ -410,7 +424,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--precision-bodies", type=int, default=1 << 22)
     ap.add_argument("--precision-steps", type=int, default=1000)
-    ap.add_argument("--precision-checkpoints", type=int, default=2)
+    ap.add_argument("--precision-checkpoints", type=int, default=2,
+                    help="energy checkpoints of the precision workload (0: steps only, no energy)")
     ap.add_argument("--near-field", type=float, default=2.0,
                     help="precision workload: near field in leaf widths (0: the reference's walk)")
     args = ap.parse_args()

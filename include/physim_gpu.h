@@ -188,11 +188,12 @@ typedef struct pg_stats {
 } pg_stats;
 int pg_sim_stats(pg_sim* sim, pg_stats* out);
 /* Direct-sum near field (no reference counterpart; 0, the default, is the
- * reference's walk): a tree cell is opened rather than taken whole when the
- * body lies within leaf_widths widths of its own leaf cell of the cell's box,
- * so the bodies that close contribute their pair forces directly. A
- * deviation from astro2's interaction lists, for the colliding-spheres
- * precision study (2 leaf widths). */
+ * reference's walk): a tree cell at the scale of the body's own leaf cell
+ * (that level and the two below) is opened rather than taken whole when the
+ * body lies within leaf_widths widths of its leaf of the cell's box, so the
+ * bodies that close contribute their pair forces directly. A deviation from
+ * astro2's interaction lists, for the colliding-spheres precision study (2
+ * leaf widths). */
 int pg_sim_set_near_field(pg_sim* sim, double leaf_widths);
 /* Count interactions per body in the force pass (debug; off by default). */
 int pg_sim_set_counting(pg_sim* sim, int enable);

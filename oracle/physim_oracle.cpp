@@ -425,10 +425,13 @@ extern "C" size_t or_tree_node_count(const OrTree* t) { return t->arena[0].has ?
 // order) and optionally the arena index of each contributing node.
 //
 // margin > 0 adds the build's near field (no reference counterpart): a cell
-// is not taken whole while loc lies within `margin` of its box (half-width
-// ext + margin on every axis the tree splits); it is opened instead.
+// at the scale of the body's own leaf (half-width leaf_ext, leaf_ext / 2 or
+// leaf_ext / 4: the leaf's level and the two below) is not taken whole while
+// loc lies within `margin` of its box (half-width ext + margin on every axis
+// the tree splits); it is opened instead.
 static void tree_walk(const OrTree* t, const double loc[3], double theta,
-                      std::vector<const TNode*>& out, double margin = 0.0) {
+                      std::vector<const TNode*>& out, double margin = 0.0,
+                      double leaf_ext = 0.0) {
     std::vector<const TNode*> stack;
     stack.reserve(100);
     stack.push_back(&t->arena[0]);
@@ -439,7 +442,7 @@ static void tree_walk(const OrTree* t, const double loc[3], double theta,
             double d0 = loc[0] - n->c[0], d1 = loc[1] - n->c[1], d2 = loc[2] - n->c[2];
             double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
             bool take = n->ext / r < theta;
-            if (take && margin > 0.0) {
+            if (take && margin > 0.0 && n->ext <= leaf_ext && n->ext >= 0.25 * leaf_ext) {
                 const double reach = n->ext + margin;
                 take = !(fabs(d0) < reach && fabs(d1) < reach && (t->dim == 2 || fabs(d2) < reach));
             }
@@ -873,7 +876,8 @@ extern "C" int or_bh_transform_near(int dim, const Entity* s, size_t n, double t
         double f[3] = {0.0, 0.0, 0.0};
         const double loc[3] = {a.x, a.y, a.z};
         list.clear();
-        tree_walk(t, loc, theta, list, near * (2.0 * leaf_ext(t, loc)));
+        const double lx = leaf_ext(t, loc);
+        tree_walk(t, loc, theta, list, near * (2.0 * lx), lx);
         if (counts) counts[r] = (int64_t)list.size();
         for (const TNode* nb : list) {
             const Entity& b = nb->e;

@@ -1706,8 +1706,8 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
         // the internal nodes that start at body b (shallowest first), then its leaf
         for (; k < ni && span[k].x == b; ++k) {
             const NodeHot& h = hot[k];
-            const int L = (int)h.level;
-            put(L, L > 0 ? (int64_t)h.octant : -1, -1, (int64_t)b, h.mass, h.mx, h.my, h.mz,
+            const int L = h.level();
+            put(L, L > 0 ? (int64_t)h.octant() : -1, -1, (int64_t)b, h.mass, h.mx, h.my, h.mz,
                 h.cx, h.cy, h.cz);
         }
         // the leaf's level, octant and centre: the reference's descent

@@ -87,15 +87,17 @@ struct SegState {
     uint32_t vperm;         // velocities must travel with the state (not yet derivable)
 };
 
-// Tree node record, one 64-byte line (what the walk reads per visited node).
+// Internal tree node record, one 64-byte line (what the walk reads per
+// visited node). Every field is a whole number of 32-bit words, so the walk
+// fetches a record through the scalar unit (which has no 16-bit loads).
 struct NodeHot {
     double cx, cy, cz;      // geometric centre (reference rounding)
-    double mx, my, mz;      // centre of mass (leaf: the body's exact position)
+    double mx, my, mz;      // centre of mass
     double mass;
-    uint32_t skip;          // index of the next node after this subtree
-    uint16_t level;
-    uint8_t octant;         // child index within the parent (0 for a root)
-    uint8_t pad;
+    uint32_t skip;          // index of the next internal node after this subtree
+    uint32_t lvl_oct;       // level (bits 0-15), child index within the parent (bits 16-23)
+    __host__ __device__ int level() const { return (int)(lvl_oct & 0xFFFFu); }
+    __host__ __device__ uint32_t octant() const { return (lvl_oct >> 16) & 0xFFu; }
 };
 
 // Batch-wide counters reset at the start of every step.

@@ -1851,7 +1851,7 @@ static_assert(kTopSlots * (kNodeTile / (kSmallNode + 1)) <= 2 * kNodeQueue && kT
 // (mass, skip, level | octant << 16), or as eight 8-byte words.
 static_assert(sizeof(NodeHot) == 64, "one node record per 64-byte line");
 static_assert(offsetof(NodeHot, mass) == 48 && offsetof(NodeHot, skip) == 56 &&
-                  offsetof(NodeHot, level) == 60 && offsetof(NodeHot, octant) == 62,
+                  offsetof(NodeHot, lvl_oct) == 60,
               "chunk layout of a node record");
 __device__ __forceinline__ uint4 pack2(double a, double b) {
     const uint64_t x = f64_bits(a), y = f64_bits(b);
@@ -2344,6 +2344,9 @@ __device__ __forceinline__ void pair_fp64(double ax, double ay, double az, doubl
 // state, or the acceleration into acc_out (upload order) without it.
 // (eight waves per SIMD: the walk waits on its chain of node loads, and more
 // waves hide more of it; the spilled scalar registers cost less)
+// Levels below a body's leaf at which its near field still opens cells.
+constexpr int kNearLevels = 2;
+
 template <int DIM, bool FP64, bool INTEGRATE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_walk(const SegDesc* __restrict__ desc, uint64_t total,
                                                  const uint64_t* __restrict__ skeys, int segbits,
@@ -2407,12 +2410,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         const double e = d0.e;
         const float ef = (float)e;
         const bool walker = mine && !fixed;
-        // near field (no reference counterpart; off at 0): a cell is opened,
-        // never taken whole, when the body lies within `near` widths of its
-        // own leaf cell of the cell's box, so every body that close adds its
-        // own pair force
+        // near field (no reference counterpart; off at 0): a cell at the
+        // scale of the body's own leaf cell (its level and the kNearLevels
+        // below) is opened, never taken whole, when the body lies within
+        // `near` widths of that leaf cell of the cell's box, so the bodies
+        // that close at that scale add their own pair forces. Larger and
+        // much smaller cells stay under the opening test alone: a body alone
+        // in a large leaf (far out in a halo) would otherwise take the finely
+        // split core beside it body by body.
         const bool nearf = d0.near > 0.0;
-        const double margin = nearf && walker ? d0.near * ldexp(E, 1 - (int)leaf_lv[g]) : 0.0;
+        const int lleaf = nearf && walker ? (int)leaf_lv[g] : 0;
+        const double margin = nearf && walker ? d0.near * ldexp(E, 1 - lleaf) : 0.0;
         uint32_t skip_until = 0;  // a lane skips the items of the bodies below this
         // The walk visits the tree in preorder as two merged sequences: the
         // next item is internal node k when k starts at body b (its span's
@@ -2439,9 +2447,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
                 bool open = false;
                 if (walker && b >= skip_until) {
                     const double d0x = x - h.cx, d0y = y - h.cy, d0z = z - h.cz;
-                    const double hw = ldexp(E, -(int)h.level);
+                    const double hw = ldexp(E, -h.level());
                     bool take = accept_cell(hw, d0x * d0x + d0y * d0y + d0z * d0z, theta);
-                    if (take && nearf) {
+                    if (take && nearf && h.level() >= lleaf && h.level() <= lleaf + kNearLevels) {
                         const double reach = hw + margin;
                         take = !(fabs(d0x) < reach && fabs(d0y) < reach &&
                                  (DIM == 2 || fabs(d0z) < reach));
