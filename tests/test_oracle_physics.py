@@ -8,6 +8,8 @@ two-body potential. Reductions: theta <= 0 Barnes-Hut equals the direct sum
 get no acceleration (transformers.rs:139-141), zero-mass bodies get NaN.
 """
 import math
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -201,3 +203,51 @@ def test_rk4_fixed_bodies_stay_with_zero_velocity():
     e = s.copy()
     O.integrate_steps(e, "euler", "astro2", 1.0, 0.5, 1e-3, 1)
     assert e["x"][-1] == s["x"][-1] + 3.0 * 1e-3 and e["vx"][-1] == 3.0
+
+
+OPT = "/opt/rocm/llvm/bin/opt"
+
+# What rustc emits for `s.powf(0.5)` (astro/src/lib.rs:92-93): a call of the
+# memory-free llvm.pow.f64 intrinsic with a constant exponent.
+POW_HALF_IR = """
+declare double @llvm.pow.f64(double, double)
+define double @r_norm(double %s) {
+  %p = call double @llvm.pow.f64(double %s, double 5.000000e-01)
+  ret double %p
+}
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(OPT), reason="LLVM opt not installed")
+def test_powf_half_is_lowered_to_sqrt(tmp_path):
+    """The reference's r_norm is s.powf(0.5) for a sum of squares s, built at
+    opt-level 3 (astro/Cargo.toml). The oracle takes IEEE sqrt. That is what
+    LLVM's optimizer makes of the intrinsic (fabs(sqrt(s)), +inf for -inf;
+    both equal sqrt(s) for s >= 0), and the choice matters: this libm's pow
+    is off by one ulp on some inputs, and the oracle's force follows sqrt."""
+    src = tmp_path / "pow.ll"
+    src.write_text(POW_HALF_IR)
+    r = subprocess.run([OPT, "-O3", "-S", str(src), "-o", "-"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    body = r.stdout.split("define double @r_norm", 1)[1].split("}", 1)[0]
+    assert "@llvm.sqrt.f64(double %s)" in body and "@llvm.pow" not in body, body
+
+    # a separation whose s has pow(s, 0.5) != sqrt(s) in this libm
+    rng = np.random.default_rng(11)
+    for _ in range(100000):
+        d = rng.uniform(0.1, 2.0, 3)
+        s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+        if math.pow(s, 0.5) != math.sqrt(s):
+            break
+    else:
+        pytest.fail("no input where pow and sqrt differ")
+    a, b = ent(0.0, 0.0, 0.0, 2.0), ent(d[0], d[1], d[2], 3.0)
+    f = O.gravity(a, b, 0.5)
+
+    def force(r_norm):
+        re = s + 0.5
+        return [(d[k] / r_norm) * 1.0 * 2.0 * 3.0 / re for k in range(3)]
+
+    assert list(f) == force(math.sqrt(s))
+    assert list(f) != force(math.pow(s, 0.5))
