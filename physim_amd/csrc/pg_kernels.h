@@ -1917,7 +1917,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     __shared__ uint32_t s_q[kNodeQueue];  // queued internal nodes (see above)
     __shared__ double s_path[kPathLevels][4];  // half-width and centre along the common path
     __shared__ uint64_t s_next;  // key just after the tile
-    __shared__ uint32_t s_topl, s_nq, s_ni;
+    __shared__ uint32_t s_topl, s_nq;
     __shared__ int s_lt;  // levels of the path shared by all the tile's bodies (capped)
     __shared__ Sum4 s_w4[2][kNodeThreads / 64];
     __shared__ uint32_t s_wmin[2][kNodeThreads / 64];
@@ -1936,36 +1936,27 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     const double ext0 = bits_f64(d.st->extent_bits);
     if (threadIdx.x == 0) {
         s_first[nb] = nend;
-        s_next = b1 < total ? skeys_g[b1] : 0ull;  // read only when the segment continues past b1
         s_topl = 0;
         s_nq = 0;
-        s_ni = 0;
-        // every node of the tile lies on or below this path: its cells are
-        // the ancestors of every body
-        const uint64_t ka = nb ? skeys_g[b0] : 0ull, kz = nb ? skeys_g[b1 - 1] : 0ull;
-        const int lt = min(key_lcp<DIM>(ka, kz, lmax), kPathLevels - 1);
-        double e = ext0, x = 0.0, y = 0.0, z = 0.0;
-        for (int l = 0;; ++l) {
-            s_path[l][0] = e;
-            s_path[l][1] = x;
-            s_path[l][2] = y;
-            s_path[l][3] = z;
-            if (l == lt) break;
-            descend<DIM>(ka, 0ull, lmax, l, l + 1, e, x, y, z);
-        }
-        s_lt = lt;
+        if (!nb) s_next = 0ull;
     }
-    // Bodies: staging, checks.
+    // Bodies: staging, checks; in a node-parallel tile each body also lists
+    // its internal nodes at their tile-local places (node order).
     for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
         const uint64_t i = b0 + li;
         const uint64_t key = skeys_g[i];
         const uint32_t lo = lo_level[i];
         const uint32_t f = node_local[i] + n0;
         const uint32_t fn = li + 1u < nb ? node_local[i + 1] + n0 : nend;
+        // the key just after the tile (read only when the segment continues)
+        if (li + 1u == nb) s_next = b1 < total ? skeys_g[b1] : 0ull;
         s_keys[li] = key;
         s_lo[li] = (uint8_t)lo;
         s_first[li] = f;
         first_node[i] = f;
+        if (node_par)
+            for (uint32_t nd = f; nd < min(fn, n1); ++nd)
+                s_ilist[nd - n0] = (uint16_t)(li | ((nd - f) << 10));
         const uint64_t j = i - d.off;
         const double4 p = leafpos[i];
         const double x = p.x, y = p.y, z = p.z, m = p.w;
@@ -1987,6 +1978,23 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         if (d.count && hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(&d.st->max_level, hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // every node of the tile lies on or below this path: its cells are
+        // the ancestors of every body
+        const uint64_t ka = nb ? s_keys[0] : 0ull, kz = nb ? s_keys[nb - 1] : 0ull;
+        const int lt = min(key_lcp<DIM>(ka, kz, lmax), kPathLevels - 1);
+        double e = ext0, x = 0.0, y = 0.0, z = 0.0;
+        for (int l = 0;; ++l) {
+            s_path[l][0] = e;
+            s_path[l][1] = x;
+            s_path[l][2] = y;
+            s_path[l][3] = z;
+            if (l == lt) break;
+            descend<DIM>(ka, 0ull, lmax, l, l + 1, e, x, y, z);
+        }
+        s_lt = lt;
     }
     __syncthreads();
     const int lt = s_lt;
@@ -2081,24 +2089,12 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         emit(nd, li, L, cx, cy, cz, oct);
     };
     if (node_par) {
-        // the bodies list their internal nodes (at most kNodeQueue in a
-        // node-parallel tile, unless the capacity cut it short, an error),
-        // then one thread builds each
-        for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
-            const uint32_t f = s_first[li], fn = s_first[li + 1];
-            const uint32_t ni = (min(fn, n1) > f) ? min(fn, n1) - f : 0u;
-            if (ni) {
-                const uint32_t p = atomicAdd(&s_ni, ni);
-                for (uint32_t j = 0; j < ni && p + j < (uint32_t)kNodeQueue; ++j)
-                    s_ilist[p + j] = (uint16_t)(li | (j << 10));
-            }
-        }
-        __syncthreads();
-        const uint32_t ni = min(s_ni, (uint32_t)kNodeQueue);
-        for (uint32_t q = threadIdx.x; q < ni; q += kNodeThreads) {
+        // one thread per internal node (listed in node order by the bodies:
+        // at most kNodeQueue of them in a node-parallel tile)
+        for (uint32_t q = threadIdx.x; q < nloc; q += kNodeThreads) {
             const uint32_t ent = s_ilist[q];
             const uint32_t li = ent & 1023u, j = ent >> 10;
-            build(s_first[li] + j, li, (int)s_lo[li] + (int)j);
+            build(n0 + q, li, (int)s_lo[li] + (int)j);
         }
     } else {
         // one thread per body: one descent through the body's internal nodes
