@@ -1909,14 +1909,20 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     Sum4* __restrict__ tot_part, const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc,
     const double4* __restrict__ leafpos, uint2* __restrict__ node_span) {
     static_assert(kNodeTile <= 65536, "owner indices are 16-bit");
-    __shared__ uint64_t s_keys[kNodeTile];
+    // s_mask[L][c]: bit b set iff body 64 c + b starts a cell at level L
+    // (lo <= L), for cmin[c] <= L < cmax[c]; no body of chunk c starts one
+    // below cmin[c], and every body does from cmax[c] up
+    __shared__ uint64_t s_mask[kTopSlots - 1][kNodeTile / 64];
+    __shared__ uint8_t s_cmin[kNodeTile / 64], s_cmax[kNodeTile / 64];
     __shared__ uint32_t s_first[kNodeTile + 1];
     __shared__ uint8_t s_lo[kNodeTile];
     __shared__ double s_bm[kNodeTile], s_bx[kNodeTile], s_by[kNodeTile], s_bz[kNodeTile];
     __shared__ uint16_t s_ilist[kNodeQueue];  // a node-parallel tile's internal nodes
     __shared__ uint32_t s_q[kNodeQueue];  // queued internal nodes (see above)
     __shared__ double s_path[kPathLevels][4];  // half-width and centre along the common path
-    __shared__ uint64_t s_next;  // key just after the tile
+    __shared__ uint32_t s_lonext;  // lo of the body just after the tile (0: none, or another segment)
+    // per level L < s_lonext: the last body and the skip of the tile-spanning node
+    __shared__ uint32_t s_span_last[kTopSlots], s_span_skip[kTopSlots];
     __shared__ uint32_t s_topl, s_nq;
     __shared__ int s_lt;  // levels of the path shared by all the tile's bodies (capped)
     __shared__ Sum4 s_w4[2][kNodeThreads / 64];
@@ -1938,52 +1944,89 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         s_first[nb] = nend;
         s_topl = 0;
         s_nq = 0;
-        if (!nb) s_next = 0ull;
+        if (!nb) s_lonext = 0u;
     }
     // Bodies: staging, checks; in a node-parallel tile each body also lists
-    // its internal nodes at their tile-local places (node order).
-    for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
-        const uint64_t i = b0 + li;
-        const uint64_t key = skeys_g[i];
-        const uint32_t lo = lo_level[i];
-        const uint32_t f = node_local[i] + n0;
-        const uint32_t fn = li + 1u < nb ? node_local[i + 1] + n0 : nend;
-        // the key just after the tile (read only when the segment continues)
-        if (li + 1u == nb) s_next = b1 < total ? skeys_g[b1] : 0ull;
-        s_keys[li] = key;
-        s_lo[li] = (uint8_t)lo;
-        s_first[li] = f;
-        first_node[i] = f;
-        if (node_par)
-            for (uint32_t nd = f; nd < min(fn, n1); ++nd)
-                s_ilist[nd - n0] = (uint16_t)(li | ((nd - f) << 10));
-        const uint64_t j = i - d.off;
-        const double4 p = leafpos[i];
-        const double x = p.x, y = p.y, z = p.z, m = p.w;
-        s_bm[li] = m;
-        s_bx[li] = x;
-        s_by[li] = y;
-        s_bz[li] = z;
-        const bool more = j + 1 < d.n;
-        const double4 p1 = more ? leafpos[i + 1] : make_double4(0.0, 0.0, 0.0, 0.0);
-        const double x1 = p1.x, y1 = p1.y, z1 = p1.z;
-        const uint32_t hi = lo + (fn - f);  // the leaf's level
-        if (fn > cap) atomicOr(&d.st->err, kErrCapacity);
-        // the reference merges bodies within 1e-9 on every axis
-        // (octree.rs:69-80); the device reports them instead
-        if (more && fabs(x - x1) < 1e-9 && fabs(y - y1) < 1e-9 && fabs(z - z1) < 1e-9)
-            atomicOr(&d.st->err, kErrCoincident);
-        // read first: once the running maximum is reached almost no lane
-        // needs the atomic (a stale read only costs an extra atomic)
-        if (d.count && hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(&d.st->max_level, hi);
+    // its internal nodes at their tile-local places (node order). Whole
+    // waves take part (a wave holds one chunk of 64 bodies): its start masks.
+    const uint32_t nch = (nb + 63u) >> 6;
+    for (uint32_t base = 0; base < nb; base += kNodeThreads) {
+        const uint32_t li = base + threadIdx.x;
+        uint32_t lo_eff = 255u;  // body 0 of the tile never ends a range in it
+        if (li < nb) {
+            const uint64_t i = b0 + li;
+            const uint32_t lo = lo_level[i];
+            const uint32_t f = node_local[i] + n0;
+            const uint32_t fn = li + 1u < nb ? node_local[i + 1] + n0 : nend;
+            if (li + 1u == nb) s_lonext = b1 < total ? (uint32_t)lo_level[b1] : 0u;
+            if (li > 0) lo_eff = lo;
+            s_lo[li] = (uint8_t)lo;
+            s_first[li] = f;
+            first_node[i] = f;
+            if (node_par)
+                for (uint32_t nd = f; nd < min(fn, n1); ++nd)
+                    s_ilist[nd - n0] = (uint16_t)(li | ((nd - f) << 10));
+            const uint64_t j = i - d.off;
+            const double4 p = leafpos[i];
+            const double x = p.x, y = p.y, z = p.z, m = p.w;
+            s_bm[li] = m;
+            s_bx[li] = x;
+            s_by[li] = y;
+            s_bz[li] = z;
+            const bool more = j + 1 < d.n;
+            const double4 p1 = more ? leafpos[i + 1] : make_double4(0.0, 0.0, 0.0, 0.0);
+            const double x1 = p1.x, y1 = p1.y, z1 = p1.z;
+            const uint32_t hi = lo + (fn - f);  // the leaf's level
+            if (fn > cap) atomicOr(&d.st->err, kErrCapacity);
+            // the reference merges bodies within 1e-9 on every axis
+            // (octree.rs:69-80); the device reports them instead
+            if (more && fabs(x - x1) < 1e-9 && fabs(y - y1) < 1e-9 && fabs(z - z1) < 1e-9)
+                atomicOr(&d.st->err, kErrCoincident);
+            // read first: once the running maximum is reached almost no lane
+            // needs the atomic (a stale read only costs an extra atomic)
+            if (d.count && hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(&d.st->max_level, hi);
+        }
+        const uint32_t c = li >> 6;
+        if ((c << 6) < nb) {
+            uint32_t mn = lo_eff, mx = lo_eff == 255u ? 0u : lo_eff;
+            for (int o = 32; o; o >>= 1) {
+                mn = min(mn, (uint32_t)__shfl_xor((int)mn, o));
+                mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+            }
+            mn = uniform_u32(mn);
+            mx = uniform_u32(mx);
+            for (uint32_t L = mn; L < mx; ++L) {
+                const uint64_t m = __ballot(lo_eff <= L);
+                if ((threadIdx.x & 63u) == 0) s_mask[L][c] = m;
+            }
+            if ((threadIdx.x & 63u) == 0) {
+                s_cmin[c] = (uint8_t)mn;
+                s_cmax[c] = (uint8_t)mx;
+            }
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    // The node of level L < s_lonext that holds the tile's last body runs
+    // into the next tile: it ends before the first later body starting a
+    // node at its level or above (the scanned per-level table), and its skip
+    // is that body's first node. One thread per level, before the build.
+    if (threadIdx.x < (uint32_t)kTopSlots && threadIdx.x < s_lonext && b1 < total) {
+        const uint32_t nx = lvl_first[(uint64_t)(tile + 1) * kTopSlots + threadIdx.x];
+        const uint64_t nxt = nx == ~0u ? total : (uint64_t)nx;
+        s_span_last[threadIdx.x] = (uint32_t)(nxt - 1);
+        s_span_skip[threadIdx.x] =
+            nxt >= total ? tile_base[tiles]
+                         : node_local[nxt] + tile_base[tile_of(desc, seg_off, nseg, nxt)];
+    }
+    // the key of this thread's first node's owner (node-parallel build)
+    const uint64_t key0 =
+        node_par && threadIdx.x < nloc ? skeys_g[b0 + (s_ilist[threadIdx.x] & 1023u)] : 0ull;
+    if (threadIdx.x == 2u * 64u) {
         // every node of the tile lies on or below this path: its cells are
         // the ancestors of every body
-        const uint64_t ka = nb ? s_keys[0] : 0ull, kz = nb ? s_keys[nb - 1] : 0ull;
+        const uint64_t ka = nb ? skeys_g[b0] : 0ull, kz = nb ? skeys_g[b1 - 1] : 0ull;
         const int lt = min(key_lcp<DIM>(ka, kz, lmax), kPathLevels - 1);
         double e = ext0, x = 0.0, y = 0.0, z = 0.0;
         for (int l = 0;; ++l) {
@@ -1999,32 +2042,38 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     __syncthreads();
     const int lt = s_lt;
 
-    // Do keys kj (body j) and key (with second word kx) share `level` levels?
-    auto same_at = [&](uint64_t kj, uint64_t j, uint64_t key, uint64_t kx, int level) -> bool {
-        if (level <= lmax) return key_prefix<DIM>(kj, lmax, level) == key_prefix<DIM>(key, lmax, level);
-        return kj == key && k2_prefix<DIM>(k2[j], lmax, level) == k2_prefix<DIM>(kx, lmax, level);
+    // Chunk c's start mask at level L.
+    auto chunk_mask = [&](uint32_t c, int L) -> uint64_t {
+        if (L < (int)s_cmin[c]) return 0ull;
+        if (L >= (int)s_cmax[c]) return ~0ull;
+        return s_mask[L][c];
     };
-    // Skip of the internal node at level L owned by body i (key): the first
-    // node after its range; its last body, and whether it leaves the tile.
-    auto internal_skip = [&](uint64_t i, uint64_t key, int L, uint64_t& last, bool& spans) -> uint32_t {
-        // levels past lmax exist only for bodies whose keys tie; they read
-        // the second key word (a run of equal keys is a few bodies: linear)
-        const uint64_t kx = L > lmax ? k2[i] : 0ull;
-        last = i;
-        spans = false;
-        if (L <= lmax)
-            last = b0 + range_end_lds<DIM>(s_keys, (uint32_t)(i - b0), nb, lmax, L);
-        else
-            while (last + 1 < b1 && same_at(s_keys[last + 1 - b0], last + 1, key, kx, L)) ++last;
-        if (last + 1 == b1 && b1 < total && same_at(s_next, b1, key, kx, L)) {
-            spans = true;
-            const uint32_t nx = lvl_first[(uint64_t)(tile + 1) * kTopSlots + L];
-            last = (nx == ~0u ? total : (uint64_t)nx) - 1;
-            const uint64_t nxt = last + 1;
-            return nxt >= total ? tile_base[tiles]
-                                : node_local[nxt] + tile_base[tile_of(desc, seg_off, nseg, nxt)];
+    // The first body after local body li that starts a cell at level L
+    // (lo <= L), or nb: the end of li's level-L cell in the tile.
+    auto next_start = [&](uint32_t li, int L) -> uint32_t {
+        const uint32_t j = li + 1u;
+        if (j >= nb) return nb;
+        uint32_t c = j >> 6;
+        uint64_t m = chunk_mask(c, L) & (~0ull << (j & 63u));
+        while (m == 0ull) {
+            if (++c >= nch) return nb;
+            m = chunk_mask(c, L);
         }
-        return s_first[last + 1 - b0];
+        return min((c << 6) + (uint32_t)__ffsll((long long)m) - 1u, nb);
+    };
+    // Skip of the internal node at level L owned by body i: the first node
+    // after its range; its last body, and whether it leaves the tile (the
+    // body after the tile shares the cell).
+    auto internal_skip = [&](uint64_t i, int L, uint64_t& last, bool& spans) -> uint32_t {
+        const uint32_t e = next_start((uint32_t)(i - b0), L);
+        last = b0 + e - 1u;
+        spans = false;
+        if (e == nb && b1 < total && (int)s_lonext > L) {
+            spans = true;
+            last = s_span_last[L];
+            return s_span_skip[L];
+        }
+        return s_first[e];
     };
     // The record of internal node nd (level L, owned by local body li,
     // geometric centre c, octant oct), and its body range. A tile-spanning
@@ -2037,7 +2086,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         bool queued = false;
         uint64_t last;
         bool spans;
-        const uint32_t skip = internal_skip(b0 + li, s_keys[li], L, last, spans);
+        const uint32_t skip = internal_skip(b0 + li, L, last, spans);
         if (spans) {
             top_slot[(uint64_t)tile * kTopSlots + L] = gen | (uint64_t)nd;
             // its last body and that body's tile (the node lies in this segment)
@@ -2079,8 +2128,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
 
     // node nd (level L) of local body li: its centre by the reference's
     // descent (from the tile's common path), then its record
-    auto build = [&](uint32_t nd, uint32_t li, int L) {
-        const uint64_t key = s_keys[li];
+    auto build = [&](uint32_t nd, uint32_t li, int L, uint64_t key) {
         const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
         const int f = min(L, lt);
         double ext = s_path[f][0], cx = s_path[f][1], cy = s_path[f][2], cz = s_path[f][3];
@@ -2094,12 +2142,12 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         for (uint32_t q = threadIdx.x; q < nloc; q += kNodeThreads) {
             const uint32_t ent = s_ilist[q];
             const uint32_t li = ent & 1023u, j = ent >> 10;
-            build(n0 + q, li, (int)s_lo[li] + (int)j);
+            build(n0 + q, li, (int)s_lo[li] + (int)j, q == threadIdx.x ? key0 : skeys_g[b0 + li]);
         }
     } else {
         // one thread per body: one descent through the body's internal nodes
         for (uint32_t li = threadIdx.x; li < nb; li += kNodeThreads) {
-            const uint64_t key = s_keys[li];
+            const uint64_t key = skeys_g[b0 + li];
             const int lo = s_lo[li];
             const uint32_t nbase = s_first[li];
             const int cnt = (int)(s_first[li + 1] - nbase);
@@ -2170,8 +2218,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         // last body (levels <= lam1) that starts in the tile and runs on
         const int mn = (int)min(mns, 255u);
         int lam1 = -1;
-        if (nb && b1 < total && seg_of_key(s_next, segbits) == seg_of_key(s_keys[nb - 1], segbits))
-            lam1 = min(key_lcp2<DIM>(s_keys[nb - 1], s_next, k2, b1 - 1, b1, lmax), kTopSlots - 1);
+        if (nb && s_lonext > 0u) lam1 = min((int)s_lonext - 1, kTopSlots - 1);
         // body li starts the level-L ancestor iff lo[li] <= L < every lo
         // after it
         if (r1 < nb && (int)body_lo(r1) <= lam1 && (int)body_lo(r1) < mn) {
@@ -2209,7 +2256,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             const int L = (int)(ent >> 10);
             uint64_t last;
             bool spans;
-            internal_skip(b0 + li, s_keys[li], L, last, spans);  // (within the tile)
+            internal_skip(b0 + li, L, last, spans);  // (within the tile)
             e = (uint32_t)(last - b0) + 1u;
             nd = s_first[li] + (uint32_t)(L - (int)s_lo[li]);
         }
