@@ -1939,26 +1939,41 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     const bool node_par = nloc <= (uint32_t)kNodeQueue;
     // a node tile lies in one segment
     const SegDesc& d = desc[seg_of(seg_off, nseg, b0)];
-    const double ext0 = bits_f64(d.st->extent_bits);
     if (threadIdx.x == 0) {
         s_first[nb] = nend;
         s_topl = 0;
         s_nq = 0;
-        if (!nb) s_lonext = 0u;
     }
     // Bodies: staging, checks; in a node-parallel tile each body also lists
-    // its internal nodes at their tile-local places (node order). Whole
-    // waves take part (a wave holds one chunk of 64 bodies): its start masks.
+    // its internal nodes at their tile-local places (node order). A thread
+    // stages bodies t and t + kNodeThreads: every load is issued before any
+    // is used. Whole waves take part (a wave holds one chunk of 64 bodies in
+    // each round): its start masks.
+    static_assert(kNodeTile == 2 * kNodeThreads, "two staged bodies per thread");
     const uint32_t nch = (nb + 63u) >> 6;
-    for (uint32_t base = 0; base < nb; base += kNodeThreads) {
-        const uint32_t li = base + threadIdx.x;
+    uint32_t st_lo[2], st_f[2], st_fn[2];
+    double4 st_p[2];
+    // (indices clamped into the tile, a tile holds a body: unconditional
+    // loads, so none waits on another at a branch)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t lc = min((uint32_t)r * kNodeThreads + threadIdx.x, nb - 1u);
+        const uint64_t i = b0 + lc;
+        st_lo[r] = lo_level[i];
+        st_f[r] = node_local[i];
+        st_fn[r] = node_local[lc + 1u < nb ? i + 1 : i];
+        st_p[r] = leafpos[i];
+    }
+    const uint32_t lo_after = lo_level[min((uint64_t)b1, total - 1)];
+    if (threadIdx.x == kNodeThreads - 1u) s_lonext = b1 < total ? lo_after : 0u;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t li = (uint32_t)r * kNodeThreads + threadIdx.x;
         uint32_t lo_eff = 255u;  // body 0 of the tile never ends a range in it
         if (li < nb) {
             const uint64_t i = b0 + li;
-            const uint32_t lo = lo_level[i];
-            const uint32_t f = node_local[i] + n0;
-            const uint32_t fn = li + 1u < nb ? node_local[i + 1] + n0 : nend;
-            if (li + 1u == nb) s_lonext = b1 < total ? (uint32_t)lo_level[b1] : 0u;
+            const uint32_t lo = st_lo[r], f = st_f[r] + n0;
+            const uint32_t fn = li + 1u < nb ? st_fn[r] + n0 : nend;
             if (li > 0) lo_eff = lo;
             s_lo[li] = (uint8_t)lo;
             s_first[li] = f;
@@ -1966,22 +1981,12 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             if (node_par)
                 for (uint32_t nd = f; nd < min(fn, n1); ++nd)
                     s_ilist[nd - n0] = (uint16_t)(li | ((nd - f) << 10));
-            const uint64_t j = i - d.off;
-            const double4 p = leafpos[i];
-            const double x = p.x, y = p.y, z = p.z, m = p.w;
-            s_bm[li] = m;
-            s_bx[li] = x;
-            s_by[li] = y;
-            s_bz[li] = z;
-            const bool more = j + 1 < d.n;
-            const double4 p1 = more ? leafpos[i + 1] : make_double4(0.0, 0.0, 0.0, 0.0);
-            const double x1 = p1.x, y1 = p1.y, z1 = p1.z;
+            s_bm[li] = st_p[r].w;
+            s_bx[li] = st_p[r].x;
+            s_by[li] = st_p[r].y;
+            s_bz[li] = st_p[r].z;
             const uint32_t hi = lo + (fn - f);  // the leaf's level
             if (fn > cap) atomicOr(&d.st->err, kErrCapacity);
-            // the reference merges bodies within 1e-9 on every axis
-            // (octree.rs:69-80); the device reports them instead
-            if (more && fabs(x - x1) < 1e-9 && fabs(y - y1) < 1e-9 && fabs(z - z1) < 1e-9)
-                atomicOr(&d.st->err, kErrCoincident);
             // read first: once the running maximum is reached almost no lane
             // needs the atomic (a stale read only costs an extra atomic)
             if (d.count && hi > __hip_atomic_load(&d.st->max_level, __ATOMIC_RELAXED,
@@ -2020,6 +2025,20 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             nxt >= total ? tile_base[tiles]
                          : node_local[nxt] + tile_base[tile_of(desc, seg_off, nseg, nxt)];
     }
+    // the reference merges bodies within 1e-9 on every axis
+    // (octree.rs:69-80); the device reports them instead (the tile's last
+    // body against the next body of its segment, read from memory)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t li = (uint32_t)r * kNodeThreads + threadIdx.x;
+        if (li < nb && b0 + li + 1u < d.off + d.n) {
+            const double4 q = li + 1u < nb ? make_double4(s_bx[li + 1], s_by[li + 1], s_bz[li + 1], 0.0)
+                                           : leafpos[b1];
+            if (fabs(st_p[r].x - q.x) < 1e-9 && fabs(st_p[r].y - q.y) < 1e-9 &&
+                fabs(st_p[r].z - q.z) < 1e-9)
+                atomicOr(&d.st->err, kErrCoincident);
+        }
+    }
     // the key of this thread's first node's owner (node-parallel build)
     const uint64_t key0 =
         node_par && threadIdx.x < nloc ? skeys_g[b0 + (s_ilist[threadIdx.x] & 1023u)] : 0ull;
@@ -2028,7 +2047,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
         // the ancestors of every body
         const uint64_t ka = nb ? skeys_g[b0] : 0ull, kz = nb ? skeys_g[b1 - 1] : 0ull;
         const int lt = min(key_lcp<DIM>(ka, kz, lmax), kPathLevels - 1);
-        double e = ext0, x = 0.0, y = 0.0, z = 0.0;
+        double e = bits_f64(d.st->extent_bits), x = 0.0, y = 0.0, z = 0.0;
         for (int l = 0;; ++l) {
             s_path[l][0] = e;
             s_path[l][1] = x;
