@@ -2404,13 +2404,15 @@ __device__ __forceinline__ void pair_fp64(double ax, double ay, double az, doubl
 // through its skip index; the wave descends into a node iff some lane opened
 // it. Then the Stormer update (verlet.rs:24-82) in place on the permuted
 // state, or the acceleration into acc_out (upload order) without it.
-// (eight waves per SIMD: the walk waits on its chain of node loads, and more
-// waves hide more of it; the spilled scalar registers cost less)
+// (a waves-per-EU hint of 6: the three records in flight fit the scalar
+// registers with almost no spills, at seven waves per SIMD; a hint of 8 spilled
+// 34 scalar registers into vector lanes, whose reloads are VALU instructions
+// in every iteration and cost more than the eighth wave hid)
 // Levels below a body's leaf at which its near field still opens cells.
 constexpr int kNearLevels = 2;
 
 template <int DIM, bool FP64, bool INTEGRATE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_walk(const SegDesc* __restrict__ desc, uint64_t total,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_walk(const SegDesc* __restrict__ desc, uint64_t total,
                                                  const uint64_t* __restrict__ skeys, int segbits,
                                                  const uint32_t* __restrict__ first_node,
                                                  const NodeHot* __restrict__ hot, uint64_t cap,
