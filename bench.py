@@ -201,11 +201,15 @@ def served_workload(P, args):
     ids, k, active, queued = [], 0, 0, 0
     while k < len(reqs) or active or queued:
         now = time.perf_counter() - w0
+        submitted = False
         while k < len(reqs) and reqs[k]["arrival"] <= now:
             r = reqs[k]
             ids.append(srv.submit(r["state"], r["steps"], theta=r["theta"], **kw))
             k += 1
-        if not active and not queued and k < len(reqs):
+            submitted = True
+        # idle only when nothing is on the server (a request just submitted
+        # is admitted by the next tick)
+        if not submitted and not active and not queued and k < len(reqs):
             time.sleep(max(0.0, reqs[k]["arrival"] - (time.perf_counter() - w0)))
             continue
         active, queued = srv.tick(args.tick_steps)
