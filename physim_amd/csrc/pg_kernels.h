@@ -25,6 +25,15 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
+// Element idx of an array, its byte offset formed in 32 bits (idx * sizeof(T)
+// below 2^32): a uniform load then takes the offset in a scalar register
+// instead of a 64-bit address add per load.
+template <class T>
+__device__ __forceinline__ const T& at32(const T* __restrict__ base, uint32_t idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+
 // Segment of global index g: largest s with seg_off[s] <= g (empty segments
 // share an offset with their successor and are skipped).
 __device__ __forceinline__ uint32_t seg_of(const uint64_t* __restrict__ seg_off, uint32_t nseg,
@@ -2495,18 +2504,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         // kend is never used). Each step raises b or k (guarded against a
         // corrupt record), so the loop ends.
         const uint32_t clamp = (uint32_t)(cap - 1);
-        NodeHot h = hot[min(k, clamp)];
-        uint2 sp = node_span[min(k, clamp)];
+        NodeHot h = at32(hot, min(k, clamp));
+        uint2 sp = at32(node_span, min(k, clamp));
         while (b < bend) {
             // this item's source (a taken node's centre of mass and mass, or
             // a leaf's body) and whether this lane adds it
             double bx, by, bz, bm;
             bool add = false;
             if (k < kend && sp.x == b) {
-                const NodeHot h_open = hot[min(k + 1u, clamp)];
-                const uint2 sp_open = node_span[min(k + 1u, clamp)];
-                const NodeHot h_skip = hot[min(h.skip, clamp)];
-                const uint2 sp_skip = node_span[min(h.skip, clamp)];
+                const NodeHot h_open = at32(hot, min(k + 1u, clamp));
+                const uint2 sp_open = at32(node_span, min(k + 1u, clamp));
+                const NodeHot h_skip = at32(hot, min(h.skip, clamp));
+                const uint2 sp_skip = at32(node_span, min(h.skip, clamp));
                 bx = h.mx; by = h.my; bz = h.mz; bm = h.mass;
                 bool open = false;
                 if (walker && b >= skip_until) {
@@ -2534,7 +2543,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 }
             } else {
                 // the leaf of body b: its exact position and mass
-                const double4 lp = leafpos[b];
+                const double4 lp = at32(leafpos, b);
                 bx = lp.x; by = lp.y; bz = lp.z; bm = lp.w;
                 add = walker && b >= skip_until;
                 b = uniform_u32(b + 1u);
