@@ -182,7 +182,7 @@ typedef struct pg_stats {
     uint64_t nodes;        /* tree nodes of the last build */
     uint64_t interactions; /* interaction-list entries of the last force pass */
     uint32_t max_level;    /* deepest leaf level of the last build */
-    uint32_t key_levels;   /* levels the Morton key holds */
+    uint32_t key_levels;   /* tree levels the keys resolve: two bodies share at most key_levels - 1 */
     uint32_t error;        /* sticky device-side pg_status of this sim */
     uint32_t reserved;
 } pg_stats;

@@ -96,7 +96,7 @@ struct pg_engine {
     uint32_t* tile_base = nullptr;   // first node of each tile (+ total)
     uint32_t* tile_start = nullptr;  // first body of each node tile (+ total)
     uint32_t* lvl_first = nullptr;   // per tile and level: first node-starting body, scanned
-    uint64_t* k2 = nullptr;          // second key words of bodies whose keys tie
+    DeepKey* deep = nullptr;         // deep keys of bodies whose first key words tie
     NodeHot* hot = nullptr;
     double* acc_out = nullptr;
     SegDesc* desc[2] = {nullptr, nullptr};
@@ -199,12 +199,13 @@ static uint64_t state_bytes(uint64_t n) {
            align256(n * 8) + 256;
 }
 
-// Nodes of a tree over n bodies with keys of 2 * lmax levels (a second key
-// word for bodies whose first words tie): n leaves plus the total ascent of
-// the common-prefix sequence, at most half its total variation.
+// Nodes of a tree over n bodies whose neighbours share at most kMaxLcp
+// levels: n leaves plus the total ascent of the common-prefix sequence, at
+// most half its total variation.
 static uint64_t nodes_bound(uint64_t n, int dim, int lmax) {
     (void)dim;
-    return n + (n * (uint64_t)(2 * lmax) + 1) / 2 + 1;
+    (void)lmax;
+    return n + (n * (uint64_t)(kMaxLcp + 1) + 1) / 2 + 1;
 }
 
 extern "C" uint64_t pg_sim_bytes(uint64_t n, int32_t kind) {
@@ -438,7 +439,7 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     const uint64_t o_ip = take((node_tiles + 2) * kTopSlots * sizeof(Sum4));
     const uint64_t o_tp = take((node_tiles + 2) * sizeof(Sum4));
     const uint64_t o_lf = take((node_tiles + 2) * kTopSlots * 4);
-    const uint64_t o_k2 = take(bodies * 8);
+    const uint64_t o_dk = take(bodies * sizeof(DeepKey));
     const uint64_t o_hot = take(nodes * sizeof(NodeHot));
     const uint64_t o_span = take(nodes * sizeof(uint2)), o_lp = take(bodies * sizeof(double4));
     const uint64_t o_acc = take(bodies * 24);
@@ -473,7 +474,7 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     e->tile_base = (uint32_t*)(b + o_tb);
     e->tile_start = (uint32_t*)(b + o_ts);
     e->lvl_first = (uint32_t*)(b + o_lf);
-    e->k2 = (uint64_t*)(b + o_k2);
+    e->deep = (DeepKey*)(b + o_dk);
     e->hot = (NodeHot*)(b + o_hot);
     e->node_span = (uint2*)(b + o_span);
     e->leafpos = (double4*)(b + o_lp);
@@ -673,7 +674,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         launch(e, cap, "bucket_sort", [&] {
             k_bucket_sort<DIM><<<nb, kBucketThreads, 0, st>>>(
                 e->keys[1], e->vals[1], e->keys[0], e->vals[0], e->bcount, e->bfill, e->bbase,
-                e->bucket_cap, desc, total, e->k2, bp.nseg, bp.segbits, bp.lmax, e->quant, e->sc);
+                e->bucket_cap, desc, total, e->deep, bp.nseg, bp.segbits, bp.lmax, e->quant, e->sc);
         });
     } else {
         for (int p = 0; p < bp.passes; ++p) {
@@ -699,13 +700,13 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
     if (!bp.buckets)
         launch(e, cap, "ties", [&] {
             k_ties<DIM><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
-                desc, total, skeys, svals, e->k2, bp.nseg, bp.segbits, bp.lmax);
+                desc, total, skeys, svals, e->deep, bp.nseg, bp.segbits, bp.lmax);
         });
     const unsigned tiles = bp.ntiles;
     launch(e, cap, "scan_gather", [&] {
         k_scan_gather<DIM><<<tiles, kBlock, 0, st>>>(
             desc, e->tile_start, e->ext_partial, bp.nchunks, skeys, svals, bp.nseg, bp.segbits, bp.lmax, e->node_local,
-            e->tile_tot, e->lo_level, e->lvl_first, e->k2, e->err_global, e->leaf_lv, e->leafpos);
+            e->tile_tot, e->lo_level, e->lvl_first, e->deep, e->err_global, e->leaf_lv, e->leafpos);
     });
     launch(e, cap, "tree_scan", [&] {
         k_tree_scan<<<1 + kTopSlots, kScanThreads, 0, st>>>(e->tile_tot, e->tile_base, tiles,
@@ -715,7 +716,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         k_nodes<DIM><<<tiles, kNodeThreads, 0, st>>>(
             desc, e->seg_off, e->tile_start, total, skeys, bp.nseg, bp.segbits, bp.lmax,
             e->node_local, e->tile_base, e->lvl_first, e->first_node, e->lo_level, e->hot,
-            e->cap_nodes, e->top_slot, e->top_last, e->out_part, e->in_part, e->tot_part, e->k2,
+            e->cap_nodes, e->top_slot, e->top_last, e->out_part, e->in_part, e->tot_part, e->deep,
             e->sc, e->leafpos, e->node_span);
     });
     launch(e, cap, "com_span", [&] {
@@ -1616,7 +1617,7 @@ extern "C" int pg_sim_stats(pg_sim* s, pg_stats* out) {
     out->interactions = st.interactions;
     out->max_level = st.max_level;
     out->error = map_err(st.err);
-    out->key_levels = 2u * (uint32_t)key_lmax(key_dim(s->p.kind), 0);
+    out->key_levels = (uint32_t)kMaxLcp + 1u;
     if (e->last_sim == s) {
         // internal nodes stored, plus one leaf per body
         uint32_t ni = 0;
@@ -1667,7 +1668,8 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
     std::vector<uint32_t> id(n);
     std::vector<uint8_t> lo(n);
     std::vector<double4> lp(n);
-    std::vector<uint64_t> keys(n), k2;
+    std::vector<uint64_t> keys(n);
+    std::vector<DeepKey> deep;
     if (ni) {
         HIP_TRY(hipMemcpy(hot.data(), e->hot, ni * sizeof(NodeHot), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(span.data(), e->node_span, ni * sizeof(uint2), hipMemcpyDeviceToHost));
@@ -1714,14 +1716,16 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
         // (octree.rs:160-215 / quadtree.rs:161-194) along the body's key, the
         // same operations as the device's
         const int L = (int)lo[b] + (int)(fn[b + 1] - fn[b]);
-        if (L > lmax && k2.empty()) {
-            k2.resize(n);
-            HIP_TRY(hipMemcpy(k2.data(), e->k2, n * 8, hipMemcpyDeviceToHost));
+        if (L > lmax && deep.empty()) {
+            deep.resize(n);
+            HIP_TRY(hipMemcpy(deep.data(), e->deep, n * sizeof(DeepKey), hipMemcpyDeviceToHost));
         }
-        const uint64_t key = keys[b], kx = L > lmax ? k2[b] : 0ull;
+        const uint64_t key = keys[b];
+        const int wl = dim == 3 ? 21 : 32;  // deep key levels per word
         auto digit = [&](int l) {
-            return l < lmax ? (uint32_t)((key >> (dim * (lmax - 1 - l))) & mask)
-                            : (uint32_t)((kx >> (dim * (2 * lmax - 1 - l))) & mask);
+            if (l < lmax) return (uint32_t)((key >> (dim * (lmax - 1 - l))) & mask);
+            const int t = l - lmax;
+            return (uint32_t)((deep[b].w[t / wl] >> (dim * (wl - 1 - t % wl))) & mask);
         };
         double ext = E, c[3] = {0.0, 0.0, 0.0};
         for (int l = 0; l < L; ++l) {

@@ -31,7 +31,17 @@ constexpr int kNodeThreads = 512;  // threads per node-building block (two bodie
 constexpr int kNodeLds = 1792;     // kNodeTile bodies + up to kNodeLds - kNodeTile internal nodes: built one
                                    // node per thread (more internal nodes: body by body)
 constexpr int kTopSlots = 65;      // tree levels 0..64: one tile-spanning node each
-constexpr int kMaxTieRun = 256;    // longest run of equal keys ordered by the second key word
+constexpr int kMaxTieRun = 256;    // longest run of equal keys ordered by their deep keys
+constexpr int kMaxLcp = 64;        // deepest common prefix, in levels, of two bodies: the
+                                   // reference recurses to depth 64 (octree.rs:60-62)
+constexpr int kDeepWords = 3;      // words of a tied body's deep key
+static_assert(kMaxLcp + 1 == kTopSlots, "every level of an internal node has its slot");
+
+// Deep key of a body whose first key word ties with a neighbour's: the
+// digits of the levels past the first word, from the same descent.
+struct DeepKey {
+    uint64_t w[kDeepWords];
+};
 constexpr int kComSpanGroups = 5;   // level groups (of 4 waves) per tile in k_com_span
 static_assert(kScanTile == kNodeTile, "node offsets and node building share their tiles");
 static_assert(kNodeTile == 2 * kNodeThreads, "the partial sums take two bodies per thread");

@@ -77,31 +77,50 @@ __device__ __forceinline__ int key_lcp(uint64_t a, uint64_t b, int lmax) {
     return lz / DIM;
 }
 
-// Second key word. Bodies whose keys tie (they share every one of the lmax
-// levels a key holds) carry k2: the next lmax levels of the same descent,
-// level lmax + t in bits [DIM*(lmax-1-t), DIM*(lmax-t)). The tree then holds
-// up to 2*lmax levels; k2 is defined for tied bodies only.
+// Deep key. Bodies whose first words tie (they share every one of the lmax
+// levels a key holds) carry the next levels of the same descent, up to the
+// deepest level two bodies may share (kMaxLcp): kWordLevels levels per word,
+// level lmax + t in word t / W at bits [DIM (W - 1 - t % W), DIM (W - t % W)).
+// Defined for tied bodies only.
 template <int DIM>
-__device__ __forceinline__ uint64_t k2_prefix(uint64_t k2, int lmax, int level) {
-    const int sh = DIM * (2 * lmax - level);
-    return sh >= 64 ? 0ull : k2 >> sh;
+constexpr int kWordLevels = DIM == 3 ? 21 : 32;
+static_assert(21 + kDeepWords * 21 > kMaxLcp && 32 + kDeepWords * 32 > kMaxLcp,
+              "a first word and the deep key resolve every level a prefix may reach");
+
+template <int DIM>
+__device__ __forceinline__ uint32_t deep_digit(const DeepKey& k, int t) {
+    constexpr int W = kWordLevels<DIM>;
+    return (uint32_t)((k.w[t / W] >> (DIM * (W - 1 - t % W))) & ((1u << DIM) - 1u));
 }
 
 // Octant digit taken at tree depth d (from level d to level d + 1).
 template <int DIM>
-__device__ __forceinline__ uint32_t key_digit(uint64_t key, uint64_t k2, int lmax, int d) {
+__device__ __forceinline__ uint32_t key_digit(uint64_t key, const DeepKey& dk, int lmax, int d) {
     const uint64_t dmask = (1u << DIM) - 1u;
     return d < lmax ? (uint32_t)((key >> (DIM * (lmax - 1 - d))) & dmask)
-                    : (uint32_t)((k2 >> (DIM * (2 * lmax - 1 - d))) & dmask);
+                    : deep_digit<DIM>(dk, d - lmax);
+}
+
+// Common prefix length, in levels, of two deep keys.
+template <int DIM>
+__device__ __forceinline__ int deep_lcp(const DeepKey& a, const DeepKey& b) {
+    constexpr int W = kWordLevels<DIM>;
+    int l = 0;
+    for (int i = 0; i < kDeepWords; ++i) {
+        const int c = key_lcp<DIM>(a.w[i], b.w[i], W);
+        l += c;
+        if (c < W) break;
+    }
+    return l;
 }
 
 // Common prefix length in levels of bodies ia and ib (keys a, b), reading
-// the second words only when the first ones are equal.
+// the deep keys only when the first words are equal.
 template <int DIM>
-__device__ __forceinline__ int key_lcp2(uint64_t a, uint64_t b, const uint64_t* __restrict__ k2,
+__device__ __forceinline__ int key_lcp2(uint64_t a, uint64_t b, const DeepKey* __restrict__ dk,
                                         uint64_t ia, uint64_t ib, int lmax) {
     const int l = key_lcp<DIM>(a, b, lmax);
-    return l < lmax ? l : lmax + key_lcp<DIM>(k2[ia], k2[ib], lmax);
+    return l < lmax ? l : lmax + deep_lcp<DIM>(dk[ia], dk[ib]);
 }
 
 // DPP move across a wave's lanes (CTRL: the DPP control, ROWS: the rows
@@ -596,13 +615,21 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
 // The reference's tree keeps splitting bodies that share every level a key
 // holds (octree.rs:92-125, down to depth 64). After the sort, each run of
 // equal keys (in practice two or three bodies of a dense core) gets the
-// next lmax levels of the same descent as second key words, and is put in
-// (second word, index) order by the run's first thread (insertion sort).
-// Equal second words, or a run longer than kMaxTieRun, are a depth error.
+// levels past the first word of the same descent as deep keys, and is put
+// in (deep key, index) order by the run's first thread (insertion sort).
+// Bodies that share more than kMaxLcp levels, or a run longer than
+// kMaxTieRun, are a depth error (the reference panics past depth 64).
+template <int DIM>
+__device__ __forceinline__ bool deep_less(const DeepKey& a, uint32_t va, const DeepKey& b,
+                                          uint32_t vb) {
+    for (int i = 0; i < kDeepWords; ++i)
+        if (a.w[i] != b.w[i]) return a.w[i] < b.w[i];
+    return va < vb;
+}
 template <int DIM>
 __device__ void order_tie_run(const SegDesc* __restrict__ desc, uint64_t total,
                               const uint64_t* __restrict__ skeys, uint32_t* __restrict__ svals,
-                              uint64_t* __restrict__ k2, uint32_t nseg, int segbits, int lmax,
+                              DeepKey* __restrict__ deep, uint32_t nseg, int segbits, int lmax,
                               uint64_t g) {
     if (g + 1 >= total) return;
     const uint64_t key = skeys[g];
@@ -617,6 +644,9 @@ __device__ void order_tie_run(const SegDesc* __restrict__ desc, uint64_t total,
         return;
     }
     const double E = bits_f64(d.st->extent_bits);
+    constexpr int W = kWordLevels<DIM>;
+    // the descent down to the first level no two bodies may share
+    const int deepest = min(kMaxLcp + 1, lmax + kDeepWords * W);
     for (uint64_t j = g; j <= r; ++j) {
         const uint64_t src = (uint64_t)svals[j] - d.off;
         if (src >= d.n) {
@@ -625,8 +655,9 @@ __device__ void order_tie_run(const SegDesc* __restrict__ desc, uint64_t total,
         }
         const double px = d.sx[src], py = d.sy[src], pz = d.sz[src];
         double ext = E, cx = 0.0, cy = 0.0, cz = 0.0;
-        uint64_t kk = 0;
-        for (int l = 0; l < 2 * lmax; ++l) {
+        DeepKey dk;
+        for (int i = 0; i < kDeepWords; ++i) dk.w[i] = 0;
+        for (int l = 0; l < deepest; ++l) {
             uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
             if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
             const double h2 = half(ext);
@@ -634,20 +665,23 @@ __device__ void order_tie_run(const SegDesc* __restrict__ desc, uint64_t total,
             cy = (o & 2u) ? cy + h2 : cy - h2;
             if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
             ext = h2;
-            if (l >= lmax) kk = (kk << DIM) | o;
+            if (l >= lmax) {
+                const int t = l - lmax;
+                dk.w[t / W] |= (uint64_t)o << (DIM * (W - 1 - t % W));
+            }
         }
-        k2[j] = kk;
+        deep[j] = dk;
     }
     for (uint64_t j = g + 1; j <= r; ++j) {
-        const uint64_t kv = k2[j];
+        const DeepKey kv = deep[j];
         const uint32_t vv = svals[j];
         uint64_t m = j;
-        while (m > g && (k2[m - 1] > kv || (k2[m - 1] == kv && svals[m - 1] > vv))) {
-            k2[m] = k2[m - 1];
+        while (m > g && deep_less<DIM>(kv, vv, deep[m - 1], svals[m - 1])) {
+            deep[m] = deep[m - 1];
             svals[m] = svals[m - 1];
             --m;
         }
-        k2[m] = kv;
+        deep[m] = kv;
         svals[m] = vv;
     }
 }
@@ -658,9 +692,9 @@ template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ desc, uint64_t total,
                                                  const uint64_t* __restrict__ skeys,
                                                  uint32_t* __restrict__ svals,
-                                                 uint64_t* __restrict__ k2, uint32_t nseg,
+                                                 DeepKey* __restrict__ deep, uint32_t nseg,
                                                  int segbits, int lmax) {
-    order_tie_run<DIM>(desc, total, skeys, svals, k2, nseg, segbits, lmax,
+    order_tie_run<DIM>(desc, total, skeys, svals, deep, nseg, segbits, lmax,
                        (uint64_t)blockIdx.x * kBlock + threadIdx.x);
 }
 
@@ -1362,7 +1396,7 @@ __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(
     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, uint64_t* __restrict__ tkeys,
     uint32_t* __restrict__ tvals, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bfill,
     const uint32_t* __restrict__ bbase, uint32_t cap, const SegDesc* __restrict__ desc,
-    uint64_t total, uint64_t* __restrict__ k2, uint32_t nseg, int segbits, int lmax,
+    uint64_t total, DeepKey* __restrict__ deep, uint32_t nseg, int segbits, int lmax,
     const uint64_t* __restrict__ quant, const StepCounters* __restrict__ sc) {
     constexpr int kWaves = kBucketThreads / 64;
     static_assert(kBucketCap == 8 * kBucketThreads, "up to eight keys per thread");
@@ -1405,7 +1439,7 @@ __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(
         }
         __syncthreads();
         for (uint32_t i = tid; i < cnt; i += kBucketThreads)
-            order_tie_run<DIM>(desc, total, keys, vals, k2, nseg, segbits, lmax, base + i);
+            order_tie_run<DIM>(desc, total, keys, vals, deep, nseg, segbits, lmax, base + i);
         return;
     }
     // HBM: stable LSD passes over the range, chunk by chunk in order
@@ -1461,7 +1495,7 @@ __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(
     }
     __syncthreads();
     for (uint32_t i = tid; i < cnt; i += kBucketThreads)
-        order_tie_run<DIM>(desc, total, keys, vals, k2, nseg, segbits, lmax, base + i);
+        order_tie_run<DIM>(desc, total, keys, vals, deep, nseg, segbits, lmax, base + i);
 }
 
 // Per sorted body: its common-prefix lengths with its neighbours give the
@@ -1487,7 +1521,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
     const uint32_t* __restrict__ svals, uint32_t nseg, int segbits, int lmax,
     uint32_t* __restrict__ node_local, uint32_t* __restrict__ tile_tot,
     uint8_t* __restrict__ lo_level, uint32_t* __restrict__ lvl_first,
-    const uint64_t* __restrict__ k2, uint32_t* err, uint8_t* __restrict__ leaf_lv,
+    const DeepKey* __restrict__ deep, uint32_t* err, uint8_t* __restrict__ leaf_lv,
     double4* __restrict__ leafpos) {
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_cnt[kScanTile];
@@ -1521,9 +1555,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
             } else {
                 const SegDesc& d = desc[s];
                 const uint64_t end = d.off + d.n;
-                const int depth = 2 * lmax;
-                int lcpL = i > d.off ? key_lcp2<DIM>(skeys[i - 1], key, k2, i - 1, i, lmax) : -1;
-                int lcpR = i + 1 < end ? key_lcp2<DIM>(key, skeys[i + 1], k2, i, i + 1, lmax) : -1;
+                // two bodies share at most kMaxLcp levels (the reference
+                // panics past depth 64, octree.rs:60-62)
+                const int depth = kMaxLcp + 1;
+                int lcpL = i > d.off ? key_lcp2<DIM>(skeys[i - 1], key, deep, i - 1, i, lmax) : -1;
+                int lcpR = i + 1 < end ? key_lcp2<DIM>(key, skeys[i + 1], deep, i, i + 1, lmax) : -1;
                 if (lcpR >= depth) {
                     atomicOr(&d.st->err, kErrDepth);
                     lcpR = depth - 1;
@@ -1554,7 +1590,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
             lo_level[i] = (uint8_t)lo;
             // by sorted position, read by a walk with a near field
             if (s < nseg && desc[s].near > 0.0) leaf_lv[i] = (uint8_t)leaf;
-            atomicMin(&s_firstlo[lo], (uint32_t)i);
+            // (a body below every internal level, lo = kMaxLcp + 1, starts none)
+            if (lo < kTopSlots) atomicMin(&s_firstlo[lo], (uint32_t)i);
         }
         s_cnt[li] = c;
     }
@@ -1825,15 +1862,18 @@ __device__ __forceinline__ void descend_w(uint64_t w, int n, double& ext, double
     }
 }
 // Levels [from, to) of a body's path: the first key word's digits, then the
-// second's (levels past lmax, bodies whose first words tie).
+// deep key's (levels past lmax, bodies whose first words tie).
 template <int DIM>
-__device__ __forceinline__ void descend(uint64_t key, uint64_t kx, int lmax, int from, int to,
+__device__ __forceinline__ void descend(uint64_t key, const DeepKey& dk, int lmax, int from, int to,
                                         double& ext, double& cx, double& cy, double& cz) {
     const int a = min(to, lmax);
     if (from < a) descend_w<DIM>(~key << (64 - DIM * (lmax - from)), a - from, ext, cx, cy, cz);
-    if (to > lmax) {
-        const int f2 = max(from, lmax);
-        descend_w<DIM>(~kx << (64 - DIM * (2 * lmax - f2)), to - f2, ext, cx, cy, cz);
+    constexpr int W = kWordLevels<DIM>;
+    for (int l = max(from, lmax); l < to;) {
+        const int t = l - lmax, p = t % W;
+        const int n = min(to - l, W - p);
+        descend_w<DIM>(~dk.w[t / W] << (64 - DIM * (W - p)), n, ext, cx, cy, cz);
+        l += n;
     }
 }
 // Levels of a tile's common path kept per tile (the rest descend from there).
@@ -1915,13 +1955,13 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     uint32_t* __restrict__ first_node, const uint8_t* __restrict__ lo_level,
     NodeHot* __restrict__ hot, uint64_t cap, uint64_t* __restrict__ top_slot,
     uint2* __restrict__ top_last, Sum4* __restrict__ out_part, Sum4* __restrict__ in_part,
-    Sum4* __restrict__ tot_part, const uint64_t* __restrict__ k2, StepCounters* __restrict__ sc,
+    Sum4* __restrict__ tot_part, const DeepKey* __restrict__ deep, StepCounters* __restrict__ sc,
     const double4* __restrict__ leafpos, uint2* __restrict__ node_span) {
     static_assert(kNodeTile <= 65536, "owner indices are 16-bit");
     // s_mask[L][c]: bit b set iff body 64 c + b starts a cell at level L
     // (lo <= L), for cmin[c] <= L < cmax[c]; no body of chunk c starts one
     // below cmin[c], and every body does from cmax[c] up
-    __shared__ uint64_t s_mask[kTopSlots - 1][kNodeTile / 64];
+    __shared__ uint64_t s_mask[kTopSlots][kNodeTile / 64];
     __shared__ uint8_t s_cmin[kNodeTile / 64], s_cmax[kNodeTile / 64];
     __shared__ uint32_t s_first[kNodeTile + 1];
     __shared__ uint8_t s_lo[kNodeTile];
@@ -2063,7 +2103,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             s_path[l][2] = y;
             s_path[l][3] = z;
             if (l == lt) break;
-            descend<DIM>(ka, 0ull, lmax, l, l + 1, e, x, y, z);
+            descend<DIM>(ka, DeepKey{}, lmax, l, l + 1, e, x, y, z);  // (lt < lmax)
         }
         s_lt = lt;
     }
@@ -2157,7 +2197,8 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     // node nd (level L) of local body li: its centre by the reference's
     // descent (from the tile's common path), then its record
     auto build = [&](uint32_t nd, uint32_t li, int L, uint64_t key) {
-        const uint64_t kx = L > lmax ? k2[b0 + li] : 0ull;
+        DeepKey kx{};
+        if (L > lmax) kx = deep[b0 + li];
         const int f = min(L, lt);
         double ext = s_path[f][0], cx = s_path[f][1], cy = s_path[f][2], cz = s_path[f][3];
         descend<DIM>(key, kx, lmax, f, L, ext, cx, cy, cz);
@@ -2180,7 +2221,8 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             const uint32_t nbase = s_first[li];
             const int cnt = (int)(s_first[li + 1] - nbase);
             const int hi = lo + cnt - 1;
-            const uint64_t kx = hi > lmax ? k2[b0 + li] : 0ull;
+            DeepKey kx{};
+            if (hi > lmax) kx = deep[b0 + li];
             const int f = min(lo, lt);
             double ext = s_path[f][0], cx = s_path[f][1], cy = s_path[f][2], cz = s_path[f][3];
             descend<DIM>(key, kx, lmax, f, lo, ext, cx, cy, cz);

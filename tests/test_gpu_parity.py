@@ -166,6 +166,79 @@ def test_pairs_deeper_than_one_key_word(engine, precision):
     assert np.all(np.isfinite(sim.download()["x"]))
 
 
+def deep_pairs_state(extent, n=1500, seed=9):
+    """A small cube with close pairs (2e-9 to 4e-8 apart along x) and a light
+    body at (extent, 0, 0) that sets the root extent: a pair d apart shares
+    about log2(2 extent / d) levels of the reference's descent."""
+    s = small_cube(n, seed=seed)
+    extra = []
+    for k, dx in enumerate((2e-9, 7e-9, 4e-8)):
+        b = s[[31 * k + 5]].copy()
+        b["x"] += dx
+        extra.append(b)
+    far = s[[1]].copy()
+    far["x"], far["y"], far["z"], far["mass"] = extent, 0.0, 0.0, 1e-6
+    return P.concat([s] + extra + [far])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pairs_deeper_than_two_key_words(engine, precision):
+    """Pairs that share about 47 levels (a root extent of 2e5): past what a
+    first key word and one more word of the same length hold, within the 64
+    levels the reference reaches (octree.rs:60-62). Topology and centres
+    bit-exact against the oracle's insertion tree, accelerations as for any
+    other state."""
+    s = deep_pairs_state(2e5)
+    t = O.Tree.from_state(s, 3)
+    assert t.merges == 0 and t.fault == 0
+    orec, ofl = t.serialize()
+    assert int(orec[:, 0].max()) > 42
+    sim = P.Sim(engine, s, kind="astro2", precision=precision, **CFG1)
+    a = sim.accel()
+    rec, fl = sim.tree()
+    np.testing.assert_array_equal(rec[:, :3], orec[:, :3])
+    np.testing.assert_array_equal(fl[:, 4:8], ofl[:, 4:8])
+    ref, _, fault = O.bh_transform(s, CFG1["theta"], CFG1["e"])
+    assert fault == 0
+    r = rel_err(a, ref)
+    if precision == "fp64":
+        assert fp64_ok(r)
+    else:
+        assert np.quantile(r, 0.999) <= 1e-4
+
+
+def test_depth_does_not_depend_on_the_batch(engine):
+    """Pairs about 41 levels deep (a root extent of 2e3) in a batch of 512
+    simulations, whose keys give the segment 9 bits and the tree 18 levels
+    per word: the batch steps them as the simulation does alone, bit for bit."""
+    big = deep_pairs_state(2e3, n=1000, seed=11)
+    assert int(O.Tree.from_state(big, 3).serialize()[0][:, 0].max()) > 36
+    kw = dict(theta=1.0, e=0.5, dt=1e-5, precision="fp64")
+    sim = P.Sim(engine, big, **kw)
+    sim.step(3)
+    alone = sim.download()
+    sim.close()
+    smalls = [small_cube(4, seed=100 + k) for k in range(511)]
+    sims = [P.Sim(engine, big, **kw)] + [P.Sim(engine, st, **kw) for st in smalls]
+    engine.batch_step(sims, 3)
+    assert sims[0].download().tobytes() == alone.tobytes()
+    for x in sims:
+        x.close()
+
+
+def test_pairs_past_depth_64_are_an_error(engine):
+    """Two bodies 2e-9 apart under a root extent of 1e12 share about 69
+    levels: the reference panics past depth 64 (octree.rs:60-62) and the
+    oracle reports the fault; the device reports PG_ERR_DEPTH."""
+    s = P.concat([P.gen_star(x=0.3, y=0.2, z=0.1, mass=1.0),
+                  P.gen_star(x=0.3 + 2e-9, y=0.2, z=0.1, mass=1.0),
+                  P.gen_star(x=1e12, y=0.0, z=0.0, mass=1e-6)])
+    assert O.Tree.from_state(s, 3).fault != 0
+    with pytest.raises(P.PgError) as ei:
+        P.Sim(engine, s, theta=1.0, e=0.5, precision="fp64").accel()
+    assert ei.value.code == 5
+
+
 def test_clustered_tile_tree(engine):
     """Close pairs throughout a tile (each pair ~2e-7 apart, ~24 levels deep):
     the tile has far more nodes than bodies, so it is built body by body, and
