@@ -160,9 +160,9 @@ struct pg_sim {
     pg_params p{};
     void* block = nullptr;
     uint64_t bytes = 0;
-    double *x[2], *y[2], *z[2], *px[2], *py[2], *pz[2], *vx[2], *vy[2], *vz[2], *m[2];
-    uint8_t* fix[2];
-    uint32_t* id[2];
+    double4* pos[2];  // (x, y, z, m) per body, in the last build's Morton order
+    double4* prv[2];  // (x_prev, y_prev, z_prev, tag: upload index and fixed flag)
+    double *vx[2], *vy[2], *vz[2];
     Entity* aos = nullptr;
     SegState* st = nullptr;
     int64_t* counts = nullptr;
@@ -194,8 +194,9 @@ static inline int key_lmax(int dim, int segbits) {
 static inline uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
 
 static uint64_t state_bytes(uint64_t n) {
-    // two SoA copies (9 doubles + mass + fixed + id), the uploaded AoS, counts, SegState
-    return 2 * (align256(n * 8) * 10 + align256(n) + align256(n * 4)) + align256(n * 80) +
+    // two copies (position and previous-position records, three velocity
+    // arrays), the uploaded AoS, counts, SegState
+    return 2 * (align256(n * 32) * 2 + align256(n * 8) * 3) + align256(n * 80) +
            align256(n * 8) + 256;
 }
 
@@ -935,14 +936,10 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
             const int src = sm->cur ^ q;
             const int dst = bp.kind == PG_KIND_DIRECT && !integrate ? src : src ^ 1;
             SegDesc& d = e->h_desc[q][s];
-            d.sx = sm->x[src]; d.sy = sm->y[src]; d.sz = sm->z[src];
-            d.spx = sm->px[src]; d.spy = sm->py[src]; d.spz = sm->pz[src];
+            d.spos = sm->pos[src]; d.sprv = sm->prv[src];
             d.svx = sm->vx[src]; d.svy = sm->vy[src]; d.svz = sm->vz[src];
-            d.sm = sm->m[src]; d.sfix = sm->fix[src]; d.sid = sm->id[src];
-            d.dx = sm->x[dst]; d.dy = sm->y[dst]; d.dz = sm->z[dst];
-            d.dpx = sm->px[dst]; d.dpy = sm->py[dst]; d.dpz = sm->pz[dst];
+            d.dpos = sm->pos[dst]; d.dprv = sm->prv[dst];
             d.dvx = sm->vx[dst]; d.dvy = sm->vy[dst]; d.dvz = sm->vz[dst];
-            d.dm = sm->m[dst]; d.dfix = sm->fix[dst]; d.did = sm->id[dst];
             d.off = off;
             d.n = sm->n;
             d.theta = sm->p.theta;
@@ -1105,12 +1102,10 @@ static void carve_state(pg_sim* s, char* b, uint64_t n) {
         return q;
     };
     for (int k = 0; k < 2; ++k) {
-        s->x[k] = (double*)take(n * 8); s->y[k] = (double*)take(n * 8); s->z[k] = (double*)take(n * 8);
-        s->px[k] = (double*)take(n * 8); s->py[k] = (double*)take(n * 8); s->pz[k] = (double*)take(n * 8);
+        // (contiguous: a copy holds an Entity array of n bodies when staged)
+        s->pos[k] = (double4*)take(n * 32);
+        s->prv[k] = (double4*)take(n * 32);
         s->vx[k] = (double*)take(n * 8); s->vy[k] = (double*)take(n * 8); s->vz[k] = (double*)take(n * 8);
-        s->m[k] = (double*)take(n * 8);
-        s->fix[k] = (uint8_t*)take(n);
-        s->id[k] = (uint32_t*)take(n * 4);
     }
     s->aos = (Entity*)take(n * 80);
     s->counts = (int64_t*)take(n * 8);
@@ -1206,8 +1201,7 @@ extern "C" int pg_sim_upload(pg_sim* s, const Entity* state, uint64_t n) {
     if (n) {
         HIP_TRY(hipMemcpyAsync(s->aos, state, n * sizeof(Entity), hipMemcpyHostToDevice, e->stream));
         k_unpack<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(
-            s->aos, n, s->x[0], s->y[0], s->z[0], s->px[0], s->py[0], s->pz[0], s->vx[0],
-            s->vy[0], s->vz[0], s->m[0], s->fix[0], s->id[0]);
+            s->aos, n, s->pos[0], s->prv[0], s->vx[0], s->vy[0], s->vz[0]);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemsetAsync(s->st, 0, sizeof(SegState), e->stream));
@@ -1228,13 +1222,12 @@ extern "C" int pg_sim_upload_state(pg_sim* s, const Entity* state, const Entity*
     pg_engine* e = s->eng;
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     HIP_TRY(hipSetDevice(e->device));
-    // the previous input is staged in the other state buffer (its ten FP64
-    // arrays hold at least n * 80 contiguous bytes; upload made buffer 0
+    // the previous input is staged in the other state buffer (its records
+    // and velocities hold n * 88 contiguous bytes; upload made buffer 0
     // current), its positions become the history of buffer 0
-    Entity* stage = (Entity*)s->x[1];
+    Entity* stage = (Entity*)s->pos[1];
     HIP_TRY(hipMemcpyAsync(stage, previous, n * sizeof(Entity), hipMemcpyHostToDevice, e->stream));
-    k_unpack_prev<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(stage, n, s->px[0], s->py[0],
-                                                                   s->pz[0]);
+    k_unpack_prev<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(stage, n, s->prv[0]);
     HIP_TRY(hipGetLastError());
     // one step taken: the next is a Stormer step (verlet.rs:104-106)
     SegState st;
@@ -1252,7 +1245,7 @@ static int velocity_source(pg_sim* s, VelocitySource* vs) {
     HIP_TRY(hipMemcpy(&st, s->st, sizeof st, hipMemcpyDeviceToHost));
     const int c = s->cur;
     vs->vx = s->vx[c]; vs->vy = s->vy[c]; vs->vz = s->vz[c];
-    vs->px = s->px[c]; vs->py = s->py[c]; vs->pz = s->pz[c];
+    vs->prv = s->prv[c];
     vs->derive = s->integ == PG_INTEG_VERLET && st.steps_done >= 2 ? 1 : 0;
     vs->dt = s->dt_last;
     return PG_OK;
@@ -1270,8 +1263,7 @@ extern "C" int pg_sim_download(pg_sim* s, Entity* out, uint64_t n) {
     const int c = s->cur;
     VelocitySource vs;
     PG_TRY(velocity_source(s, &vs));
-    k_pack<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(s->aos, n, s->x[c], s->y[c], s->z[c],
-                                                           vs, s->id[c]);
+    k_pack<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(s->aos, n, s->pos[c], vs, s->prv[c]);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, s->aos, n * sizeof(Entity), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1280,8 +1272,7 @@ extern "C" int pg_sim_download(pg_sim* s, Entity* out, uint64_t n) {
 
 static StageState stage_state(pg_sim* s) {
     const int c = s->cur;
-    return StageState{s->x[c], s->y[c], s->z[c], s->vx[c], s->vy[c], s->vz[c],
-                      s->fix[c], s->id[c], s->n};
+    return StageState{s->pos[c], s->prv[c], s->vx[c], s->vy[c], s->vz[c], s->n};
 }
 
 // euler.rs:17-49 / rk4.rs:24-182 on the device: each acceleration pass is
@@ -1336,11 +1327,10 @@ extern "C" int pg_sim_checkpoint(pg_sim* s, Entity* state, Entity* previous, uin
         return PG_OK;
     }
     // the previous positions, staged in the other state buffer (dead
-    // between steps; its ten FP64 arrays hold n * 80 bytes)
+    // between steps; its records and velocities hold n * 88 bytes)
     const int c = s->cur;
-    Entity* stage = (Entity*)s->x[c ^ 1];
-    k_pack_prev<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(stage, s->aos, n, s->px[c],
-                                                                 s->py[c], s->pz[c], s->id[c]);
+    Entity* stage = (Entity*)s->pos[c ^ 1];
+    k_pack_prev<<<grid_for(n, 1u << 30), kBlock, 0, e->stream>>>(stage, s->aos, n, s->prv[c]);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(previous, stage, n * sizeof(Entity), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1380,8 +1370,8 @@ extern "C" int pg_sim_set_integrator(pg_sim* s, int32_t integrator) {
         VelocitySource vs;
         PG_TRY(velocity_source(s, &vs));
         if (vs.derive)
-            k_store_velocity<<<grid_for(s->n, 1u << 30), kBlock, 0, e->stream>>>(
-                stage_state(s), vs.px, vs.py, vs.pz, vs.dt);
+            k_store_velocity<<<grid_for(s->n, 1u << 30), kBlock, 0, e->stream>>>(stage_state(s),
+                                                                                 vs.dt);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemsetAsync(s->st, 0, sizeof(SegState), e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1579,8 +1569,7 @@ extern "C" int pg_sim_energy(pg_sim* s, double out[6]) {
         (void)hipFree(part);
         return vrc;
     }
-    k_energy<<<nb, kBlock, 0, e->stream>>>(s->x[c], s->y[c], s->z[c], vs, s->m[c], n,
-                                          std::fabs(s->p.e), part);
+    k_energy<<<nb, kBlock, 0, e->stream>>>(s->pos[c], vs, n, std::fabs(s->p.e), part);
     hipError_t le = hipGetLastError();
     std::vector<double> h((size_t)nb * 5);
     hipError_t ce = hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, e->stream);
@@ -1645,7 +1634,12 @@ extern "C" int pg_sim_morton(pg_sim* s, uint64_t* keys, uint32_t* perm, uint64_t
     if (n != s->n) return pg_set_error(PG_ERR_ARG, "n mismatch");
     HIP_TRY(hipSetDevice(e->device));
     if (keys) HIP_TRY(hipMemcpy(keys, e->keys[e->last_sorted], n * 8, hipMemcpyDeviceToHost));
-    if (perm) HIP_TRY(hipMemcpy(perm, s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
+    if (perm) {
+        // the upload indices: the tags of the permuted records
+        std::vector<double4> pr(n);
+        HIP_TRY(hipMemcpy(pr.data(), s->prv[s->cur], n * sizeof(double4), hipMemcpyDeviceToHost));
+        for (uint64_t j = 0; j < n; ++j) perm[j] = tag_id(pr[j].w);
+    }
     return PG_OK;
 }
 
@@ -1675,7 +1669,11 @@ extern "C" int pg_sim_tree(pg_sim* s, int64_t* rec, double* fl, uint64_t cap, ui
         HIP_TRY(hipMemcpy(span.data(), e->node_span, ni * sizeof(uint2), hipMemcpyDeviceToHost));
     }
     // the permuted ids live in the buffer the build wrote: the current one
-    HIP_TRY(hipMemcpy(id.data(), s->id[s->cur], n * 4, hipMemcpyDeviceToHost));
+    {
+        std::vector<double4> pr(n);
+        HIP_TRY(hipMemcpy(pr.data(), s->prv[s->cur], n * sizeof(double4), hipMemcpyDeviceToHost));
+        for (uint64_t j = 0; j < n; ++j) id[j] = tag_id(pr[j].w);
+    }
     HIP_TRY(hipMemcpy(lo.data(), e->lo_level, n, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(lp.data(), e->leafpos, n * sizeof(double4), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(keys.data(), e->keys[e->last_sorted], n * 8, hipMemcpyDeviceToHost));

@@ -59,18 +59,28 @@ enum : uint32_t {
 
 struct SegState;
 
+// A body's tag, the fourth word of its previous-position record: its upload
+// index (low 32 bits) and its fixed flag (bit 32), a bit pattern that no
+// arithmetic touches.
+__host__ __device__ inline double make_tag(uint32_t id, bool fixed) {
+    return __builtin_bit_cast(double, (uint64_t)id | ((uint64_t)(fixed ? 1u : 0u) << 32));
+}
+__host__ __device__ inline uint32_t tag_id(double t) {
+    return (uint32_t)__builtin_bit_cast(uint64_t, t);
+}
+__host__ __device__ inline bool tag_fixed(double t) {
+    return (__builtin_bit_cast(uint64_t, t) >> 32) != 0;
+}
+
 // One simulation's view inside a batched launch: the current state (src) and
 // the buffer the step writes (dst), both SoA in the order of the previous
 // build's Morton sort.
 struct SegDesc {
-    const double* sx; const double* sy; const double* sz;
-    const double* spx; const double* spy; const double* spz;
+    // per body: (x, y, z, m) and (x_prev, y_prev, z_prev, tag); velocities
+    const double4* spos; const double4* sprv;
     const double* svx; const double* svy; const double* svz;
-    const double* sm; const uint8_t* sfix; const uint32_t* sid;
-    double* dx; double* dy; double* dz;
-    double* dpx; double* dpy; double* dpz;
+    double4* dpos; double4* dprv;
     double* dvx; double* dvy; double* dvz;
-    double* dm; uint8_t* dfix; uint32_t* did;
     uint64_t off;   // first global (batch) index of this segment
     uint64_t n;
     double theta, e, dt;

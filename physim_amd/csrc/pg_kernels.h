@@ -238,7 +238,8 @@ __global__ __launch_bounds__(kBlock) void k_extent_partial(const SegDesc* __rest
         uint64_t m = 0;
         bool bad = false;
         for (uint64_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
-            const double x = d.sx[j], y = d.sy[j], z = d.sz[j];
+            const double4 p = d.spos[j];
+            const double x = p.x, y = p.y, z = p.z;
             bad |= !isfinite(x) || !isfinite(y) || !isfinite(z);
             const uint64_t b = f64_bits(fmax(fmax(fabs(x), fabs(y)), fabs(z)));
             m = b > m ? b : m;
@@ -398,7 +399,8 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
             }
             if (g == d.off) open_segment(d.st, m, integrate);
             const uint64_t j = g - d.off;
-            const double px = d.sx[j], py = d.sy[j], pz = d.sz[j];
+            const double4 q = d.spos[j];
+            const double px = q.x, py = q.y, pz = q.z;
             double ext = bits_f64(m);
             const double inv = 1.0 / ext;
             const double scale = ldexp(1.0, lmax - 1), margin = ldexp(1.0, lmax - 41);
@@ -653,7 +655,8 @@ __device__ void order_tie_run(const SegDesc* __restrict__ desc, uint64_t total,
             atomicOr(&d.st->err, kErrCapacity);
             return;
         }
-        const double px = d.sx[src], py = d.sy[src], pz = d.sz[src];
+        const double4 q = d.spos[src];
+        const double px = q.x, py = q.y, pz = q.z;
         double ext = E, cx = 0.0, cy = 0.0, cz = 0.0;
         DeepKey dk;
         for (int i = 0; i < kDeepWords; ++i) dk.w[i] = 0;
@@ -1574,18 +1577,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 // position and mass as one record in sorted order: the node
                 // builder's input and the walk's leaves (the walk writes the
                 // state's own arrays)
-                leafpos[i] = make_double4(d.sx[src], d.sy[src], d.sz[src], d.sm[src]);
-                // the previous positions only where the Stormer history is
-                // valid (the Taylor step writes them; force-only calls never
-                // read them)
-                if (d.st->history) {
-                    d.dpx[j] = d.spx[src]; d.dpy[j] = d.spy[src]; d.dpz[j] = d.spz[src];
-                }
+                leafpos[i] = d.spos[src];
+                // the previous position (meaningful where the Stormer history
+                // is valid) and the tag: one record
+                d.dprv[j] = d.sprv[src];
                 if (d.st->vperm) {
                     d.dvx[j] = d.svx[src]; d.dvy[j] = d.svy[src]; d.dvz[j] = d.svz[src];
                 }
-                d.dfix[j] = d.sfix[src];
-                d.did[j] = d.sid[src];
             }
             lo_level[i] = (uint8_t)lo;
             // by sorted position, read by a walk with a near field
@@ -2502,7 +2500,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         const double4 p = leafpos[g];
         x = p.x; y = p.y; z = p.z;
         m = p.w;
-        fixed = d.dfix[j] != 0;
+        fixed = tag_fixed(d.dprv[j].w);
     }
     Kahan3 ka;
     ka.zero();
@@ -2634,7 +2632,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 az = (double)ka.sz - (double)ka.cz;
             }
         }
-        if (d.count) d.counts[d.did[j]] = count;
+        const double4 pr = d.dprv[j];  // previous position and tag
+        if (d.count) d.counts[tag_id(pr.w)] = count;
         if (INTEGRATE) {
             // verlet.rs:24-82. After the Taylor first step the velocity is
             // (x' - x) / dt of the stored positions and is formed on download.
@@ -2650,23 +2649,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 d.dvy[j] = uy + ay * dt;
                 d.dvz[j] = uz + az * dt;
             } else {
-                nx = 2.0 * x - d.dpx[j] + ax * dt2;
-                ny = 2.0 * y - d.dpy[j] + ay * dt2;
-                nz = 2.0 * z - d.dpz[j] + az * dt2;
+                nx = 2.0 * x - pr.x + ax * dt2;
+                ny = 2.0 * y - pr.y + ay * dt2;
+                nz = 2.0 * z - pr.z + az * dt2;
             }
-            d.dx[j] = nx; d.dy[j] = ny; d.dz[j] = nz;
-            d.dpx[j] = x; d.dpy[j] = y; d.dpz[j] = z;
-            d.dm[j] = m;
+            d.dpos[j] = make_double4(nx, ny, nz, m);
+            d.dprv[j] = make_double4(x, y, z, pr.w);
             emax = f64_bits(fmax(fmax(fabs(nx), fabs(ny)), fabs(nz)));
             nonfinite = !isfinite(nx) || !isfinite(ny) || !isfinite(nz);
             chunk = d.chunk0 + (uint32_t)(j / kExtentChunk);
             if (nonfinite) atomicOr(&d.st->err, kErrNonFinite);
         } else {
-            double* o = acc_out + 3 * (d.off + d.did[j]);
+            double* o = acc_out + 3 * (d.off + tag_id(pr.w));
             o[0] = ax; o[1] = ay; o[2] = az;
             // the permuted state keeps its positions and masses
-            d.dx[j] = x; d.dy[j] = y; d.dz[j] = z;
-            d.dm[j] = m;
+            d.dpos[j] = make_double4(x, y, z, m);
         }
     }
     if (INTEGRATE) {
@@ -2727,6 +2724,7 @@ template <bool FP64, bool INTEGRATE>
 __device__ __forceinline__ void direct_store(const SegDesc& d, uint64_t i, double x, double y,
                                              double z, double m, bool fixed, double fx,
                                              double fy, double fz, double* acc_out) {
+    const double4 pr = d.sprv[i];  // previous position and tag
     double ax = 0.0, ay = 0.0, az = 0.0;
     if (!fixed) {
         if (FP64) {
@@ -2751,20 +2749,17 @@ __device__ __forceinline__ void direct_store(const SegDesc& d, uint64_t i, doubl
             d.dvy[i] = uy + ay * dt;
             d.dvz[i] = uz + az * dt;
         } else {
-            nx = 2.0 * x - d.spx[i] + ax * dt2;
-            ny = 2.0 * y - d.spy[i] + ay * dt2;
-            nz = 2.0 * z - d.spz[i] + az * dt2;
+            nx = 2.0 * x - pr.x + ax * dt2;
+            ny = 2.0 * y - pr.y + ay * dt2;
+            nz = 2.0 * z - pr.z + az * dt2;
         }
-        d.dx[i] = nx; d.dy[i] = ny; d.dz[i] = nz;
+        d.dpos[i] = make_double4(nx, ny, nz, m);
         // a non-finite new position fails the simulation (as the tree
         // step's walk reports it)
         if (!isfinite(nx) || !isfinite(ny) || !isfinite(nz)) atomicOr(&d.st->err, kErrNonFinite);
-        d.dpx[i] = x; d.dpy[i] = y; d.dpz[i] = z;
-        d.dm[i] = m;
-        d.dfix[i] = d.sfix[i];
-        d.did[i] = d.sid[i];
+        d.dprv[i] = make_double4(x, y, z, pr.w);
     } else {
-        double* o = acc_out + 3 * (d.off + d.sid[i]);
+        double* o = acc_out + 3 * (d.off + tag_id(pr.w));
         o[0] = ax; o[1] = ay; o[2] = az;
     }
 }
@@ -2795,9 +2790,10 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
     double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
     bool fixed = true;
     if (valid) {
-        x = d.sx[i]; y = d.sy[i]; z = d.sz[i];
-        m = d.sm[i];
-        fixed = d.sfix[i] != 0;
+        const double4 p = d.spos[i];
+        x = p.x; y = p.y; z = p.z;
+        m = p.w;
+        fixed = tag_fixed(d.sprv[i].w);
     }
     const double e = d.e;
     const float ef = (float)e;
@@ -2807,9 +2803,11 @@ __global__ __launch_bounds__(kBlock) void k_direct(const SegDesc* __restrict__ d
     for (uint64_t t0 = j0; t0 < j1; t0 += kBlock) {
         const uint64_t jj = t0 + threadIdx.x;
         // mixed precision stages the mass as FP32 in the fourth word
-        if (jj < j1)
-            sj[threadIdx.x] = make_double4(d.sx[jj], d.sy[jj], d.sz[jj],
-                                           FP64 ? d.sm[jj] : bits_f64((uint64_t)f32_bits((float)d.sm[jj])));
+        if (jj < j1) {
+            const double4 p = d.spos[jj];
+            sj[threadIdx.x] =
+                make_double4(p.x, p.y, p.z, FP64 ? p.w : bits_f64((uint64_t)f32_bits((float)p.w)));
+        }
         __syncthreads();
         const int cnt = (int)min((uint64_t)kBlock, j1 - t0);
         if (FP64) {
@@ -2878,22 +2876,25 @@ __global__ __launch_bounds__(kBlock) void k_direct_finish(const SegDesc* __restr
         const double* o = part + 3 * (d.part_off + (uint64_t)sp * d.n + i);
         fx += o[0]; fy += o[1]; fz += o[2];
     }
-    direct_store<FP64, INTEGRATE>(d, i, d.sx[i], d.sy[i], d.sz[i], d.sm[i], d.sfix[i] != 0, fx,
-                                  fy, fz, acc_out);
+    const double4 p = d.spos[i];
+    direct_store<FP64, INTEGRATE>(d, i, p.x, p.y, p.z, p.w, tag_fixed(d.sprv[i].w), fx, fy, fz,
+                                  acc_out);
 }
 
 // Velocity of body j: stored (before and right after the Taylor first step)
 // or the reference's backward difference (x - x_prev) / dt (verlet.rs:68-70).
 struct VelocitySource {
-    const double *vx, *vy, *vz, *px, *py, *pz;
+    const double *vx, *vy, *vz;
+    const double4* prv;
     int derive;
     double dt;
     __device__ __forceinline__ void at(uint64_t j, double x, double y, double z, double& ox,
                                        double& oy, double& oz) const {
         if (derive) {
-            ox = (x - px[j]) / dt;
-            oy = (y - py[j]) / dt;
-            oz = (z - pz[j]) / dt;
+            const double4 p = prv[j];
+            ox = (x - p.x) / dt;
+            oy = (y - p.y) / dt;
+            oz = (z - p.z) / dt;
         } else {
             ox = vx[j];
             oy = vy[j];
@@ -2905,23 +2906,23 @@ struct VelocitySource {
 // FP64 energy terms of one state: per block partial sums of
 // {V (half of the all-pairs sum), K, Px, Py, Pz}. U(r) = -(ma mb / sqrt(e))
 // atan(sqrt(e) / r) is the potential of |F| = ma mb / (r^2 + e).
-__global__ __launch_bounds__(kBlock) void k_energy(const double* __restrict__ x,
-                                                   const double* __restrict__ y,
-                                                   const double* __restrict__ z,
-                                                   VelocitySource vs,
-                                                   const double* __restrict__ m, uint64_t n,
+__global__ __launch_bounds__(kBlock) void k_energy(const double4* __restrict__ pos,
+                                                   VelocitySource vs, uint64_t n,
                                                    double e, double* __restrict__ partial) {
     __shared__ double4 sj[kBlock];
     __shared__ double red[5][kBlock];
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < n;
     double xi = 0, yi = 0, zi = 0, mi = 0;
-    if (valid) { xi = x[i]; yi = y[i]; zi = z[i]; mi = m[i]; }
+    if (valid) {
+        const double4 p = pos[i];
+        xi = p.x; yi = p.y; zi = p.z; mi = p.w;
+    }
     const double se = sqrt(e);
     double V = 0.0;
     for (uint64_t t0 = 0; t0 < n; t0 += kBlock) {
         const uint64_t jj = t0 + threadIdx.x;
-        if (jj < n) sj[threadIdx.x] = make_double4(x[jj], y[jj], z[jj], m[jj]);
+        if (jj < n) sj[threadIdx.x] = pos[jj];
         __syncthreads();
         const int cnt = (int)min((uint64_t)kBlock, n - t0);
         if (valid) {
@@ -2958,48 +2959,45 @@ __global__ __launch_bounds__(kBlock) void k_energy(const double* __restrict__ x,
 }
 
 // AoS <-> SoA at the host boundary.
-__global__ void k_unpack(const Entity* __restrict__ aos, uint64_t n, double* x, double* y,
-                         double* z, double* px, double* py, double* pz, double* vx, double* vy,
-                         double* vz, double* m, uint8_t* fix, uint32_t* id) {
+__global__ void k_unpack(const Entity* __restrict__ aos, uint64_t n, double4* __restrict__ pos,
+                         double4* __restrict__ prv, double* vx, double* vy, double* vz) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Entity e = aos[i];
-    x[i] = e.x; y[i] = e.y; z[i] = e.z;
-    px[i] = e.x; py[i] = e.y; pz[i] = e.z;
+    pos[i] = make_double4(e.x, e.y, e.z, e.mass);
+    prv[i] = make_double4(e.x, e.y, e.z, make_tag((uint32_t)i, e.fixed != 0));
     vx[i] = e.vx; vy[i] = e.vy; vz[i] = e.vz;
-    m[i] = e.mass;
-    fix[i] = e.fixed ? 1 : 0;
-    id[i] = (uint32_t)i;
 }
 
 // The previous positions (Stormer history) of an uploaded state.
-__global__ void k_unpack_prev(const Entity* __restrict__ aos, uint64_t n, double* px, double* py,
-                              double* pz) {
+__global__ void k_unpack_prev(const Entity* __restrict__ aos, uint64_t n,
+                              double4* __restrict__ prv) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Entity e = aos[i];
-    px[i] = e.x; py[i] = e.y; pz[i] = e.z;
+    prv[i] = make_double4(e.x, e.y, e.z, prv[i].w);
 }
 
 // The Stormer history as Entity records in upload order: the current
 // record (radius, mass, id, fixed, velocity) with the previous position.
 __global__ void k_pack_prev(Entity* __restrict__ out, const Entity* __restrict__ cur, uint64_t n,
-                            const double* px, const double* py, const double* pz,
-                            const uint32_t* id) {
+                            const double4* __restrict__ prv) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    const uint32_t u = id[j];
+    const double4 p = prv[j];
+    const uint32_t u = tag_id(p.w);
     Entity e = cur[u];
-    e.x = px[j]; e.y = py[j]; e.z = pz[j];
+    e.x = p.x; e.y = p.y; e.z = p.z;
     out[u] = e;
 }
 
-__global__ void k_pack(Entity* __restrict__ aos, uint64_t n, const double* x, const double* y,
-                       const double* z, VelocitySource v, const uint32_t* id) {
+__global__ void k_pack(Entity* __restrict__ aos, uint64_t n, const double4* __restrict__ pos,
+                       VelocitySource v, const double4* __restrict__ prv) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    Entity& e = aos[id[j]];
-    e.x = x[j]; e.y = y[j]; e.z = z[j];
+    Entity& e = aos[tag_id(prv[j].w)];
+    const double4 p = pos[j];
+    e.x = p.x; e.y = p.y; e.z = p.z;
     v.at(j, e.x, e.y, e.z, e.vx, e.vy, e.vz);
 }
 
@@ -3024,9 +3022,9 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 // kernels work in the state's current order and keep the base state and the
 // rk4 sums in upload order.
 struct StageState {
-    double *x, *y, *z, *vx, *vy, *vz;
-    const uint8_t* fix;
-    const uint32_t* id;
+    double4* pos;        // (x, y, z, m): the stages write x, y, z
+    const double4* prv;  // the tags
+    double *vx, *vy, *vz;
     uint64_t n;
 };
 
@@ -3035,13 +3033,13 @@ struct StageState {
 __global__ void k_euler_update(StageState st, const double* __restrict__ acc, double dt) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= st.n) return;
-    const double* a = acc + 3 * (uint64_t)st.id[j];
+    const double* a = acc + 3 * (uint64_t)tag_id(st.prv[j].w);
     const double dt2 = dt * dt;
-    const double x = st.x[j], y = st.y[j], z = st.z[j];
+    const double4 p = st.pos[j];
+    const double x = p.x, y = p.y, z = p.z;
     const double vx = st.vx[j], vy = st.vy[j], vz = st.vz[j];
-    st.x[j] = x + vx * dt + 0.5 * a[0] * dt2;
-    st.y[j] = y + vy * dt + 0.5 * a[1] * dt2;
-    st.z[j] = z + vz * dt + 0.5 * a[2] * dt2;
+    st.pos[j] = make_double4(x + vx * dt + 0.5 * a[0] * dt2, y + vy * dt + 0.5 * a[1] * dt2,
+                             z + vz * dt + 0.5 * a[2] * dt2, p.w);
     st.vx[j] = vx + a[0] * dt;
     st.vy[j] = vy + a[1] * dt;
     st.vz[j] = vz + a[2] * dt;
@@ -3051,8 +3049,9 @@ __global__ void k_euler_update(StageState st, const double* __restrict__ acc, do
 __global__ void k_rk4_begin(StageState st, double* __restrict__ base) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= st.n) return;
-    double* b = base + 6 * (uint64_t)st.id[j];
-    b[0] = st.x[j]; b[1] = st.y[j]; b[2] = st.z[j];
+    double* b = base + 6 * (uint64_t)tag_id(st.prv[j].w);
+    const double4 p = st.pos[j];
+    b[0] = p.x; b[1] = p.y; b[2] = p.z;
     b[3] = st.vx[j]; b[4] = st.vy[j]; b[5] = st.vz[j];
 }
 
@@ -3068,7 +3067,8 @@ __global__ void k_rk4_stage(StageState st, const double* __restrict__ acc,
                             int stage) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= st.n) return;
-    const uint64_t u = st.id[j];
+    const double4 pr = st.prv[j];
+    const uint64_t u = tag_id(pr.w);
     const double* a = acc + 3 * u;
     const double* b = base + 6 * u;
     double* ks = ksum + 6 * u;
@@ -3081,28 +3081,29 @@ __global__ void k_rk4_stage(StageState st, const double* __restrict__ acc,
         else sum[c] = ks[c] + 2.0 * k[c];
         ks[c] = sum[c];
     }
-    double* out[6] = {st.x, st.y, st.z, st.vx, st.vy, st.vz};
+    double o[6];
     if (stage < 3) {
         const double w = stage < 2 ? 0.5 : 1.0;
-        for (int c = 0; c < 6; ++c) out[c][j] = b[c] + w * k[c];
-    } else if (st.fix[j] == 0) {
-        for (int c = 0; c < 6; ++c) out[c][j] = b[c] + sum[c] / 6.0;
+        for (int c = 0; c < 6; ++c) o[c] = b[c] + w * k[c];
+    } else if (!tag_fixed(pr.w)) {
+        for (int c = 0; c < 6; ++c) o[c] = b[c] + sum[c] / 6.0;
     } else {
-        for (int c = 0; c < 3; ++c) out[c][j] = b[c];
-        for (int c = 3; c < 6; ++c) out[c][j] = 0.0;
+        for (int c = 0; c < 3; ++c) o[c] = b[c];
+        for (int c = 3; c < 6; ++c) o[c] = 0.0;
     }
+    st.pos[j] = make_double4(o[0], o[1], o[2], st.pos[j].w);
+    st.vx[j] = o[3]; st.vy[j] = o[4]; st.vz[j] = o[5];
 }
 
 // v = (x - x_prev) / dt (the Stormer velocity, verlet.rs:68-70) written into
 // the stored velocity arrays, for a change of integrator.
-__global__ void k_store_velocity(StageState st, const double* __restrict__ px,
-                                 const double* __restrict__ py, const double* __restrict__ pz,
-                                 double dt) {
+__global__ void k_store_velocity(StageState st, double dt) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= st.n) return;
-    st.vx[j] = (st.x[j] - px[j]) / dt;
-    st.vy[j] = (st.y[j] - py[j]) / dt;
-    st.vz[j] = (st.z[j] - pz[j]) / dt;
+    const double4 p = st.pos[j], q = st.prv[j];
+    st.vx[j] = (p.x - q.x) / dt;
+    st.vy[j] = (p.y - q.y) / dt;
+    st.vz[j] = (p.z - q.z) / dt;
 }
 
 // dst[k] += src[k], FP64.
