@@ -30,14 +30,15 @@ SORT_TILE = 2048   # keys per radix-sort tile
 def algorithmic_bytes(kernel, n, nodes):
     """This build's own table: the bytes one launch of `kernel` reads and
     writes for n bodies and the given tree size, each input read once and
-    each output written once, in the layouts the kernels use (FP64 state,
-    64-byte node records)."""
+    each output written once, in the layouts the kernels use (per body a
+    32-byte (x, y, z, m) record and a 32-byte (previous position, tag)
+    record; per internal node a 64-byte record and an 8-byte body span)."""
     tiles = (n + SORT_TILE - 1) // SORT_TILE
     table = {
-        "extent": 24.0 * n,                          # read x, y, z
-        # read x, y, z; write key, index; then digit counts (radix) or each
-        # key's bucket (sample sort)
-        "keys": 36.0 * n + 2.0 * n,
+        "extent": 32.0 * n,                          # read the position records
+        # read the position records; write key, index; then digit counts
+        # (radix) or each key's bucket (sample sort)
+        "keys": 44.0 * n + 2.0 * n,
         "sort_count": 8.0 * n + 1024.0 * tiles,      # read keys; write digit counts
         "sort_scan": 2048.0 * tiles,                 # read + write the count matrix
         "sort_scatter": 24.0 * n + 1024.0 * tiles,   # read + write (key, index); read offsets
@@ -46,13 +47,15 @@ def algorithmic_bytes(kernel, n, nodes):
         "bucket_count": (8 + 2) * n,
         "bucket_scatter": (8 + 4 + 2 + 12) * n,
         "bucket_sort": 24.0 * n,
-        # keys, sorted index, state read (x y z px py pz m id fixed) and written
-        # (+ velocities on the first two steps only), node offsets, levels
-        "scan_gather": (8 + 4 + 65 + 65 + 4 + 1) * n,
-        # keys, node offsets, levels, leaf x y z m; node records written once
-        "nodes": 45.0 * n + 64.0 * nodes,
-        # own x y z m fixed + Stormer x_prev in, x and x_prev out; tree read once
-        "walk": (33.0 + 24.0 + 48.0) * n + 64.0 * nodes,
+        # keys, sorted index, both records read and written (+ velocities on
+        # the first two steps only), node offsets, levels
+        "scan_gather": (8 + 4 + 64 + 64 + 4 + 1) * n,
+        # keys, node offsets, levels, leaf records; node records and spans
+        # written once
+        "nodes": 45.0 * n + 72.0 * nodes,
+        # own position record and previous-position record in, both out (the
+        # tag stays); the tree's records and spans read once
+        "walk": (32.0 + 32.0 + 32.0 + 24.0) * n + 72.0 * nodes,
         "direct": 32.0 * n,
     }
     return table.get(kernel, 0.0)
@@ -486,7 +489,7 @@ def main():
     internal = st["nodes"] - n
     sort_passes = 2 if "bucket_sort" in prof else 8  # sample sort: scatter + bucket sort
     cbytes = compulsory_bytes(dom, n, internal, sort_passes)
-    abytes = algorithmic_bytes(dom, n, st["nodes"])
+    abytes = algorithmic_bytes(dom, n, internal)  # records exist for internal nodes only
     achieved = cbytes / (avg_ms * 1e-3) / 1e9
     step_bytes = compulsory_step_bytes(n, internal, sort_passes)
 
@@ -546,7 +549,8 @@ def main():
                           "achieved": abytes / (avg_ms * 1e-3) / 1e9,
                           "frac": abytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                           "basis": "bytes each kernel reads and writes in its own layouts "
-                                   "(FP64 state, 64-byte records for every node)"},
+                                   "(two 32-byte records per body; a 64-byte record and an "
+                                   "8-byte span per internal node)"},
             "step": {"compulsory_bytes_per_body": step_bytes / n,
                      "sort_passes": sort_passes, "internal_nodes": internal,
                      "achieved": step_bytes / (elapsed / args.steps) / 1e9,
