@@ -314,6 +314,25 @@ __device__ __forceinline__ uint32_t bucket_of(const uint64_t* s_spl, uint32_t nb
     return lo;
 }
 
+// The same count for K keys at once: a fixed number of halvings (no branch,
+// no divergence), the K searches' loads in flight together.
+template <int K>
+__device__ __forceinline__ void buckets_of(const uint64_t* s_spl, uint32_t nb, const uint64_t* key,
+                                           uint32_t* b) {
+    static_assert(kMaxBuckets <= 1024, "ten halvings cover the splitters");
+    const uint32_t ns = nb - 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) b[k] = 0;
+#pragma unroll
+    for (uint32_t step = kMaxBuckets / 2; step; step >>= 1) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t q = b[k] + step;
+            if (q <= ns && s_spl[q - 1] <= key[k]) b[k] = q;
+        }
+    }
+}
+
 // Morton key by exact FP64 descent (octree.rs:160-215 / quadtree.rs:161-194):
 // at each level the octant is (x > cx) | (y > cy) << 1 [| (z > cz) << 2] and
 // the child centre is c -/+ extent/2, rounded as the reference rounds it;
@@ -438,11 +457,14 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
     if (sort_tag == 0) {
         if (threadIdx.x < 256) cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
     } else if (fuse) {
+        constexpr int kItems = kSortTile / kKeyThreads;
+        uint32_t bk[kItems];
+        buckets_of<kItems>(s_spl, nb, tk, bk);
 #pragma unroll
-        for (int k = 0; k < kSortTile / kKeyThreads; ++k) {
+        for (int k = 0; k < kItems; ++k) {
             const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
             const bool valid = g < total;
-            const uint32_t b = valid ? bucket_of(s_spl, nb, tk[k]) : 0u;
+            const uint32_t b = valid ? bk[k] : 0u;
             if (valid) bkt[g] = (uint16_t)b;
             // a wave's keys mostly share a bucket (nearly sorted input): then
             // one atomic for the wave, else one per key
