@@ -2430,15 +2430,18 @@ struct Kahan3 {
 // 1e-6 * theta the estimate decides as the reference does. Inside the band,
 // or with operands outside FP32's normal range, the FP64 square root and
 // division decide.
-__device__ __forceinline__ bool accept_cell(double hw, double s, double theta) {
-    if (s > 1e-30 && s < 1e30 && hw > 1e-30 && hw < 1e30) {
-        const float q = (float)hw * __builtin_amdgcn_rsqf((float)s);
-        const float t = (float)theta;
-        const float band = 1e-6f * fabsf(t);
-        if (q < t - band) return true;
-        if (q > t + band) return false;
-    }
-    return hw / sqrt(s) < theta;
+// Lanes for which `act` holds take the decision; the FP32 estimate is formed
+// for every lane and only the FP64 test (lanes inside the band) is behind a
+// branch, so the common path has no exec-mask handling.
+__device__ __forceinline__ bool accept_cell(double hw, double s, double theta, bool act) {
+    const bool inr = s > 1e-30 && s < 1e30 && hw > 1e-30 && hw < 1e30;
+    const float q = (float)hw * __builtin_amdgcn_rsqf((float)s);
+    const float t = (float)theta;
+    const float band = 1e-6f * fabsf(t);
+    const bool yes = q < t - band, no = q > t + band;
+    bool take = yes;
+    if (act && !(inr && (yes || no))) take = hw / sqrt(s) < theta;
+    return take;
 }
 
 // Pair acceleration on body a from (b, mb), mixed precision: the separation
@@ -2579,20 +2582,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 const NodeHot h_skip = at32(hot, min(h.skip, clamp));
                 const uint2 sp_skip = at32(node_span, min(h.skip, clamp));
                 bx = h.mx; by = h.my; bz = h.mz; bm = h.mass;
-                bool open = false;
-                if (walker && b >= skip_until) {
-                    const double d0x = x - h.cx, d0y = y - h.cy, d0z = z - h.cz;
-                    const double hw = ldexp(E, -h.level());
-                    bool take = accept_cell(hw, d0x * d0x + d0y * d0y + d0z * d0z, theta);
-                    if (take && nearf && h.level() >= lleaf && h.level() <= lleaf + kNearLevels) {
-                        const double reach = hw + margin;
-                        take = !(fabs(d0x) < reach && fabs(d0y) < reach &&
-                                 (DIM == 2 || fabs(d0z) < reach));
-                    }
-                    if (take) skip_until = sp.y;
-                    add = take;
-                    open = !take;
+                // the opening test for every lane, its result kept by the
+                // lanes still walking (no branch on the common path)
+                const bool act = walker && b >= skip_until;
+                const double d0x = x - h.cx, d0y = y - h.cy, d0z = z - h.cz;
+                const double hw = ldexp(E, -h.level());
+                bool take = accept_cell(hw, d0x * d0x + d0y * d0y + d0z * d0z, theta, act);
+                if (nearf) {
+                    const int lv = h.level();
+                    const double reach = hw + margin;
+                    const bool close = fabs(d0x) < reach && fabs(d0y) < reach &&
+                                       (DIM == 2 || fabs(d0z) < reach);
+                    take = take && !(lv >= lleaf && lv <= lleaf + kNearLevels && close);
                 }
+                take = take && act;
+                const bool open = act && !take;
+                skip_until = take ? sp.y : skip_until;
+                add = take;
                 if (__ballot(open) || h.skip <= k || sp.y <= b) {
                     k = uniform_u32(k + 1u);
                     h = h_open;
