@@ -2444,17 +2444,21 @@ __device__ __forceinline__ bool accept_cell(double hw, double s, double theta, b
     return take;
 }
 
+
 // Pair acceleration on body a from (b, mb), mixed precision: the separation
 // b - a is taken in FP64 and rounded once; |F|/m_a = mb / (s + e) along the
-// unsoftened direction (astro/src/lib.rs:84-113, with m_a cancelled).
+// unsoftened direction (astro/src/lib.rs:84-113, with m_a cancelled). Where
+// `use` is false (or the separation rounds to zero) the term is zero, by a
+// select rather than a branch: no exec-mask handling in the walk's loop.
 __device__ __forceinline__ void pair_mixed(double ax, double ay, double az, double bx, double by,
-                                           double bz, double mb, float ef, Kahan3& acc) {
+                                              double bz, double mb, float ef, Kahan3& acc,
+                                              bool use) {
     const float fx = (float)(bx - ax), fy = (float)(by - ay), fz = (float)(bz - az);
     const float s = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-    if (s > 0.0f) {
-        const float w = (float)mb * __builtin_amdgcn_rsqf(s) * __builtin_amdgcn_rcpf(s + ef);
-        acc.add(fx * w, fy * w, fz * w);
-    }
+    const float w = use && s > 0.0f
+                        ? (float)mb * __builtin_amdgcn_rsqf(s) * __builtin_amdgcn_rcpf(s + ef)
+                        : 0.0f;
+    acc.add(fx * w, fy * w, fz * w);
 }
 
 // Pair force on a from b exactly as astro/src/lib.rs:84-113 computes it (FP64,
@@ -2619,11 +2623,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             if (add) {
                 ++count;
                 // (the reference skips a source at the body's own position)
-                if (!(bx == x && by == y && bz == z)) {
-                    if (FP64)
-                        pair_fp64(x, y, z, m, bx, by, bz, bm, e, f64);
-                    else
-                        pair_mixed(x, y, z, bx, by, bz, bm, ef, ka);
+                const bool other = !(bx == x && by == y && bz == z);
+                if (FP64) {
+                    if (other) pair_fp64(x, y, z, m, bx, by, bz, bm, e, f64);
+                } else {
+                    pair_mixed(x, y, z, bx, by, bz, bm, ef, ka, other);
                 }
             }
         }
