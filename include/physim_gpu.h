@@ -89,19 +89,20 @@ uint64_t pg_sim_bytes(uint64_t n, int32_t kind);
 /* Capture repeated steps in hipGraphs (default on). */
 int pg_engine_set_graphs(pg_engine* eng, int enable);
 /* Key sorting of the tree build: a sample sort into at most max_buckets
- * buckets (<= 1024; a larger batch, or max_buckets = 0, uses the 8-bit LSD
+ * buckets (<= 8192; a larger batch, or max_buckets = 0, uses the 8-bit LSD
  * radix sort), each sorted in LDS when it holds at most bucket_cap keys
- * (<= 4096) and in HBM otherwise. Both give the same order. Defaults 1024,
+ * (<= 4096) and in HBM otherwise. Both give the same order. Defaults 8192,
  * 4096. A tuning and testing control; no reference counterpart. */
 int pg_engine_set_sort(pg_engine* eng, uint32_t max_buckets, uint32_t bucket_cap);
 /* Launch choices that change the time of a step but not its results (the
  * sort's order, the tree and every sum are the same under all of them). No
  * reference counterpart. */
 typedef struct pg_tuning {
-    uint32_t max_buckets;   /* sample sort up to this many buckets (<= 1024; 0: radix sort) */
+    uint32_t max_buckets;   /* sample sort up to this many buckets (<= 8192; 0: radix sort) */
     uint32_t bucket_keys;   /* mean keys per sample-sort bucket (256..4096; default 1664) */
     uint32_t walk_run_log2; /* the walk's blocks go to each XCD in runs of 2^this (default 4) */
-    uint32_t reserved;
+    uint32_t lds_buckets;   /* up to this many buckets every key tile holds the splitters in
+                               LDS; more: in HBM behind a coarse table (<= 1024; 0: 1024) */
 } pg_tuning;
 int pg_engine_set_tuning(pg_engine* eng, const pg_tuning* tuning);
 int pg_engine_get_tuning(pg_engine* eng, pg_tuning* tuning);

@@ -67,10 +67,10 @@ class Stats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("max_buckets", ctypes.c_uint32), ("bucket_keys", ctypes.c_uint32),
-                ("walk_run_log2", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("walk_run_log2", ctypes.c_uint32), ("lds_buckets", ctypes.c_uint32)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_[:3]}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class RequestInfo(ctypes.Structure):
@@ -287,7 +287,7 @@ class Engine:
     def set_profiling(self, on):
         check(lib().pg_engine_set_profiling(self.h, int(on)))
 
-    def set_sort(self, max_buckets=1024, bucket_cap=4096):
+    def set_sort(self, max_buckets=8192, bucket_cap=4096):
         """Key sort of the tree build: sample sort into at most max_buckets
         buckets (0: the LSD radix sort), buckets above bucket_cap keys
         sorted in HBM instead of LDS. Both orders are the same."""
@@ -335,8 +335,8 @@ class Engine:
         check(lib().pg_engine_get_tuning(self.h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, max_buckets=1024, bucket_keys=1664, walk_run_log2=4):
-        t = Tuning(max_buckets, bucket_keys, walk_run_log2, 0)
+    def set_tuning(self, max_buckets=8192, bucket_keys=1664, walk_run_log2=4, lds_buckets=1024):
+        t = Tuning(max_buckets, bucket_keys, walk_run_log2, lds_buckets)
         check(lib().pg_engine_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, sims, trial_steps=4):

@@ -81,11 +81,13 @@ struct pg_engine {
     uint32_t* sort_tot = nullptr;  // per-digit totals of one pass
     uint16_t* bkt = nullptr;       // sample sort: bucket of each key
     uint64_t* spl = nullptr;       // sample sort: splitters
+    uint64_t* spc = nullptr;       // sample sort: coarse splitter table (many buckets)
     uint64_t* quant = nullptr;     // sample sort: finer quantiles per bucket
     uint32_t* bcount = nullptr;    // sample sort: keys per bucket
     uint32_t* bfill = nullptr;     // sample sort: reserved keys per bucket
     uint32_t* bbase = nullptr;     // sample sort: first key of each bucket
-    uint32_t max_buckets = kMaxBuckets;  // larger batches use the radix sort
+    uint32_t max_buckets = kMaxWideBuckets;  // larger batches use the radix sort
+    uint32_t lds_buckets = kMaxBuckets;  // more buckets: splitters in HBM (coarse table in LDS)
     uint32_t bucket_keys = kBucketKeys;  // keys per sample-sort bucket (mean)
     uint32_t walk_run_log2 = 4;          // walk blocks per XCD run: 2^walk_run_log2
     uint64_t ext_valid = 0;  // batch whose extent partials the last step left (0: none)
@@ -286,9 +288,9 @@ extern "C" int pg_engine_set_graphs(pg_engine* e, int enable) {
 
 extern "C" int pg_engine_set_sort(pg_engine* e, uint32_t max_buckets, uint32_t bucket_cap) {
     if (!e) return pg_set_error(PG_ERR_ARG, "engine is NULL");
-    if (max_buckets > kMaxBuckets || bucket_cap > kBucketCap)
+    if (max_buckets > kMaxWideBuckets || bucket_cap > kBucketCap)
         return pg_set_error(PG_ERR_ARG, "sample sort limits are %d buckets of %d keys in LDS",
-                            kMaxBuckets, kBucketCap);
+                            kMaxWideBuckets, kBucketCap);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->stream);
@@ -315,17 +317,19 @@ static void invalidate_launches(pg_engine* e) {
 
 extern "C" int pg_engine_set_tuning(pg_engine* e, const pg_tuning* t) {
     if (!e || !t) return pg_set_error(PG_ERR_ARG, "engine or tuning is NULL");
-    if (t->max_buckets > kMaxBuckets || t->bucket_keys < 256 || t->bucket_keys > kBucketCap ||
-        t->walk_run_log2 > 8)
+    if (t->max_buckets > kMaxWideBuckets || t->bucket_keys < 256 || t->bucket_keys > kBucketCap ||
+        t->walk_run_log2 > 8 || t->lds_buckets > kMaxBuckets)
         return pg_set_error(PG_ERR_ARG,
                             "tuning out of range (max_buckets <= %d, bucket_keys in [256, %d], "
-                            "walk_run_log2 <= 8)", kMaxBuckets, kBucketCap);
+                            "walk_run_log2 <= 8, lds_buckets <= %d)",
+                            kMaxWideBuckets, kBucketCap, kMaxBuckets);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     HIP_TRY(hipSetDevice(e->device));
     invalidate_launches(e);
     e->max_buckets = t->max_buckets;
     e->bucket_keys = t->bucket_keys;
     e->walk_run_log2 = t->walk_run_log2;
+    e->lds_buckets = t->lds_buckets ? t->lds_buckets : (uint32_t)kMaxBuckets;
     return PG_OK;
 }
 
@@ -334,7 +338,7 @@ extern "C" int pg_engine_get_tuning(pg_engine* e, pg_tuning* t) {
     t->max_buckets = e->max_buckets;
     t->bucket_keys = e->bucket_keys;
     t->walk_run_log2 = e->walk_run_log2;
-    t->reserved = 0;
+    t->lds_buckets = e->lds_buckets;
     return PG_OK;
 }
 
@@ -431,9 +435,11 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     const uint64_t o_v0 = take(bodies * 4), o_v1 = take(bodies * 4);
     const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies), o_ll = take(bodies);
     const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
-    const uint64_t o_bk = take(bodies * 2), o_sp = take(kMaxBuckets * 8);
-    const uint64_t o_bc = take(kMaxBuckets * 4), o_bf = take(kMaxBuckets * 4);
-    const uint64_t o_bb = take(kMaxBuckets * 4), o_qt = take((uint64_t)kMaxBuckets * kQuantiles * 8);
+    const uint64_t o_bk = take(bodies * 2), o_sp = take(kMaxWideBuckets * 8);
+    const uint64_t o_spc = take(kMaxWideBuckets / kWideGroup * 8);
+    const uint64_t o_bc = take(kMaxWideBuckets * 4), o_bf = take(kMaxWideBuckets * 4);
+    const uint64_t o_bb = take(kMaxWideBuckets * 4);
+    const uint64_t o_qt = take((uint64_t)kMaxWideBuckets * kQuantiles * 8);
     const uint64_t o_nl = take((bodies + 1) * 4), o_tt = take((node_tiles + 2) * 4);
     const uint64_t o_tb = take((node_tiles + 2) * 4), o_ts = take((node_tiles + 2) * 4);
     const uint64_t o_op = take((node_tiles + 2) * kTopSlots * sizeof(Sum4));
@@ -466,6 +472,7 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     e->sort_tot = (uint32_t*)(b + o_st);
     e->bkt = (uint16_t*)(b + o_bk);
     e->spl = (uint64_t*)(b + o_sp);
+    e->spc = (uint64_t*)(b + o_spc);
     e->quant = (uint64_t*)(b + o_qt);
     e->bcount = (uint32_t*)(b + o_bc);
     e->bfill = (uint32_t*)(b + o_bf);
@@ -647,14 +654,35 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
                 desc, e->ext_seg, e->ext_chunk, e->ext_partial, e->sc, bp.sort_tag, bp.nchunks);
         });
     const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
+    const bool wide = bp.buckets > e->lds_buckets;
     launch(e, cap, "keys", [&] {
         k_keys<DIM><<<sort_tiles, kKeyThreads, 0, st>>>(
             desc, e->seg_off, bp.nseg, total, bp.segbits, bp.lmax, e->ext_partial, e->sc,
             INTEG ? 1 : 0, bp.buckets ? bp.sort_tag : 0ull, e->keys[0], e->vals[0], e->sort_cnt,
-            sort_tiles, fused && bp.buckets ? 1 : 0, e->keys[1], bp.buckets, e->bkt, e->bcount,
-            e->quant);
+            sort_tiles, fused && bp.buckets && !wide ? 1 : 0, e->keys[1], bp.buckets, e->bkt,
+            e->bcount, e->quant);
     });
-    if (bp.buckets) {
+    if (bp.buckets && wide) {
+        // (the splitter kernel takes the previous order where the step
+        // counters say it is this batch's, so `fused` plays no part)
+        const uint32_t nb = bp.buckets;
+        launch(e, cap, "splitters", [&] {
+            k_splitters_wide<<<1, kSplitThreads, 0, st>>>(e->keys[1], e->keys[0], total, nb, e->spl,
+                                                          e->spc, e->bcount, e->bfill, e->sc);
+        });
+        launch(e, cap, "bucket_count", [&] {
+            k_bucket_count_wide<<<sort_tiles, kSortThreads, 0, st>>>(
+                e->keys[0], total, e->spl, e->spc, nb, e->bkt, e->bcount, e->keys[1], e->quant,
+                e->sc);
+        });
+        launch(e, cap, "bucket_base", [&] {
+            k_bucket_base<<<1, kSplitThreads, 0, st>>>(e->bcount, nb, e->bbase);
+        });
+        launch(e, cap, "bucket_scatter", [&] {
+            k_bucket_scatter_wide<<<sort_tiles, kSortThreads, 0, st>>>(
+                e->keys[0], e->vals[0], e->bkt, total, e->bbase, e->bfill, e->keys[1], e->vals[1]);
+        });
+    } else if (bp.buckets) {
         const uint32_t nb = bp.buckets;
         if (!fused) {
             launch(e, cap, "splitters", [&] {
@@ -672,6 +700,9 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
                 e->keys[0], e->vals[0], e->bkt, total, e->bcount, e->bfill, e->bbase, nb,
                 e->keys[1], e->vals[1]);
         });
+    }
+    if (bp.buckets) {
+        const uint32_t nb = bp.buckets;
         launch(e, cap, "bucket_sort", [&] {
             k_bucket_sort<DIM><<<nb, kBucketThreads, 0, st>>>(
                 e->keys[1], e->vals[1], e->keys[0], e->vals[0], e->bcount, e->bfill, e->bbase,
@@ -1419,13 +1450,14 @@ extern "C" int pg_engine_autotune(pg_engine* e, pg_sim* const* sims, uint32_t co
     std::vector<std::pair<uint32_t, uint32_t>> sorts;  // (max_buckets, bucket_keys)
     if (tree) {
         for (uint32_t bk : {1280u, 1664u, 2048u})
-            if ((total + bk - 1) / bk <= (uint64_t)kMaxBuckets) sorts.push_back({kMaxBuckets, bk});
+            if ((total + bk - 1) / bk <= (uint64_t)kMaxWideBuckets)
+                sorts.push_back({kMaxWideBuckets, bk});
         sorts.push_back({0u, kBucketKeys});
     } else {
         sorts.push_back({e->max_buckets, e->bucket_keys});
     }
     for (auto so : sorts)
-        for (uint32_t r : runs) cand.push_back(pg_tuning{so.first, so.second, r, 0u});
+        for (uint32_t r : runs) cand.push_back(pg_tuning{so.first, so.second, r, e->lds_buckets});
     pg_tuning keep;
     PG_TRY(pg_engine_get_tuning(e, &keep));
     // the saved state: each simulation's device block and host step fields

@@ -19,8 +19,12 @@ constexpr int kBucketKeys = 1664;  // keys per bucket of the sample sort (on ave
                                    // buckets stay within 2048, the slot sort's reach)
 constexpr int kBucketCap = 4096;   // largest bucket sorted in LDS (larger ones: in HBM)
 constexpr int kBucketThreads = 512;   // threads of a bucket-sorting block (up to 8 keys each)
-constexpr int kMaxBuckets = 1024;  // sample sort up to kMaxBuckets * kBucketKeys keys
+constexpr int kMaxBuckets = 1024;  // buckets whose splitters every key tile holds in LDS
+constexpr int kMaxWideBuckets = 8192;  // more buckets: splitters in HBM behind a coarse table
+constexpr int kWideGroup = 8;      // splitters per coarse-table entry (kMaxWideBuckets / kWideGroup
+                                   // entries fit a tile's LDS)
 constexpr int kSampleOver = 4;     // splitter samples per bucket (sampling from unsorted keys)
+constexpr int kWideSampleOver = 2; // the same with more than kMaxBuckets buckets (128 KB of LDS)
 constexpr int kSplitThreads = 1024;  // threads of the splitter block
 constexpr int kQuantiles = 256;    // per bucket: quantiles of the previous sorted keys
 constexpr int kScanThreads = 1024;  // threads of a per-digit scan over the sort tiles

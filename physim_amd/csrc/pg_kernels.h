@@ -737,6 +737,25 @@ __global__ __launch_bounds__(kBlock) void k_ties(const SegDesc* __restrict__ des
 // larger one in HBM, by one block). Runs of equal keys then take (second key
 // word, index) order in k_ties, the order a stable sort by key gives.
 
+// Ascending bitonic sort of s[0, p2) in LDS (p2 a power of two) by a block of
+// NT threads; the caller has synchronised the block after filling s.
+template <int NT>
+__device__ __forceinline__ void lds_bitonic_sort(uint64_t* s, uint32_t p2) {
+    for (uint32_t k = 2; k <= p2; k <<= 1) {
+        for (uint32_t h = k >> 1; h > 0; h >>= 1) {
+            for (uint32_t i = threadIdx.x; i < p2 / 2; i += NT) {
+                const uint32_t a = ((i & ~(h - 1)) << 1) | (i & (h - 1)), b = a + h;
+                const uint64_t x = s[a], y = s[b];
+                if ((x > y) == ((a & k) == 0)) {
+                    s[a] = y;
+                    s[b] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 __global__ __launch_bounds__(kSplitThreads) void k_splitters(
     const uint64_t* __restrict__ prev, const uint64_t* __restrict__ keys, uint64_t n, uint32_t nb,
     uint64_t* __restrict__ spl, uint32_t* __restrict__ bcount, uint32_t* __restrict__ bfill,
@@ -775,19 +794,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters(
     for (uint32_t j = threadIdx.x; j < p2; j += kSplitThreads)
         s[j] = j < m ? keys[(uint64_t)j * n / m] : ~0ull;
     __syncthreads();
-    for (uint32_t k = 2; k <= p2; k <<= 1) {
-        for (uint32_t h = k >> 1; h > 0; h >>= 1) {
-            for (uint32_t i = threadIdx.x; i < p2 / 2; i += kSplitThreads) {
-                const uint32_t a = ((i & ~(h - 1)) << 1) | (i & (h - 1)), b = a + h;
-                const uint64_t x = s[a], y = s[b];
-                if ((x > y) == ((a & k) == 0)) {
-                    s[a] = y;
-                    s[b] = x;
-                }
-            }
-            __syncthreads();
-        }
-    }
+    lds_bitonic_sort<kSplitThreads>(s, p2);
     for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads) spl[b] = s[(b + 1) * kSampleOver];
 }
 
@@ -995,6 +1002,284 @@ __device__ __forceinline__ void chunk_rank(const uint32_t (&dig)[4], uint32_t (&
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         pos[k] = dig[k] < 256u ? s_dstart[dig[k]] + s_wcnt[wave][dig[k]] + rank[k] : 0u;
+}
+
+// ---------------------------------------------- sample sort, many buckets
+// With more than kMaxBuckets buckets (up to kMaxWideBuckets: a batch of 8.8M
+// bodies takes about 5,300) the splitters no longer fit every key tile's LDS
+// (k_keys and k_bucket_scatter hold a slot per bucket). They stay in HBM,
+// behind a coarse table of every kWideGroup-th splitter that does fit: a
+// key's bucket is a search of the coarse table in LDS, then three halvings
+// over the splitters in L2. A tile ranks its keys by bucket with the stable
+// chunk rank (the bucket less the tile's lowest, 8 bits a pass) rather than
+// through a per-bucket table. The kernels: k_splitters_wide,
+// k_bucket_count_wide, k_bucket_base, k_bucket_scatter_wide, then
+// k_bucket_sort as for fewer buckets; the order is the same.
+
+// (1) One block: the splitters (spl[0, nb - 1)) and the coarse table
+// (spc[i] = spl[kWideGroup i + kWideGroup - 1] for i < (nb - 1) /
+// kWideGroup), from the previous step's sorted keys where they are this
+// batch's and in order, else from kWideSampleOver sorted samples per bucket.
+// Also clears the bucket counters.
+__global__ __launch_bounds__(kSplitThreads) void k_splitters_wide(
+    const uint64_t* __restrict__ prev, const uint64_t* __restrict__ keys, uint64_t n, uint32_t nb,
+    uint64_t* __restrict__ spl, uint64_t* __restrict__ spc, uint32_t* __restrict__ bcount,
+    uint32_t* __restrict__ bfill, StepCounters* __restrict__ sc) {
+    static_assert(kMaxWideBuckets * kWideSampleOver * 8 <= 160 * 1024, "samples in one CU's LDS");
+    __shared__ uint64_t s[kMaxWideBuckets * kWideSampleOver];
+    __shared__ uint32_t s_unsorted;
+    for (uint32_t b = threadIdx.x; b < nb; b += kSplitThreads) {
+        bcount[b] = 0;
+        bfill[b] = 0;
+    }
+    const bool prev_ok = sc->split_prev != 0;
+    if (threadIdx.x == 0) {
+        sc->quantiles = 0;
+        s_unsorted = 0;
+    }
+    const uint32_t ncoarse = (nb - 1) / kWideGroup;
+    __syncthreads();
+    if (prev_ok) {
+        for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads)
+            s[b] = prev[(uint64_t)(b + 1) * n / nb];
+        __syncthreads();
+        bool bad = false;
+        for (uint32_t b = threadIdx.x; b + 2 < nb; b += kSplitThreads) bad |= s[b] > s[b + 1];
+        if (bad) s_unsorted = 1;
+        __syncthreads();
+        if (!s_unsorted) {
+            for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads) spl[b] = s[b];
+            for (uint32_t i = threadIdx.x; i < ncoarse; i += kSplitThreads)
+                spc[i] = s[i * kWideGroup + kWideGroup - 1];
+            if (threadIdx.x == 0) sc->quantiles = 1;
+            return;
+        }
+        __syncthreads();
+    }
+    const uint32_t m = nb * kWideSampleOver;
+    uint32_t p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    for (uint32_t j = threadIdx.x; j < p2; j += kSplitThreads)
+        s[j] = j < m ? keys[(uint64_t)j * n / m] : ~0ull;
+    __syncthreads();
+    lds_bitonic_sort<kSplitThreads>(s, p2);
+    for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads)
+        spl[b] = s[(b + 1) * kWideSampleOver];
+    for (uint32_t i = threadIdx.x; i < ncoarse; i += kSplitThreads)
+        spc[i] = s[(i + 1) * kWideGroup * kWideSampleOver];
+}
+
+// (2) Each key's bucket (the number of splitters <= key) and the bucket
+// sizes. The keys arrive in the previous step's order, so a tile's buckets
+// lie mostly within a window from its first key's: counts in that window go
+// through LDS, the rest (bodies that moved far) straight to HBM. Copies the
+// finer quantiles for the slot sort as k_bucket_count does.
+__global__ __launch_bounds__(kSortThreads) void k_bucket_count_wide(
+    const uint64_t* __restrict__ keys, uint64_t n, const uint64_t* __restrict__ spl,
+    const uint64_t* __restrict__ spc, uint32_t nb, uint16_t* __restrict__ bkt,
+    uint32_t* __restrict__ bcount, const uint64_t* __restrict__ prev,
+    uint64_t* __restrict__ quant, const StepCounters* __restrict__ sc) {
+    constexpr uint32_t kCoarseMax = kMaxWideBuckets / kWideGroup;
+    constexpr uint32_t kWindow = 1024;
+    static_assert((kCoarseMax & (kCoarseMax - 1)) == 0 && (kWideGroup & (kWideGroup - 1)) == 0,
+                  "halvings cover the tables");
+    static_assert(kMaxWideBuckets <= 65536, "16-bit bucket numbers");
+    __shared__ uint64_t s_c[kCoarseMax];
+    __shared__ uint32_t s_h[kWindow];
+    __shared__ uint32_t s_wb;
+    const int tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    uint64_t key[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t idx = base + (uint64_t)k * kSortThreads + tid;
+        key[k] = idx < n ? keys[idx] : 0ull;
+    }
+    const bool copy_quant = sc->quantiles != 0;
+    const uint32_t ns = nb - 1, nc = ns / kWideGroup;
+    for (uint32_t i = tid; i < nc; i += kSortThreads) s_c[i] = spc[i];
+    for (uint32_t i = tid; i < kWindow; i += kSortThreads) s_h[i] = 0;
+    if (copy_quant) {
+        const uint64_t m = (uint64_t)nb * kQuantiles;
+        for (uint64_t t = (uint64_t)blockIdx.x * kSortThreads + tid; t < m;
+             t += (uint64_t)gridDim.x * kSortThreads)
+            quant[t] = prev[t * n / m];
+    }
+    __syncthreads();
+    // coarse: the number c of coarse splitters <= key, so the key's bucket
+    // lies in [kWideGroup c, kWideGroup c + kWideGroup - 1]; fine: how many of
+    // the splitters in between are <= key (fixed halvings, no divergence)
+    uint32_t b[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) b[k] = 0;
+#pragma unroll
+    for (uint32_t step = kCoarseMax / 2; step; step >>= 1) {
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            const uint32_t q = b[k] + step;
+            if (q <= nc && s_c[q - 1] <= key[k]) b[k] = q;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) b[k] *= kWideGroup;
+#pragma unroll
+    for (uint32_t step = kWideGroup / 2; step; step >>= 1) {
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            const uint32_t q = b[k] + step;
+            if (q <= ns && spl[q - 1] <= key[k]) b[k] = q;
+        }
+    }
+    // the window starts a quarter of it below the tile's first key's bucket
+    if (tid == 0) s_wb = b[0] > kWindow / 4 ? b[0] - kWindow / 4 : 0u;
+    __syncthreads();
+    const uint32_t wb = s_wb;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint64_t idx = base + (uint64_t)k * kSortThreads + tid;
+        if (idx < n) {
+            bkt[idx] = (uint16_t)b[k];
+            const uint32_t r = b[k] - wb;  // (wraps past kWindow below the window)
+            if (r < kWindow) atomicAdd(&s_h[r], 1u);
+            else atomicAdd(&bcount[b[k]], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t r = tid; r < kWindow; r += kSortThreads)
+        if (s_h[r]) atomicAdd(&bcount[wb + r], s_h[r]);
+}
+
+// (3) One block: each bucket's first key (exclusive scan of the sizes).
+__global__ __launch_bounds__(kSplitThreads) void k_bucket_base(const uint32_t* __restrict__ bcount,
+                                                               uint32_t nb,
+                                                               uint32_t* __restrict__ bbase) {
+    constexpr int kPer = kMaxWideBuckets / kSplitThreads;
+    static_assert(kPer * kSplitThreads == kMaxWideBuckets, "whole buckets per thread");
+    __shared__ uint32_t s_scan[kSplitThreads / 64];
+    const uint32_t b0 = threadIdx.x * (uint32_t)kPer;
+    uint32_t c[kPer], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        c[j] = b0 + j < nb ? bcount[b0 + j] : 0u;
+        sum += c[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan<kSplitThreads>(sum, s_scan, &total);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        if (b0 + j < nb) bbase[b0 + j] = run;
+        run += c[j];
+    }
+}
+
+// (4) Each tile ranks its keys by bucket in LDS, reserves one range per run
+// of a bucket (bfill, which the bucket sort clears) and writes the runs.
+__global__ __launch_bounds__(kSortThreads) void k_bucket_scatter_wide(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    const uint16_t* __restrict__ bkt, uint64_t n, const uint32_t* __restrict__ bbase,
+    uint32_t* __restrict__ bfill, uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    static_assert(kSortThreads == kBucketThreads && kSortTile == kBucketChunk,
+                  "a tile is one chunk of the chunk rank");
+    static_assert(kSortTile < 65536, "16-bit tile positions");
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint16_t s_b[kSortTile];
+    __shared__ uint16_t s_rid[kSortTile];        // each position's run
+    __shared__ uint16_t s_rhead[kSortTile + 1];  // each run's first position (+ the end)
+    __shared__ __align__(16) uint32_t s_rank[kWaves * 256];  // the chunk rank's counts, then
+                                                              // each run's destination
+    __shared__ uint32_t s_dstart[256], s_dcnt[256], s_scan[kWaves];
+    __shared__ uint64_t s_red[kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    const uint32_t valid_n = (uint32_t)min((uint64_t)kSortTile, n - base);
+    uint64_t key[4];
+    uint32_t val[4], bk[4], dig[4], pos[4];
+    uint32_t lo = 0xFFFFu, hi = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t ci = (uint32_t)wave * 256u + (uint32_t)k * 64u + (uint32_t)lane;
+        const bool valid = ci < valid_n;
+        key[k] = valid ? kin[base + ci] : 0ull;
+        val[k] = valid ? vin[base + ci] : 0u;
+        bk[k] = valid ? (uint32_t)bkt[base + ci] : 0xFFFFu;
+        lo = valid ? min(lo, bk[k]) : lo;
+        hi = valid ? max(hi, bk[k]) : hi;
+    }
+    const uint32_t bmin = 0xFFFFu - (uint32_t)block_max_u64<kSortThreads>(0xFFFFu - lo, s_red);
+    const uint32_t bmax = (uint32_t)block_max_u64<kSortThreads>(hi, s_red);
+    uint32_t(*s_wcnt)[256] = reinterpret_cast<uint32_t(*)[256]>(s_rank);
+    // stable ranks by the low 8 bits of (bucket - bmin), then, where the
+    // tile spans 256 buckets or more, by the high 8
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        dig[k] = bk[k] != 0xFFFFu ? (bk[k] - bmin) & 255u : 256u;
+    chunk_rank(dig, pos, s_wcnt, s_dstart, s_dcnt, s_scan);
+    if (bmax - bmin >= 256u) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (dig[k] < 256u) {
+                s_keys[pos[k]] = key[k];
+                s_vals[pos[k]] = val[k];
+                s_b[pos[k]] = (uint16_t)bk[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t ci = (uint32_t)wave * 256u + (uint32_t)k * 64u + (uint32_t)lane;
+            const bool valid = ci < valid_n;
+            key[k] = valid ? s_keys[ci] : 0ull;
+            val[k] = valid ? s_vals[ci] : 0u;
+            bk[k] = valid ? (uint32_t)s_b[ci] : 0xFFFFu;
+            dig[k] = valid ? (bk[k] - bmin) >> 8 : 256u;
+        }
+        // (the rank's first barrier orders these reads before the writes below)
+        chunk_rank(dig, pos, s_wcnt, s_dstart, s_dcnt, s_scan);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (dig[k] < 256u) {
+            s_keys[pos[k]] = key[k];
+            s_vals[pos[k]] = val[k];
+            s_b[pos[k]] = (uint16_t)bk[k];
+        }
+    }
+    __syncthreads();
+    // runs of one bucket: thread t flags the run heads among positions
+    // 4t .. 4t + 3, a scan numbers them
+    uint32_t head[4], heads = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t p = 4u * (uint32_t)tid + (uint32_t)j;
+        head[j] = p < valid_n && (p == 0 || s_b[p] != s_b[p - 1]) ? 1u : 0u;
+        heads += head[j];
+    }
+    uint32_t nruns;
+    uint32_t rid = block_exclusive_scan<kSortThreads>(heads, s_scan, &nruns);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t p = 4u * (uint32_t)tid + (uint32_t)j;
+        if (p < valid_n) {
+            if (head[j]) s_rhead[rid++] = (uint16_t)p;
+            s_rid[p] = (uint16_t)(rid - 1u);
+        }
+    }
+    if (tid == 0) s_rhead[nruns] = (uint16_t)valid_n;
+    __syncthreads();
+    uint32_t* s_rdst = s_rank;
+    for (uint32_t r = tid; r < nruns; r += kSortThreads) {
+        const uint32_t h = s_rhead[r], b = s_b[h];
+        s_rdst[r] = bbase[b] + atomicAdd(&bfill[b], (uint32_t)s_rhead[r + 1] - h);
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < valid_n; p += kSortThreads) {
+        const uint32_t r = s_rid[p];
+        const uint32_t o = s_rdst[r] + (p - (uint32_t)s_rhead[r]);
+        kout[o] = s_keys[p];
+        vout[o] = s_vals[p];
+    }
 }
 
 // Highest bit in which any of the block's keys differs from `ref` (-1: none).

@@ -542,13 +542,24 @@ def clustered_state(n=40000, seed=13):
     clustered_state,
 ], ids=["plummer60k", "cube100k", "ties", "clusters"])
 def test_sample_sort_equals_radix_sort(engine, make):
-    """The sample sort (buckets in LDS, or in HBM past a small bucket cap)
-    and the LSD radix sort give the same Morton order, tree and steps."""
+    """The sample sort (buckets in LDS, or in HBM past a small bucket cap;
+    splitters in every tile's LDS, or, with more buckets than that holds, in
+    HBM behind a coarse table) and the LSD radix sort give the same Morton
+    order, tree and steps. Small buckets and a small LDS limit put these
+    states on the many-bucket path: up to 391 buckets, which a tile of
+    random-order keys spans (its two-pass rank by bucket)."""
     s = make()
     out = {}
+    default = engine.tuning()
+    small = dict(default, bucket_keys=256, lds_buckets=16)
     try:
-        for name, limits in (("radix", (0, 4096)), ("sample", (1024, 4096)),
-                             ("sample_hbm", (1024, 64)), ("all_hbm", (1024, 0))):
+        for name, tuning, limits in (("radix", default, (0, 4096)),
+                                     ("sample", default, (8192, 4096)),
+                                     ("sample_hbm", default, (8192, 64)),
+                                     ("all_hbm", default, (8192, 0)),
+                                     ("wide", small, (8192, 4096)),
+                                     ("wide_hbm", small, (8192, 64))):
+            engine.set_tuning(**tuning)
             engine.set_sort(*limits)
             sim = P.Sim(engine, s, kind="astro2", theta=0.9, e=0.05, dt=1e-5)
             sim.step(3)
@@ -558,10 +569,35 @@ def test_sample_sort_equals_radix_sort(engine, make):
                          rec.tobytes(), fl.tobytes())
             sim.close()
     finally:
-        engine.set_sort(1024, 4096)
-    for name in ("sample", "sample_hbm", "all_hbm"):
+        engine.set_tuning(**default)
+        engine.set_sort(8192, 4096)
+    for name in ("sample", "sample_hbm", "all_hbm", "wide", "wide_hbm"):
         for a, b, what in zip(out["radix"], out[name], ("state", "keys", "perm", "tree", "nodes")):
             assert a == b, (name, what)
+
+
+def test_many_bucket_sort_over_steps(engine):
+    """A 300,000-body cube of 1,172 buckets (more than a tile's LDS holds
+    splitters for) over 40 steps, graphs on: the first step samples
+    splitters from random-order keys, the later ones take the previous
+    step's quantiles, and every step's order is the radix sort's."""
+    s = small_cube(300000, seed=21)
+    kw = dict(kind="astro2", theta=1.0, e=0.05, dt=1e-4)
+    default = engine.tuning()
+    out = {}
+    try:
+        for name, tuning in (("radix", dict(default, max_buckets=0)),
+                             ("wide", dict(default, bucket_keys=256))):
+            engine.set_tuning(**tuning)
+            sim = P.Sim(engine, s, **kw)
+            sim.step(40)
+            keys, perm = sim.morton()
+            out[name] = (sim.download().tobytes(), keys.tobytes(), perm.tobytes())
+            sim.close()
+    finally:
+        engine.set_tuning(**default)
+    for a, b, what in zip(out["radix"], out["wide"], ("state", "keys", "perm")):
+        assert a == b, what
 
 
 # ---------------------------------------------------------------- batching
