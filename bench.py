@@ -47,15 +47,16 @@ def algorithmic_bytes(kernel, n, nodes):
         "bucket_count": (8 + 2) * n,
         "bucket_scatter": (8 + 4 + 2 + 12) * n,
         "bucket_sort": 24.0 * n,
-        # keys, sorted index, both records read and written (+ velocities on
-        # the first two steps only), node offsets, levels
-        "scan_gather": (8 + 4 + 64 + 64 + 4 + 1) * n,
+        # keys, sorted index, the position record read and written in sorted
+        # order (+ velocities on the first two steps only), node offsets, levels
+        "scan_gather": (8 + 4 + 32 + 32 + 4 + 1) * n,
         # keys, node offsets, levels, leaf records; node records and spans
         # written once
         "nodes": 45.0 * n + 72.0 * nodes,
-        # own position record and previous-position record in, both out (the
-        # tag stays); the tree's records and spans read once
-        "walk": (32.0 + 32.0 + 32.0 + 24.0) * n + 72.0 * nodes,
+        # own position record, sorted index and previous-position record
+        # (through the index) in, both records out in sorted order; the tree's
+        # records and spans read once
+        "walk": (32.0 + 4.0 + 32.0 + 32.0 + 32.0) * n + 72.0 * nodes,
         "direct": 32.0 * n,
     }
     return table.get(kernel, 0.0)
@@ -272,7 +273,8 @@ def ensembles_workload(P, args):
     solo_time = 0.0
     sample = 10
     for sim, n in zip(sims, ns):
-        sim.step(2)
+        # untimed: the eager steps and the capture of the sim's own step graph
+        sim.step(sample)
         eng.synchronize()
         t1 = time.perf_counter()
         sim.step(sample)
@@ -287,7 +289,8 @@ def ensembles_workload(P, args):
                    "bodies": total, "steps": args.ensemble_steps},
         "batched_body_steps_per_s": total * args.ensemble_steps / batched,
         "alone_body_steps_per_s": total * args.ensemble_steps / solo_time,
-        "alone_sample": f"each ensemble alone for {sample} steps after 2, scaled to {args.ensemble_steps}",
+        "alone_sample": f"each ensemble alone for {sample} graph-replayed steps after {sample} "
+                        f"untimed ones, scaled to {args.ensemble_steps}",
         "tuning": {"default": default, "chosen": best,
                    "trials_ms_per_step": [[t, ms] for t, ms in tried]},
     }

@@ -317,7 +317,8 @@ static void invalidate_launches(pg_engine* e) {
 
 extern "C" int pg_engine_set_tuning(pg_engine* e, const pg_tuning* t) {
     if (!e || !t) return pg_set_error(PG_ERR_ARG, "engine or tuning is NULL");
-    if (t->max_buckets > kMaxWideBuckets || t->bucket_keys < 256 || t->bucket_keys > kBucketCap ||
+    if (t->max_buckets > kMaxWideBuckets || t->bucket_keys < kMinBucketKeys ||
+        t->bucket_keys > kBucketCap ||
         t->walk_run_log2 > 8 || t->lds_buckets > kMaxBuckets)
         return pg_set_error(PG_ERR_ARG,
                             "tuning out of range (max_buckets <= %d, bucket_keys in [256, %d], "
@@ -435,11 +436,12 @@ static uint64_t carve_scratch(pg_engine* e, char* b, uint64_t bodies, uint64_t n
     const uint64_t o_v0 = take(bodies * 4), o_v1 = take(bodies * 4);
     const uint64_t o_fn = take((bodies + 1) * 4), o_lo = take(bodies), o_ll = take(bodies);
     const uint64_t o_ss = take(sort_tiles * 256 * 4), o_st = take(256 * 4);
-    const uint64_t o_bk = take(bodies * 2), o_sp = take(kMaxWideBuckets * 8);
-    const uint64_t o_spc = take(kMaxWideBuckets / kWideGroup * 8);
-    const uint64_t o_bc = take(kMaxWideBuckets * 4), o_bf = take(kMaxWideBuckets * 4);
-    const uint64_t o_bb = take(kMaxWideBuckets * 4);
-    const uint64_t o_qt = take((uint64_t)kMaxWideBuckets * kQuantiles * 8);
+    // (a batch of `bodies` keys has at most bodies / kMinBucketKeys + 1 buckets)
+    const uint64_t maxb = std::min<uint64_t>(kMaxWideBuckets, bodies / kMinBucketKeys + 2);
+    const uint64_t o_bk = take(bodies * 2), o_sp = take(maxb * 8);
+    const uint64_t o_spc = take(maxb / kWideGroup * 8 + 8);
+    const uint64_t o_bc = take(maxb * 4), o_bf = take(maxb * 4), o_bb = take(maxb * 4);
+    const uint64_t o_qt = take(maxb * kQuantiles * 8);
     const uint64_t o_nl = take((bodies + 1) * 4), o_tt = take((node_tiles + 2) * 4);
     const uint64_t o_tb = take((node_tiles + 2) * 4), o_ts = take((node_tiles + 2) * 4);
     const uint64_t o_op = take((node_tiles + 2) * kTopSlots * sizeof(Sum4));
@@ -760,7 +762,7 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
         k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
             desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
             e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2,
-            e->leafpos, e->node_span);
+            e->leafpos, e->node_span, svals);
     });
 }
 

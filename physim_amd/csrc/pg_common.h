@@ -18,6 +18,7 @@ constexpr int kKeyThreads = 512;   // threads of a key-building block (one sort 
 constexpr int kBucketKeys = 1664;  // keys per bucket of the sample sort (on average; steady-state
                                    // buckets stay within 2048, the slot sort's reach)
 constexpr int kBucketCap = 4096;   // largest bucket sorted in LDS (larger ones: in HBM)
+constexpr int kMinBucketKeys = 256;  // smallest mean bucket a tuning may ask for
 constexpr int kBucketThreads = 512;   // threads of a bucket-sorting block (up to 8 keys each)
 constexpr int kMaxBuckets = 1024;  // buckets whose splitters every key tile holds in LDS
 constexpr int kMaxWideBuckets = 8192;  // more buckets: splitters in HBM behind a coarse table

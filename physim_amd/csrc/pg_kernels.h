@@ -1883,11 +1883,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 const uint64_t j = i - d.off;
                 // position and mass as one record in sorted order: the node
                 // builder's input and the walk's leaves (the walk writes the
-                // state's own arrays)
+                // state's own arrays, and moves each body's record of previous
+                // position and tag itself: it is that record's only reader)
                 leafpos[i] = d.spos[src];
-                // the previous position (meaningful where the Stormer history
-                // is valid) and the tag: one record
-                d.dprv[j] = d.sprv[src];
                 if (d.st->vperm) {
                     d.dvx[j] = d.svx[src]; d.dvy[j] = d.svy[src]; d.dvz[j] = d.svz[src];
                 }
@@ -2785,7 +2783,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                                                  const uint8_t* __restrict__ leaf_lv,
                                                  uint32_t run_log2,
                                                  const double4* __restrict__ leafpos,
-                                                 const uint2* __restrict__ node_span) {
+                                                 const uint2* __restrict__ node_span,
+                                                 const uint32_t* __restrict__ svals) {
     // Workgroups go to the 8 XCDs in turn. Each XCD takes runs of 2^run_log2
     // consecutive blocks (16 by default: 4096 sorted bodies), so neighbouring
     // walks share their node records in one L2, while the runs of a batch's
@@ -2803,18 +2802,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     uint32_t s = 0;
     double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
     bool fixed = true;
-    uint64_t j = 0;
+    uint64_t j = 0, src = 0;
     if (valid) {
-        s = seg_of_key(skeys[g], segbits);
+        const uint64_t key = skeys[g];
+        const uint32_t v = svals[g];
+        s = seg_of_key(key, segbits);
         valid = s < nseg;
+        src = v;
     }
     if (valid) {
         const SegDesc& d = desc[s];
         j = g - d.off;
+        // the body's index in the previous order (k_scan_gather flags one
+        // outside its segment; such a body is not stepped)
+        src -= d.off;
+        valid = src < d.n;
+    }
+    if (valid) {
+        const SegDesc& d = desc[s];
         const double4 p = leafpos[g];
         x = p.x; y = p.y; z = p.z;
         m = p.w;
-        fixed = tag_fixed(d.dprv[j].w);
+        fixed = tag_fixed(d.sprv[src].w);
     }
     Kahan3 ka;
     ka.zero();
@@ -2949,7 +2958,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 az = (double)ka.sz - (double)ka.cz;
             }
         }
-        const double4 pr = d.dprv[j];  // previous position and tag
+        const double4 pr = d.sprv[src];  // previous position and tag, previous order
         if (d.count) d.counts[tag_id(pr.w)] = count;
         if (INTEGRATE) {
             // verlet.rs:24-82. After the Taylor first step the velocity is
@@ -2979,8 +2988,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         } else {
             double* o = acc_out + 3 * (d.off + tag_id(pr.w));
             o[0] = ax; o[1] = ay; o[2] = az;
-            // the permuted state keeps its positions and masses
+            // the permuted state keeps its positions, masses and history
             d.dpos[j] = make_double4(x, y, z, m);
+            d.dprv[j] = pr;
         }
     }
     if (INTEGRATE) {
