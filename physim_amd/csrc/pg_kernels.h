@@ -2318,6 +2318,23 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     }
     const uint32_t lo_after = lo_level[min((uint64_t)b1, total - 1)];
     if (threadIdx.x == kNodeThreads - 1u) s_lonext = b1 < total ? lo_after : 0u;
+    // Loads for the work between the staging and the build, issued now so
+    // they arrive during the staging: the next tile's row of the scanned
+    // per-level table (tile-spanning nodes), the tile's first and last keys
+    // and the extent (the common path), and the body after the tile (the
+    // coincidence check of the tile's last body).
+    const bool span_t = threadIdx.x < (uint32_t)kTopSlots && b1 < total;
+    const uint32_t span_nx = span_t ? lvl_first[(uint64_t)(tile + 1) * kTopSlots + threadIdx.x] : ~0u;
+    constexpr uint32_t kPathThread = 2u * 64u;
+    uint64_t path_ka = 0, path_kz = 0, path_e = 0;
+    if (threadIdx.x == kPathThread) {
+        path_ka = skeys_g[b0];
+        path_kz = skeys_g[b1 - 1];
+        path_e = d.st->extent_bits;
+    }
+    const uint32_t last_li = nb - 1u;  // (a tile holds a body)
+    double4 after = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (threadIdx.x == (last_li & (kNodeThreads - 1u)) && b1 < d.off + d.n) after = leafpos[b1];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         const uint32_t li = (uint32_t)r * kNodeThreads + threadIdx.x;
@@ -2364,53 +2381,59 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
             }
         }
     }
-    __syncthreads();
     // The node of level L < s_lonext that holds the tile's last body runs
     // into the next tile: it ends before the first later body starting a
     // node at its level or above (the scanned per-level table), and its skip
-    // is that body's first node. One thread per level, before the build.
-    if (threadIdx.x < (uint32_t)kTopSlots && threadIdx.x < s_lonext && b1 < total) {
-        const uint32_t nx = lvl_first[(uint64_t)(tile + 1) * kTopSlots + threadIdx.x];
-        const uint64_t nxt = nx == ~0u ? total : (uint64_t)nx;
+    // is that body's first node. That body lies in this segment or is the
+    // next segment's first body (lo 0), whose tile follows this segment's
+    // last. One thread per level, before the build.
+    if (span_t && threadIdx.x < lo_after) {
+        const uint64_t nxt = span_nx == ~0u ? total : (uint64_t)span_nx;
         s_span_last[threadIdx.x] = (uint32_t)(nxt - 1);
-        s_span_skip[threadIdx.x] =
-            nxt >= total ? tile_base[tiles]
-                         : node_local[nxt] + tile_base[tile_of(desc, seg_off, nseg, nxt)];
-    }
-    // the reference merges bodies within 1e-9 on every axis
-    // (octree.rs:69-80); the device reports them instead (the tile's last
-    // body against the next body of its segment, read from memory)
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        const uint32_t li = (uint32_t)r * kNodeThreads + threadIdx.x;
-        if (li < nb && b0 + li + 1u < d.off + d.n) {
-            const double4 q = li + 1u < nb ? make_double4(s_bx[li + 1], s_by[li + 1], s_bz[li + 1], 0.0)
-                                           : leafpos[b1];
-            if (fabs(st_p[r].x - q.x) < 1e-9 && fabs(st_p[r].y - q.y) < 1e-9 &&
-                fabs(st_p[r].z - q.z) < 1e-9)
-                atomicOr(&d.st->err, kErrCoincident);
+        uint32_t skip = tile_base[tiles];
+        if (nxt < total) {
+            const uint64_t rel = nxt - d.off;
+            const uint32_t t2 =
+                d.tile0 + (uint32_t)(rel < d.n ? rel / kNodeTile : (d.n + kNodeTile - 1) / kNodeTile);
+            skip = node_local[nxt] + tile_base[t2];
         }
+        s_span_skip[threadIdx.x] = skip;
     }
-    // the key of this thread's first node's owner (node-parallel build)
-    const uint64_t key0 =
-        node_par && threadIdx.x < nloc ? skeys_g[b0 + (s_ilist[threadIdx.x] & 1023u)] : 0ull;
-    if (threadIdx.x == 2u * 64u) {
+    if (threadIdx.x == kPathThread) {
         // every node of the tile lies on or below this path: its cells are
         // the ancestors of every body
-        const uint64_t ka = nb ? skeys_g[b0] : 0ull, kz = nb ? skeys_g[b1 - 1] : 0ull;
-        const int lt = min(key_lcp<DIM>(ka, kz, lmax), kPathLevels - 1);
-        double e = bits_f64(d.st->extent_bits), x = 0.0, y = 0.0, z = 0.0;
+        const int lt = min(key_lcp<DIM>(path_ka, path_kz, lmax), kPathLevels - 1);
+        double e = bits_f64(path_e), x = 0.0, y = 0.0, z = 0.0;
         for (int l = 0;; ++l) {
             s_path[l][0] = e;
             s_path[l][1] = x;
             s_path[l][2] = y;
             s_path[l][3] = z;
             if (l == lt) break;
-            descend<DIM>(ka, DeepKey{}, lmax, l, l + 1, e, x, y, z);  // (lt < lmax)
+            descend<DIM>(path_ka, DeepKey{}, lmax, l, l + 1, e, x, y, z);  // (lt < lmax)
         }
         s_lt = lt;
     }
+    // (this barrier also publishes the common path and the tile-spanning
+    // levels: the build reads nothing written after it)
     __syncthreads();
+    // the key of this thread's first node's owner (node-parallel build)
+    const uint64_t key0 =
+        node_par && threadIdx.x < nloc ? skeys_g[b0 + (s_ilist[threadIdx.x] & 1023u)] : 0ull;
+    // the reference merges bodies within 1e-9 on every axis
+    // (octree.rs:69-80); the device reports them instead (the tile's last
+    // body against the next body of its segment, loaded above)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t li = (uint32_t)r * kNodeThreads + threadIdx.x;
+        if (li < nb && b0 + li + 1u < d.off + d.n) {
+            const double4 q = li + 1u < nb ? make_double4(s_bx[li + 1], s_by[li + 1], s_bz[li + 1], 0.0)
+                                           : after;
+            if (fabs(st_p[r].x - q.x) < 1e-9 && fabs(st_p[r].y - q.y) < 1e-9 &&
+                fabs(st_p[r].z - q.z) < 1e-9)
+                atomicOr(&d.st->err, kErrCoincident);
+        }
+    }
     const int lt = s_lt;
 
     // Chunk c's start mask at level L.
@@ -2803,6 +2826,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
     bool fixed = true;
     uint64_t j = 0, src = 0;
+    double4 pr = make_double4(0.0, 0.0, 0.0, 0.0);
     if (valid) {
         const uint64_t key = skeys[g];
         const uint32_t v = svals[g];
@@ -2823,7 +2847,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         const double4 p = leafpos[g];
         x = p.x; y = p.y; z = p.z;
         m = p.w;
-        fixed = tag_fixed(d.sprv[src].w);
+        pr = d.sprv[src];  // previous position and tag, previous order
+        fixed = tag_fixed(pr.w);
     }
     Kahan3 ka;
     ka.zero();
@@ -2958,7 +2983,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 az = (double)ka.sz - (double)ka.cz;
             }
         }
-        const double4 pr = d.sprv[src];  // previous position and tag, previous order
         if (d.count) d.counts[tag_id(pr.w)] = count;
         if (INTEGRATE) {
             // verlet.rs:24-82. After the Taylor first step the velocity is
