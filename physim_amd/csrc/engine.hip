@@ -656,27 +656,42 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
                 desc, e->ext_seg, e->ext_chunk, e->ext_partial, e->sc, bp.sort_tag, bp.nchunks);
         });
     const unsigned sort_tiles = (unsigned)((total + kSortTile - 1) / kSortTile);
+    // more buckets than a key tile holds splitters for: the splitters in HBM
+    // behind a coarse table (k_splitters_wide), counted by k_keys where the
+    // previous step's sorted keys give them (fused), else by k_bucket_count_wide
     const bool wide = bp.buckets > e->lds_buckets;
+    const bool fuse = fused && bp.buckets;
+    if (fuse && wide)
+        launch(e, cap, "splitters", [&] {
+            k_splitters_wide<<<1, kSplitThreads, 0, st>>>(e->keys[1], e->keys[0], total,
+                                                          bp.buckets, e->spl, e->spc, e->bcount,
+                                                          e->bfill, e->sc, 1);
+        });
     launch(e, cap, "keys", [&] {
-        k_keys<DIM><<<sort_tiles, kKeyThreads, 0, st>>>(
-            desc, e->seg_off, bp.nseg, total, bp.segbits, bp.lmax, e->ext_partial, e->sc,
-            INTEG ? 1 : 0, bp.buckets ? bp.sort_tag : 0ull, e->keys[0], e->vals[0], e->sort_cnt,
-            sort_tiles, fused && bp.buckets && !wide ? 1 : 0, e->keys[1], bp.buckets, e->bkt,
-            e->bcount, e->quant);
+#define PG_KEYS(F)                                                                             \
+    k_keys<DIM, F><<<sort_tiles, kKeyThreads, 0, st>>>(                                        \
+        desc, e->seg_off, bp.nseg, total, bp.segbits, bp.lmax, e->ext_partial, e->sc,          \
+        INTEG ? 1 : 0, bp.buckets ? bp.sort_tag : 0ull, e->keys[0], e->vals[0], e->sort_cnt,   \
+        sort_tiles, e->keys[1], bp.buckets, e->bkt, e->bcount, e->quant, e->spl, e->spc)
+        if (!fuse) PG_KEYS(0);
+        else if (!wide) PG_KEYS(1);
+        else PG_KEYS(2);
+#undef PG_KEYS
     });
     if (bp.buckets && wide) {
-        // (the splitter kernel takes the previous order where the step
-        // counters say it is this batch's, so `fused` plays no part)
         const uint32_t nb = bp.buckets;
-        launch(e, cap, "splitters", [&] {
-            k_splitters_wide<<<1, kSplitThreads, 0, st>>>(e->keys[1], e->keys[0], total, nb, e->spl,
-                                                          e->spc, e->bcount, e->bfill, e->sc);
-        });
-        launch(e, cap, "bucket_count", [&] {
-            k_bucket_count_wide<<<sort_tiles, kSortThreads, 0, st>>>(
-                e->keys[0], total, e->spl, e->spc, nb, e->bkt, e->bcount, e->keys[1], e->quant,
-                e->sc);
-        });
+        if (!fuse) {
+            launch(e, cap, "splitters", [&] {
+                k_splitters_wide<<<1, kSplitThreads, 0, st>>>(e->keys[1], e->keys[0], total, nb,
+                                                              e->spl, e->spc, e->bcount, e->bfill,
+                                                              e->sc, 0);
+            });
+            launch(e, cap, "bucket_count", [&] {
+                k_bucket_count_wide<<<sort_tiles, kSortThreads, 0, st>>>(
+                    e->keys[0], total, e->spl, e->spc, nb, e->bkt, e->bcount, e->keys[1],
+                    e->quant, e->sc);
+            });
+        }
         launch(e, cap, "bucket_base", [&] {
             k_bucket_base<<<1, kSplitThreads, 0, st>>>(e->bcount, nb, e->bbase);
         });

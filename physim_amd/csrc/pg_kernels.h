@@ -333,6 +333,40 @@ __device__ __forceinline__ void buckets_of(const uint64_t* s_spl, uint32_t nb, c
     }
 }
 
+// The same count with more than kMaxBuckets buckets: a search of the coarse
+// table s_c (every kWideGroup-th splitter, (nb - 1) / kWideGroup of them) for
+// the group c of the key's bucket, then of the group's splitters in spl (L2):
+// the bucket lies in [kWideGroup c, kWideGroup c + kWideGroup - 1]. Fixed
+// halvings, no divergence.
+template <int K>
+__device__ __forceinline__ void buckets_of_wide(const uint64_t* s_c, const uint64_t* __restrict__ spl,
+                                                uint32_t nb, const uint64_t* key, uint32_t* b) {
+    constexpr uint32_t kCoarseMax = kMaxWideBuckets / kWideGroup;
+    static_assert((kCoarseMax & (kCoarseMax - 1)) == 0 && (kWideGroup & (kWideGroup - 1)) == 0,
+                  "halvings cover the tables");
+    const uint32_t ns = nb - 1, nc = ns / kWideGroup;
+#pragma unroll
+    for (int k = 0; k < K; ++k) b[k] = 0;
+#pragma unroll
+    for (uint32_t step = kCoarseMax / 2; step; step >>= 1) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t q = b[k] + step;
+            if (q <= nc && s_c[q - 1] <= key[k]) b[k] = q;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) b[k] *= kWideGroup;
+#pragma unroll
+    for (uint32_t step = kWideGroup / 2; step; step >>= 1) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t q = b[k] + step;
+            if (q <= ns && spl[q - 1] <= key[k]) b[k] = q;
+        }
+    }
+}
+
 // Morton key by exact FP64 descent (octree.rs:160-215 / quadtree.rs:161-194):
 // at each level the octant is (x > cx) | (y > cy) << 1 [| (z > cz) << 2] and
 // the child centre is c -/+ extent/2, rounded as the reference rounds it;
@@ -340,11 +374,13 @@ __device__ __forceinline__ void buckets_of(const uint64_t* s_spl, uint32_t nb, c
 // coordinates (axis_digits).
 //
 // For the radix sort (sort_tag 0) each tile also counts its digits of the
-// first pass. With `fuse` (the sample sort, the batch's previous sorted keys
+// first pass. With FUSE (the sample sort, the batch's previous sorted keys
 // in `prev`) it does the work of k_splitters and k_bucket_count instead: the
 // same splitters (prev at the quantile positions), each key's bucket, the
 // bucket sizes (one atomic per tile and bucket; the bucket sort leaves them
-// zero) and the finer quantiles for the slot sort.
+// zero) and the finer quantiles for the slot sort (FUSE 1). FUSE 2 (more than
+// kMaxBuckets buckets) does the work of k_bucket_count_wide: the splitters
+// (spl) and their coarse table (spc) come from k_splitters_wide, run first.
 // One block per radix-sort tile of kSortTile bodies (the batch's segments are
 // concatenated), which also writes the tile's digit counts of the first radix
 // pass (digit-major: cnt[digit * tiles + tile]). The extent
@@ -352,24 +388,37 @@ __device__ __forceinline__ void buckets_of(const uint64_t* s_spl, uint32_t nb, c
 // reduced by the whole block when the tile lies in one segment; the thread of
 // a segment's first body stores it and opens the segment's step, and block 0
 // resets the batch counters.
-template <int DIM>
+template <int DIM, int FUSE>
 __global__ __launch_bounds__(kKeyThreads) void k_keys(
     const SegDesc* __restrict__ desc, const uint64_t* __restrict__ seg_off, uint32_t nseg,
     uint64_t total, int segbits, int lmax, const uint64_t* __restrict__ partial,
     StepCounters* __restrict__ sc, int integrate, uint64_t sort_tag, uint64_t* __restrict__ keys,
-    uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt, uint32_t tiles, int fuse,
+    uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt, uint32_t tiles,
     const uint64_t* __restrict__ prev, uint32_t nb, uint16_t* __restrict__ bkt,
-    uint32_t* __restrict__ bcount, uint64_t* __restrict__ quant) {
+    uint32_t* __restrict__ bcount, uint64_t* __restrict__ quant, const uint64_t* __restrict__ spl,
+    const uint64_t* __restrict__ spc) {
+    static_assert(kMaxWideBuckets / kWideGroup <= kMaxBuckets, "the coarse table in s_spl");
     __shared__ uint64_t s_wave[kKeyThreads / 64];
     __shared__ uint32_t s_h[256];
+    // the splitters (FUSE 1) or the coarse table (FUSE 2)
     __shared__ uint64_t s_spl[kMaxBuckets];
+    // bucket counts: of all buckets (FUSE 1), or of a window from s_wb (FUSE 2)
     __shared__ uint32_t s_bh[kMaxBuckets];
+    __shared__ uint32_t s_wb;
     if (threadIdx.x < 256) s_h[threadIdx.x] = 0;
-    if (fuse) {
+    if (FUSE == 1) {
         for (uint32_t b = threadIdx.x; b < nb; b += kKeyThreads) {
             s_bh[b] = 0;
             if (b + 1 < nb) s_spl[b] = prev[(uint64_t)(b + 1) * total / nb];
         }
+    } else if (FUSE == 2) {
+        const uint32_t nc = (nb - 1) / kWideGroup;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxBuckets; i += kKeyThreads) {
+            s_bh[i] = 0;
+            if (i < nc) s_spl[i] = spc[i];
+        }
+    }
+    if (FUSE) {
         const uint64_t m = (uint64_t)nb * kQuantiles;
         for (uint64_t t = (uint64_t)blockIdx.x * kKeyThreads + threadIdx.x; t < m;
              t += (uint64_t)gridDim.x * kKeyThreads)
@@ -385,7 +434,7 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
         // batch's previous step (sort_tag 0: the radix sort, which leaves none)
         sc->split_prev = sort_tag != 0 && sc->sort_tag == sort_tag ? 1u : 0u;
         sc->sort_tag = sort_tag;
-        if (fuse) sc->quantiles = 1;
+        if (FUSE) sc->quantiles = 1;
         sc->ext_tag = 0;  // the partials are read here; k_scan_gather clears them
     }
     // the tile's extent when it lies in one segment (block-uniform branch)
@@ -456,31 +505,47 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
     __syncthreads();
     if (sort_tag == 0) {
         if (threadIdx.x < 256) cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
-    } else if (fuse) {
+    } else if (FUSE) {
         constexpr int kItems = kSortTile / kKeyThreads;
         uint32_t bk[kItems];
-        buckets_of<kItems>(s_spl, nb, tk, bk);
+        // the first bucket counted in LDS: 0 (every bucket), or, with more
+        // buckets, a quarter of the window below the tile's first key's (the
+        // keys arrive in the previous step's order; bodies that moved far
+        // count straight to HBM)
+        uint32_t wb = 0;
+        if (FUSE == 1) {
+            buckets_of<kItems>(s_spl, nb, tk, bk);
+        } else {
+            buckets_of_wide<kItems>(s_spl, spl, nb, tk, bk);
+            if (threadIdx.x == 0) s_wb = bk[0] > kMaxBuckets / 4 ? bk[0] - kMaxBuckets / 4 : 0u;
+            __syncthreads();
+            wb = s_wb;
+        }
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
             const bool valid = g < total;
             const uint32_t b = valid ? bk[k] : 0u;
             if (valid) bkt[g] = (uint16_t)b;
+            const uint32_t r = b - wb;  // (wraps past the window below it)
+            const bool in = valid && r < (uint32_t)kMaxBuckets;
             // a wave's keys mostly share a bucket (nearly sorted input): then
             // one atomic for the wave, else one per key
-            const uint64_t vm = __ballot(valid);
-            const uint32_t bf = (uint32_t)__builtin_amdgcn_readlane(
-                (int)b, vm ? __ffsll((long long)vm) - 1 : 0);
-            if (__ballot(valid && b != bf) == 0) {
+            const uint64_t vm = __ballot(in);
+            const uint32_t rf = (uint32_t)__builtin_amdgcn_readlane(
+                (int)r, vm ? __ffsll((long long)vm) - 1 : 0);
+            if (__ballot(in && r != rf) == 0) {
                 if (vm && (int)(threadIdx.x & 63) == __ffsll((long long)vm) - 1)
-                    atomicAdd(&s_bh[bf], (uint32_t)__popcll(vm));
-            } else if (valid) {
-                atomicAdd(&s_bh[b], 1u);
+                    atomicAdd(&s_bh[rf], (uint32_t)__popcll(vm));
+            } else if (in) {
+                atomicAdd(&s_bh[r], 1u);
             }
+            if (valid && !in) atomicAdd(&bcount[b], 1u);
         }
         __syncthreads();
-        for (uint32_t b = threadIdx.x; b < nb; b += kKeyThreads)
-            if (s_bh[b]) atomicAdd(&bcount[b], s_bh[b]);
+        const uint32_t nw = min(nb - wb, (uint32_t)kMaxBuckets);
+        for (uint32_t r = threadIdx.x; r < nw; r += kKeyThreads)
+            if (s_bh[r]) atomicAdd(&bcount[wb + r], s_bh[r]);
     }
 }
 
@@ -1024,7 +1089,7 @@ __device__ __forceinline__ void chunk_rank(const uint32_t (&dig)[4], uint32_t (&
 __global__ __launch_bounds__(kSplitThreads) void k_splitters_wide(
     const uint64_t* __restrict__ prev, const uint64_t* __restrict__ keys, uint64_t n, uint32_t nb,
     uint64_t* __restrict__ spl, uint64_t* __restrict__ spc, uint32_t* __restrict__ bcount,
-    uint32_t* __restrict__ bfill, StepCounters* __restrict__ sc) {
+    uint32_t* __restrict__ bfill, StepCounters* __restrict__ sc, int fused) {
     static_assert(kMaxWideBuckets * kWideSampleOver * 8 <= 160 * 1024, "samples in one CU's LDS");
     __shared__ uint64_t s[kMaxWideBuckets * kWideSampleOver];
     __shared__ uint32_t s_unsorted;
@@ -1032,9 +1097,11 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters_wide(
         bcount[b] = 0;
         bfill[b] = 0;
     }
-    const bool prev_ok = sc->split_prev != 0;
+    // (fused: this step's keys are not built yet and k_keys takes the
+    // quantiles; the caller knows the previous step's sorted keys are there)
+    const bool prev_ok = fused || sc->split_prev != 0;
     if (threadIdx.x == 0) {
-        sc->quantiles = 0;
+        if (!fused) sc->quantiles = 0;
         s_unsorted = 0;
     }
     const uint32_t ncoarse = (nb - 1) / kWideGroup;
@@ -1044,14 +1111,15 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters_wide(
             s[b] = prev[(uint64_t)(b + 1) * n / nb];
         __syncthreads();
         bool bad = false;
-        for (uint32_t b = threadIdx.x; b + 2 < nb; b += kSplitThreads) bad |= s[b] > s[b + 1];
+        if (!fused)
+            for (uint32_t b = threadIdx.x; b + 2 < nb; b += kSplitThreads) bad |= s[b] > s[b + 1];
         if (bad) s_unsorted = 1;
         __syncthreads();
         if (!s_unsorted) {
             for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads) spl[b] = s[b];
             for (uint32_t i = threadIdx.x; i < ncoarse; i += kSplitThreads)
                 spc[i] = s[i * kWideGroup + kWideGroup - 1];
-            if (threadIdx.x == 0) sc->quantiles = 1;
+            if (threadIdx.x == 0 && !fused) sc->quantiles = 1;
             return;
         }
         __syncthreads();
@@ -1081,8 +1149,6 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_count_wide(
     uint64_t* __restrict__ quant, const StepCounters* __restrict__ sc) {
     constexpr uint32_t kCoarseMax = kMaxWideBuckets / kWideGroup;
     constexpr uint32_t kWindow = 1024;
-    static_assert((kCoarseMax & (kCoarseMax - 1)) == 0 && (kWideGroup & (kWideGroup - 1)) == 0,
-                  "halvings cover the tables");
     static_assert(kMaxWideBuckets <= 65536, "16-bit bucket numbers");
     __shared__ uint64_t s_c[kCoarseMax];
     __shared__ uint32_t s_h[kWindow];
@@ -1096,7 +1162,7 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_count_wide(
         key[k] = idx < n ? keys[idx] : 0ull;
     }
     const bool copy_quant = sc->quantiles != 0;
-    const uint32_t ns = nb - 1, nc = ns / kWideGroup;
+    const uint32_t nc = (nb - 1) / kWideGroup;
     for (uint32_t i = tid; i < nc; i += kSortThreads) s_c[i] = spc[i];
     for (uint32_t i = tid; i < kWindow; i += kSortThreads) s_h[i] = 0;
     if (copy_quant) {
@@ -1106,30 +1172,8 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_count_wide(
             quant[t] = prev[t * n / m];
     }
     __syncthreads();
-    // coarse: the number c of coarse splitters <= key, so the key's bucket
-    // lies in [kWideGroup c, kWideGroup c + kWideGroup - 1]; fine: how many of
-    // the splitters in between are <= key (fixed halvings, no divergence)
     uint32_t b[kSortItems];
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) b[k] = 0;
-#pragma unroll
-    for (uint32_t step = kCoarseMax / 2; step; step >>= 1) {
-#pragma unroll
-        for (int k = 0; k < kSortItems; ++k) {
-            const uint32_t q = b[k] + step;
-            if (q <= nc && s_c[q - 1] <= key[k]) b[k] = q;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) b[k] *= kWideGroup;
-#pragma unroll
-    for (uint32_t step = kWideGroup / 2; step; step >>= 1) {
-#pragma unroll
-        for (int k = 0; k < kSortItems; ++k) {
-            const uint32_t q = b[k] + step;
-            if (q <= ns && spl[q - 1] <= key[k]) b[k] = q;
-        }
-    }
+    buckets_of_wide<kSortItems>(s_c, spl, nb, key, b);
     // the window starts a quarter of it below the tile's first key's bucket
     if (tid == 0) s_wb = b[0] > kWindow / 4 ? b[0] - kWindow / 4 : 0u;
     __syncthreads();
