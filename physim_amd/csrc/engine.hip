@@ -632,6 +632,7 @@ struct BatchPlan {
     uint32_t ntiles = 0;  // node tiles
     uint32_t max_split = 1;  // direct sum: the most source splits of a segment
     bool integrate = true;
+    bool near = false;  // some segment walks with a near field (the walk that tests for it)
 };
 
 static inline unsigned grid_for(uint64_t n, unsigned cap) {
@@ -774,10 +775,16 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
             e->out_part, e->in_part, e->tot_part, e->sc);
     });
     launch(e, cap, "walk", [&] {
-        k_walk<DIM, FP64, INTEG><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
-            desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
-            e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2,
-            e->leafpos, e->node_span, svals);
+        if (bp.near)
+            k_walk<DIM, FP64, INTEG, true><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
+                desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
+                e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2,
+                e->leafpos, e->node_span, svals);
+        else
+            k_walk<DIM, FP64, INTEG, false><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
+                desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
+                e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2,
+                e->leafpos, e->node_span, svals);
     });
 }
 
@@ -886,7 +893,10 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     bp.segbits = bp.nseg > 1 ? ceil_log2(bp.nseg) : 0;
     bp.lmax = key_lmax(bp.dim, bp.segbits);
     bp.passes = (bp.segbits + bp.dim * bp.lmax + 7) / 8;
-    for (pg_sim* s : sims) bp.total += s->n;
+    for (pg_sim* s : sims) {
+        bp.total += s->n;
+        bp.near = bp.near || s->near > 0.0;
+    }
     if (bp.total >= (1ull << 32))
         return pg_set_error(PG_ERR_ARG, "a batch holds at most 2^32-1 bodies");
     {
@@ -1067,7 +1077,7 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
         // signature: sims, parities, sizes, flags and the scratch it was captured on
         std::vector<uint64_t> sig;
         sig.push_back((uint64_t)(uintptr_t)e->scratch);
-        sig.push_back((uint64_t)bp.kind * 16 + bp.precision);
+        sig.push_back(((uint64_t)bp.kind * 16 + bp.precision) * 2 + (bp.near ? 1 : 0));
         for (pg_sim* s : sims) {
             sig.push_back((uint64_t)(uintptr_t)s);
             sig.push_back(s->n * 4 + (uint64_t)s->cur * 2 + (s->counting ? 1 : 0));

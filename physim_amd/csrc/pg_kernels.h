@@ -2833,13 +2833,14 @@ __device__ __forceinline__ void pair_fp64(double ax, double ay, double az, doubl
 // it. Then the Stormer update (verlet.rs:24-82) in place on the permuted
 // state, or the acceleration into acc_out (upload order) without it.
 // (a waves-per-EU hint of 6: the three records in flight fit the scalar
-// registers with almost no spills, at seven waves per SIMD; a hint of 8 spilled
-// 34 scalar registers into vector lanes, whose reloads are VALU instructions
-// in every iteration and cost more than the eighth wave hid)
+// registers with almost no spills, at seven waves per SIMD, eight for the
+// mixed walk without the near-field test; a hint of 8 spilled 34 scalar
+// registers into vector lanes, whose reloads are VALU instructions in every
+// iteration and cost more than the eighth wave hid)
 // Levels below a body's leaf at which its near field still opens cells.
 constexpr int kNearLevels = 2;
 
-template <int DIM, bool FP64, bool INTEGRATE>
+template <int DIM, bool FP64, bool INTEGRATE, bool NEAR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_walk(const SegDesc* __restrict__ desc, uint64_t total,
                                                  const uint64_t* __restrict__ skeys, int segbits,
                                                  const uint32_t* __restrict__ first_node,
@@ -2923,7 +2924,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         // much smaller cells stay under the opening test alone: a body alone
         // in a large leaf (far out in a halo) would otherwise take the finely
         // split core beside it body by body.
-        const bool nearf = d0.near > 0.0;
+        // (NEAR: some segment of the batch has one; the walk without the
+        // test holds fewer registers)
+        const bool nearf = NEAR && d0.near > 0.0;
         const int lleaf = nearf && walker ? (int)leaf_lv[g] : 0;
         const double margin = nearf && walker ? d0.near * ldexp(E, 1 - lleaf) : 0.0;
         uint32_t skip_until = 0;  // a lane skips the items of the bodies below this
