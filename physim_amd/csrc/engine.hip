@@ -632,7 +632,8 @@ struct BatchPlan {
     uint32_t ntiles = 0;  // node tiles
     uint32_t max_split = 1;  // direct sum: the most source splits of a segment
     bool integrate = true;
-    bool near = false;  // some segment walks with a near field (the walk that tests for it)
+    bool near = false;   // some segment walks with a near field (the walk that tests for it)
+    bool count = false;  // some segment counts interactions (the walk that counts)
 };
 
 static inline unsigned grid_for(uint64_t n, unsigned cap) {
@@ -775,16 +776,21 @@ static void enqueue_tree_step(pg_engine* e, const BatchPlan& bp, int q, bool cap
             e->out_part, e->in_part, e->tot_part, e->sc);
     });
     launch(e, cap, "walk", [&] {
-        if (bp.near)
-            k_walk<DIM, FP64, INTEG, true><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
-                desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
-                e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2,
-                e->leafpos, e->node_span, svals);
-        else
-            k_walk<DIM, FP64, INTEG, false><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(
-                desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,
-                e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2,
-                e->leafpos, e->node_span, svals);
+#define PG_WALK(NEAR, COUNT)                                                                   \
+    k_walk<DIM, FP64, INTEG, NEAR, COUNT><<<grid_for(total, 1u << 30), kBlock, 0, st>>>(       \
+        desc, total, skeys, bp.segbits, e->first_node, e->hot, e->cap_nodes, bp.nseg,         \
+        e->acc_out, e->ext_partial, e->sc, bp.sort_tag, e->leaf_lv, e->walk_run_log2,          \
+        e->leafpos, e->node_span, svals)
+        // (walks without the near-field test or the interaction counters
+        // where no segment asks for them hold fewer registers)
+        if (bp.near) {
+            if (bp.count) PG_WALK(true, true);
+            else PG_WALK(true, false);
+        } else {
+            if (bp.count) PG_WALK(false, true);
+            else PG_WALK(false, false);
+        }
+#undef PG_WALK
     });
 }
 
@@ -896,6 +902,7 @@ static int run_batch(pg_engine* e, pg_sim* const* sims_in, uint32_t count, uint3
     for (pg_sim* s : sims) {
         bp.total += s->n;
         bp.near = bp.near || s->near > 0.0;
+        bp.count = bp.count || s->counting;
     }
     if (bp.total >= (1ull << 32))
         return pg_set_error(PG_ERR_ARG, "a batch holds at most 2^32-1 bodies");

@@ -2840,7 +2840,7 @@ __device__ __forceinline__ void pair_fp64(double ax, double ay, double az, doubl
 // Levels below a body's leaf at which its near field still opens cells.
 constexpr int kNearLevels = 2;
 
-template <int DIM, bool FP64, bool INTEGRATE, bool NEAR>
+template <int DIM, bool FP64, bool INTEGRATE, bool NEAR, bool COUNT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_walk(const SegDesc* __restrict__ desc, uint64_t total,
                                                  const uint64_t* __restrict__ skeys, int segbits,
                                                  const uint32_t* __restrict__ first_node,
@@ -2987,7 +2987,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 b = uniform_u32(b + 1u);
             }
             if (add) {
-                ++count;
+                if (COUNT) ++count;
                 // (the reference skips a source at the body's own position)
                 const bool other = !(bx == x && by == y && bz == z);
                 if (FP64) {
@@ -3001,7 +3001,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     }
     // interaction totals (debug counting): one atomic per wave and segment,
     // every lane converged here
-    bool cpend = valid && desc[s].count;
+    bool cpend = COUNT && valid && desc[s].count;
     while (__ballot(cpend)) {
         const int lead = __ffsll((long long)__ballot(cpend)) - 1;
         const uint32_t s0 = uniform_u32(__shfl(s, lead));
@@ -3030,7 +3030,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 az = (double)ka.sz - (double)ka.cz;
             }
         }
-        if (d.count) d.counts[tag_id(pr.w)] = count;
+        if (COUNT && d.count) d.counts[tag_id(pr.w)] = count;
         if (INTEGRATE) {
             // verlet.rs:24-82. After the Taylor first step the velocity is
             // (x' - x) / dt of the stored positions and is formed on download.
