@@ -215,6 +215,27 @@ def test_mixed_precision_energy_drift_close_to_fp64(engine, kind, theta):
 
 
 # ---------------------------------------------------------------- autotuner
+def test_tuning_limits_are_checked(engine):
+    """Tunings outside the kernels' limits are refused and leave the engine's
+    tuning as it was; lds_buckets 0 means every tile's LDS limit, 1024."""
+    default = engine.tuning()
+    try:
+        for bad in (dict(max_buckets=8193), dict(bucket_keys=255), dict(bucket_keys=4097),
+                    dict(walk_run_log2=9), dict(lds_buckets=1025)):
+            with pytest.raises(P.PgError):
+                engine.set_tuning(**dict(default, **bad))
+            assert engine.tuning() == default, bad
+        with pytest.raises(P.PgError):
+            engine.set_sort(8193, 4096)
+        with pytest.raises(P.PgError):
+            engine.set_sort(8192, 4097)
+        engine.set_tuning(**dict(default, lds_buckets=0))
+        assert engine.tuning()["lds_buckets"] == 1024
+    finally:
+        engine.set_tuning(**default)
+        engine.set_sort(8192, 4096)
+
+
 def test_autotune_keeps_results_and_state(engine):
     """Every tuning the autotuner tries gives bit-identical steps; the
     autotuner leaves the simulations' state as it found them and keeps its
