@@ -1990,15 +1990,31 @@ __global__ __launch_bounds__(kScanThreads) void k_tree_scan(
     const uint32_t* __restrict__ tile_tot, uint32_t* __restrict__ tile_base, uint32_t tiles,
     uint32_t* __restrict__ lvl_first, uint32_t* __restrict__ first_node, uint64_t total) {
     __shared__ uint32_t s_wave[kScanThreads / 64];
+    // a thread's first kKeep values stay in registers between the scan's
+    // two passes (up to kKeep * kScanThreads tiles: no second read)
+    constexpr uint32_t kKeep = 8;
     const uint32_t per = (tiles + kScanThreads - 1) / kScanThreads;
     const uint32_t b = threadIdx.x * per;
+    uint32_t v[kKeep];
     if (blockIdx.x == 0) {
         uint32_t s = 0;
-        for (uint32_t k = 0; k < per; ++k)
+#pragma unroll
+        for (uint32_t k = 0; k < kKeep; ++k) {
+            v[k] = k < per && b + k < tiles ? tile_tot[b + k] : 0u;
+            s += v[k];
+        }
+        for (uint32_t k = kKeep; k < per; ++k)
             if (b + k < tiles) s += tile_tot[b + k];
         uint32_t all;
         uint32_t run = block_exclusive_scan<kScanThreads>(s, s_wave, &all);
-        for (uint32_t k = 0; k < per; ++k) {
+#pragma unroll
+        for (uint32_t k = 0; k < kKeep; ++k) {
+            if (k < per && b + k < tiles) {
+                tile_base[b + k] = run;
+                run += v[k];
+            }
+        }
+        for (uint32_t k = kKeep; k < per; ++k) {
             if (b + k < tiles) {
                 tile_base[b + k] = run;
                 run += tile_tot[b + k];
@@ -2014,7 +2030,13 @@ __global__ __launch_bounds__(kScanThreads) void k_tree_scan(
     // suffix minimum: thread r walks the tiles from the end
     const uint32_t rb = threadIdx.x * per;
     uint32_t mn = ~0u;
-    for (uint32_t k = 0; k < per; ++k)
+#pragma unroll
+    for (uint32_t k = 0; k < kKeep; ++k) {
+        v[k] = k < per && rb + k < tiles ? lvl_first[(uint64_t)(tiles - 1 - (rb + k)) * kTopSlots + L]
+                                         : ~0u;
+        mn = min(mn, v[k]);
+    }
+    for (uint32_t k = kKeep; k < per; ++k)
         if (rb + k < tiles) mn = min(mn, lvl_first[(uint64_t)(tiles - 1 - (rb + k)) * kTopSlots + L]);
     uint32_t rrun = block_inclusive_scan<kScanThreads>(mn, s_wave, ~0u, OpMin{});
     // exclusive: the thread before's value (lane 0: the earlier waves')
@@ -2023,7 +2045,14 @@ __global__ __launch_bounds__(kScanThreads) void k_tree_scan(
         const int wave = threadIdx.x >> 6;
         for (int w = 0; w < wave; ++w) rrun = min(rrun, s_wave[w]);
     }
-    for (uint32_t k = 0; k < per; ++k) {
+#pragma unroll
+    for (uint32_t k = 0; k < kKeep; ++k) {
+        if (k < per && rb + k < tiles) {
+            rrun = min(rrun, v[k]);
+            lvl_first[(uint64_t)(tiles - 1 - (rb + k)) * kTopSlots + L] = rrun;
+        }
+    }
+    for (uint32_t k = kKeep; k < per; ++k) {
         if (rb + k < tiles) {
             uint32_t* p = lvl_first + (uint64_t)(tiles - 1 - (rb + k)) * kTopSlots + L;
             rrun = min(rrun, *p);
@@ -2726,12 +2755,19 @@ __global__ __launch_bounds__(kBlock) void k_com_span(
     const int levels = (int)min(sc->top_levels, (uint32_t)kTopSlots);
     for (int L = (int)blockIdx.y * (kBlock / 64) + wave; L < levels;
          L += (int)gridDim.y * (kBlock / 64)) {
+        // the slot, the node's last body and the owner tile's suffix sum
+        // load together (a stale slot's are read and not used)
         const uint64_t sv = top_slot[(uint64_t)t * kTopSlots + L];
+        const uint2 lt = top_last[(uint64_t)t * kTopSlots + L];
+        const Sum4 own = out_part[(uint64_t)t * kTopSlots + L];
         if ((uint32_t)(sv >> 32) != gen) continue;
         const uint32_t nd = (uint32_t)sv;
-        const uint2 lt = top_last[(uint64_t)t * kTopSlots + L];
         const uint64_t last = lt.x;
         const uint32_t t1 = lt.y;
+        // the end tile's values, all loaded before the one needed is chosen
+        const uint32_t end_start = tile_start[t1 + 1];
+        const Sum4 end_tot = tot_part[t1];
+        const Sum4 end_in = in_part[(uint64_t)t1 * kTopSlots + L];
         Sum4 a{0.0, 0.0, 0.0, 0.0};
         for (uint32_t tt = t + 1u + (uint32_t)lane; tt < t1; tt += 64u) a.add(tot_part[tt]);
         // butterfly over the lanes (lane 0's association is fixed)
@@ -2742,10 +2778,9 @@ __global__ __launch_bounds__(kBlock) void k_com_span(
         fold_lanes<2>(a);
         fold_lanes<1>(a);
         if (lane == 0) {
-            Sum4 r = out_part[(uint64_t)t * kTopSlots + L];
+            Sum4 r = own;
             r.add(a);
-            r.add(last + 1 == tile_start[t1 + 1] ? tot_part[t1]
-                                                 : in_part[(uint64_t)t1 * kTopSlots + L]);
+            r.add(last + 1 == end_start ? end_tot : end_in);
             const double inv = 1.0 / r.m;
             hot[nd].mx = r.x * inv;
             hot[nd].my = r.y * inv;
