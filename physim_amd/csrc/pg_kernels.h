@@ -1568,7 +1568,8 @@ __device__ __forceinline__ void minmax_lanes(uint64_t& lo, uint64_t& hi) {
 __device__ __forceinline__ bool bucket_sort_slots(uint64_t* __restrict__ keys,
                                                   uint32_t* __restrict__ vals, uint64_t base,
                                                   uint32_t cnt, uint8_t* s_raw,
-                                                  const uint64_t* __restrict__ quant) {
+                                                  const uint64_t* __restrict__ quant,
+                                                  uint32_t& heads) {
     constexpr int kWaves = kBucketThreads / 64;
     uint64_t* s_k = reinterpret_cast<uint64_t*>(s_raw);
     uint32_t* s_v = reinterpret_cast<uint32_t*>(s_raw + kSlots * 8);
@@ -1729,14 +1730,20 @@ __device__ __forceinline__ bool bucket_sort_slots(uint64_t* __restrict__ keys,
         }
     }
     __syncthreads();
+    // the sorted bucket goes out; bit e of `heads` marks the thread's key e
+    // as the head of a run of equal keys (equal keys share a bucket)
+    uint32_t h = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const uint32_t i = (uint32_t)e * kBucketThreads + t;
         if (i < cnt) {
-            keys[base + i] = s_k[i];
+            const uint64_t k = s_k[i];
+            keys[base + i] = k;
             vals[base + i] = s_v[i];
+            if (i + 1u < cnt && s_k[i + 1u] == k && (i == 0 || s_k[i - 1u] != k)) h |= 1u << e;
         }
     }
+    heads = h;
     return true;
 }
 
@@ -1783,17 +1790,30 @@ __global__ __launch_bounds__(kBucketThreads) __attribute__((amdgpu_waves_per_eu(
     if (cnt <= min(cap, (uint32_t)kBucketCap)) {
         uint64_t* s_bk = reinterpret_cast<uint64_t*>(s_raw);
         uint32_t* s_bv = reinterpret_cast<uint32_t*>(s_raw + kBucketCap * 8);
-        if (cnt > kSlots ||
-            !bucket_sort_slots(keys, vals, base, cnt, s_raw,
-                               sc->quantiles ? quant + (uint64_t)blockIdx.x * kQuantiles : nullptr)) {
+        uint32_t heads = 0;
+        const bool slots =
+            cnt <= kSlots &&
+            bucket_sort_slots(keys, vals, base, cnt, s_raw,
+                              sc->quantiles ? quant + (uint64_t)blockIdx.x * kQuantiles : nullptr,
+                              heads);
+        if (!slots) {
             __syncthreads();  // (the slot sort's tables share the space)
             if (cnt <= 2u * kBucketThreads) bucket_sort_regs<2>(keys, vals, base, cnt, s_bk, s_bv);
             else if (cnt <= 4u * kBucketThreads) bucket_sort_regs<4>(keys, vals, base, cnt, s_bk, s_bv);
             else bucket_sort_regs<8>(keys, vals, base, cnt, s_bk, s_bv);
         }
         __syncthreads();
-        for (uint32_t i = tid; i < cnt; i += kBucketThreads)
-            order_tie_run<DIM>(desc, total, keys, vals, deep, nseg, segbits, lmax, base + i);
+        if (slots) {
+            // only the heads the slot sort marked start runs of equal keys
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if ((heads >> e) & 1u)
+                    order_tie_run<DIM>(desc, total, keys, vals, deep, nseg, segbits, lmax,
+                                       base + (uint32_t)e * kBucketThreads + (uint32_t)tid);
+        } else {
+            for (uint32_t i = tid; i < cnt; i += kBucketThreads)
+                order_tie_run<DIM>(desc, total, keys, vals, deep, nseg, segbits, lmax, base + i);
+        }
         return;
     }
     // HBM: stable LSD passes over the range, chunk by chunk in order
