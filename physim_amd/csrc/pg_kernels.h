@@ -34,6 +34,45 @@ __device__ __forceinline__ const T& at32(const T* __restrict__ base, uint32_t id
                                        (uint32_t)(idx * (uint32_t)sizeof(T)));
 }
 
+// Global-memory access through a pointer held in a descriptor. The compiler
+// cannot tell where such a pointer points and emits flat instructions, which
+// also count against the scalar-load counter: every wait for a scalar load
+// (a descriptor field, a node record) then waits for them too. Through the
+// global address space they are plain global loads and stores.
+// (As 32-bit words: HIP's vector types do not copy from address-space
+// qualified references.)
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) uint32_t*)p);
+    } else {
+        typedef uint32_t words __attribute__((ext_vector_type(sizeof(T) / 4)));
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) words*)p);
+    }
+}
+template <class T>
+__device__ __forceinline__ void gstore(T* p, const T& v) {
+    static_assert(sizeof(T) % 4 == 0 && sizeof(T) >= 8, "whole words");
+    typedef uint32_t words __attribute__((ext_vector_type(sizeof(T) / 4)));
+    *(__attribute__((address_space(1))) words*)p = __builtin_bit_cast(words, v);
+}
+
+// floor(a * b / c) for a * b < 2^53 and c > 0 by one FP64 division (a 64-bit
+// integer division is a software loop of about 150 instructions). Both
+// operands are exact in FP64; a quotient that is not an integer lies at
+// least 1/c from the integers around it, and the division's rounding moves
+// it by at most q 2^-53 = a b / c 2^-53 < 1/c, so its floor is exact. The
+// sample positions below stay in range: indices below 2^32 (the batch's
+// body count) times at most kMaxWideBuckets * kQuantiles = 2^21.
+__device__ __forceinline__ uint64_t muldiv_floor(uint64_t a, uint64_t b, uint64_t c) {
+    return (uint64_t)floor((double)(a * b) / (double)c);
+}
+static_assert((uint64_t)kMaxWideBuckets * kQuantiles <= (1ull << 21) &&
+                  (uint64_t)kMaxWideBuckets * kWideSampleOver <= (1ull << 21) &&
+                  (uint64_t)kMaxBuckets * kSampleOver <= (1ull << 21),
+              "sample positions: index times count below 2^53");
+
 // Segment of global index g: largest s with seg_off[s] <= g (empty segments
 // share an offset with their successor and are skipped).
 __device__ __forceinline__ uint32_t seg_of(const uint64_t* __restrict__ seg_off, uint32_t nseg,
@@ -371,8 +410,40 @@ __device__ __forceinline__ void buckets_of_wide(const uint64_t* s_c, const uint6
 // at each level the octant is (x > cx) | (y > cy) << 1 [| (z > cz) << 2] and
 // the child centre is c -/+ extent/2, rounded as the reference rounds it;
 // a body clear of every cell boundary takes the same digits from its scaled
-// coordinates (axis_digits).
-//
+// coordinates (axis_digits). `ext_bits`: the root's half-width as the bits of
+// a double.
+template <int DIM>
+__device__ __forceinline__ uint64_t morton_key(double px, double py, double pz,
+                                               uint64_t ext_bits, int lmax) {
+    double ext = bits_f64(ext_bits);
+    const double inv = 1.0 / ext;
+    const double scale = ldexp(1.0, lmax - 1), margin = ldexp(1.0, lmax - 41);
+    uint64_t qx, qy, qz = 0;
+    // (an extent this far from 1 makes 1/E or E's halvings subnormal)
+    bool fast = ext >= 0x1p-900 && ext <= 0x1p+1000;
+    fast &= axis_digits(px, inv, scale, margin, qx);
+    fast &= axis_digits(py, inv, scale, margin, qy);
+    if (DIM == 3) fast &= axis_digits(pz, inv, scale, margin, qz);
+    uint64_t key = spread_bits<DIM>(qx) | (spread_bits<DIM>(qy) << 1);
+    if (DIM == 3) key |= spread_bits<DIM>(qz) << 2;
+    if (!fast) {
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        key = 0;
+        for (int l = 0; l < lmax; ++l) {
+            uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
+            if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
+            const double h2 = half(ext);
+            cx = (o & 1u) ? cx + h2 : cx - h2;
+            cy = (o & 2u) ? cy + h2 : cy - h2;
+            if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
+            ext = h2;
+            key = (key << DIM) | o;
+        }
+    }
+    return key;
+}
+
+// Each body's Morton key (morton_key) and its index into keys / vals.
 // For the radix sort (sort_tag 0) each tile also counts its digits of the
 // first pass. With FUSE (the sample sort, the batch's previous sorted keys
 // in `prev`) it does the work of k_splitters and k_bucket_count instead: the
@@ -405,28 +476,66 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
     // bucket counts: of all buckets (FUSE 1), or of a window from s_wb (FUSE 2)
     __shared__ uint32_t s_bh[kMaxBuckets];
     __shared__ uint32_t s_wb;
+    constexpr int kItems = kSortTile / kKeyThreads;
+    constexpr int kSplitItems = kMaxBuckets / kKeyThreads;
+    static_assert(kMaxBuckets % kKeyThreads == 0, "splitter loads cover the table");
+    const uint64_t g0 = (uint64_t)blockIdx.x * kSortTile;
+    const uint64_t glast = min(g0 + (uint64_t)kSortTile, total) - 1;
+    const uint32_t sa = seg_of(seg_off, nseg, g0);
+    const uint32_t sb = seg_of(seg_off, nseg, glast);
+    const bool one = sa == sb;  // block-uniform
+    // A tile in one segment (the usual case) requests its bodies' positions,
+    // the splitters and its first extent partials before anything waits on
+    // them, so the block pays one memory round trip for all of them instead
+    // of one per key (indices clamped into range rather than guarded: a load
+    // inside a divergent branch is waited for at the branch's end).
+    double4 q[kItems];
+    uint64_t mblock = 0;
+    if (one) {
+        const SegDesc& d = desc[sa];
+        const double4* sp = d.spos;
+        const uint64_t off = d.off;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            q[k] = gload(sp + (min(g0 + (uint64_t)k * kKeyThreads + threadIdx.x, glast) - off));
+        if (d.nchunks) mblock = partial[d.chunk0 + min((uint32_t)threadIdx.x, d.nchunks - 1)];
+    }
+    uint64_t sv[kSplitItems];
+    if (FUSE == 1) {
+#pragma unroll
+        for (int i = 0; i < kSplitItems; ++i) {
+            const uint32_t b = threadIdx.x + (uint32_t)i * kKeyThreads;
+            sv[i] = prev[muldiv_floor(min(b + 1u, nb - 1u), total, nb)];
+        }
+    } else if (FUSE == 2) {
+        const uint32_t nc = (nb - 1) / kWideGroup;  // >= 128: nb > kMaxBuckets
+#pragma unroll
+        for (int i = 0; i < kSplitItems; ++i)
+            sv[i] = spc[min(threadIdx.x + (uint32_t)i * kKeyThreads, nc - 1u)];
+    }
     if (threadIdx.x < 256) s_h[threadIdx.x] = 0;
     if (FUSE == 1) {
-        for (uint32_t b = threadIdx.x; b < nb; b += kKeyThreads) {
-            s_bh[b] = 0;
-            if (b + 1 < nb) s_spl[b] = prev[(uint64_t)(b + 1) * total / nb];
+#pragma unroll
+        for (int i = 0; i < kSplitItems; ++i) {
+            const uint32_t b = threadIdx.x + (uint32_t)i * kKeyThreads;
+            if (b < nb) s_bh[b] = 0;
+            if (b + 1 < nb) s_spl[b] = sv[i];
         }
     } else if (FUSE == 2) {
         const uint32_t nc = (nb - 1) / kWideGroup;
-        for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxBuckets; i += kKeyThreads) {
-            s_bh[i] = 0;
-            if (i < nc) s_spl[i] = spc[i];
+#pragma unroll
+        for (int i = 0; i < kSplitItems; ++i) {
+            const uint32_t b = threadIdx.x + (uint32_t)i * kKeyThreads;
+            s_bh[b] = 0;
+            if (b < nc) s_spl[b] = sv[i];
         }
     }
     if (FUSE) {
         const uint64_t m = (uint64_t)nb * kQuantiles;
         for (uint64_t t = (uint64_t)blockIdx.x * kKeyThreads + threadIdx.x; t < m;
              t += (uint64_t)gridDim.x * kKeyThreads)
-            quant[t] = prev[t * total / m];
+            quant[t] = prev[muldiv_floor(t, total, m)];
     }
-    const uint64_t g0 = (uint64_t)blockIdx.x * kSortTile;
-    const uint32_t sa = seg_of(seg_off, nseg, g0);
-    const uint32_t sb = seg_of(seg_off, nseg, min(g0 + (uint64_t)kSortTile, total) - 1);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->gen += 1;
         sc->top_levels = 0;
@@ -437,11 +546,10 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
         if (FUSE) sc->quantiles = 1;
         sc->ext_tag = 0;  // the partials are read here; k_scan_gather clears them
     }
-    // the tile's extent when it lies in one segment (block-uniform branch)
-    uint64_t mblock = 0;
-    if (sa == sb) {
+    // the tile's extent when it lies in one segment
+    if (one) {
         const SegDesc& d = desc[sa];
-        for (uint32_t c = threadIdx.x; c < d.nchunks; c += kKeyThreads) {
+        for (uint32_t c = threadIdx.x + kKeyThreads; c < d.nchunks; c += kKeyThreads) {
             const uint64_t b = partial[d.chunk0 + c];
             mblock = b > mblock ? b : mblock;
         }
@@ -449,64 +557,55 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys(
     } else {
         __syncthreads();
     }
-    uint64_t tk[kSortTile / kKeyThreads];  // this thread's keys, for the fused bucket count
-    for (int k = 0; k < kSortTile / kKeyThreads; ++k) {
-        const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
-        uint32_t dig = 256u;
-        tk[k] = 0;
-        if (g < total) {
-            const uint32_t s = seg_of(seg_off, nseg, g);
-            const SegDesc& d = desc[s];
-            uint64_t m = mblock;
-            if (sa != sb) {
-                m = 0;
+    uint64_t tk[kItems];  // this thread's keys, for the fused bucket count
+    if (one) {
+        const SegDesc& d = desc[sa];
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
+            uint32_t dig = 256u;
+            tk[k] = 0;
+            if (g <= glast) {
+                if (g == d.off) open_segment(d.st, mblock, integrate);
+                uint64_t key = morton_key<DIM>(q[k].x, q[k].y, q[k].z, mblock, lmax);
+                if (segbits) key |= (uint64_t)sa << (64 - segbits);
+                keys[g] = key;
+                vals[g] = (uint32_t)g;
+                dig = (uint32_t)(key & 255u);
+                tk[k] = key;
+            }
+            if (sort_tag == 0) wave_count_digit(s_h, dig);
+        }
+    } else {
+        // a tile across segments: each body finds its segment and extent
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t g = g0 + (uint64_t)k * kKeyThreads + threadIdx.x;
+            uint32_t dig = 256u;
+            tk[k] = 0;
+            if (g < total) {
+                const uint32_t s = seg_of(seg_off, nseg, g);
+                const SegDesc& d = desc[s];
+                uint64_t m = 0;
                 for (uint32_t c = 0; c < d.nchunks; ++c) {
                     const uint64_t b = partial[d.chunk0 + c];
                     m = b > m ? b : m;
                 }
+                if (g == d.off) open_segment(d.st, m, integrate);
+                const double4 p = d.spos[g - d.off];
+                uint64_t key = morton_key<DIM>(p.x, p.y, p.z, m, lmax);
+                if (segbits) key |= (uint64_t)s << (64 - segbits);
+                keys[g] = key;
+                vals[g] = (uint32_t)g;
+                dig = (uint32_t)(key & 255u);
+                tk[k] = key;
             }
-            if (g == d.off) open_segment(d.st, m, integrate);
-            const uint64_t j = g - d.off;
-            const double4 q = d.spos[j];
-            const double px = q.x, py = q.y, pz = q.z;
-            double ext = bits_f64(m);
-            const double inv = 1.0 / ext;
-            const double scale = ldexp(1.0, lmax - 1), margin = ldexp(1.0, lmax - 41);
-            uint64_t qx, qy, qz = 0;
-            // (an extent this far from 1 makes 1/E or E's halvings subnormal)
-            bool fast = ext >= 0x1p-900 && ext <= 0x1p+1000;
-            fast &= axis_digits(px, inv, scale, margin, qx);
-            fast &= axis_digits(py, inv, scale, margin, qy);
-            if (DIM == 3) fast &= axis_digits(pz, inv, scale, margin, qz);
-            uint64_t key = spread_bits<DIM>(qx) | (spread_bits<DIM>(qy) << 1);
-            if (DIM == 3) key |= spread_bits<DIM>(qz) << 2;
-            if (!fast) {
-                double cx = 0.0, cy = 0.0, cz = 0.0;
-                key = 0;
-                for (int l = 0; l < lmax; ++l) {
-                    uint32_t o = (uint32_t)(px > cx) | ((uint32_t)(py > cy) << 1);
-                    if (DIM == 3) o |= (uint32_t)(pz > cz) << 2;
-                    const double h2 = half(ext);
-                    cx = (o & 1u) ? cx + h2 : cx - h2;
-                    cy = (o & 2u) ? cy + h2 : cy - h2;
-                    if (DIM == 3) cz = (o & 4u) ? cz + h2 : cz - h2;
-                    ext = h2;
-                    key = (key << DIM) | o;
-                }
-            }
-            if (segbits) key |= (uint64_t)s << (64 - segbits);
-            keys[g] = key;
-            vals[g] = (uint32_t)g;
-            dig = (uint32_t)(key & 255u);
-            tk[k] = key;
+            if (sort_tag == 0) wave_count_digit(s_h, dig);
         }
-        if (sort_tag == 0) wave_count_digit(s_h, dig);
     }
     __syncthreads();
     if (sort_tag == 0) {
         if (threadIdx.x < 256) cnt[(uint64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
     } else if (FUSE) {
-        constexpr int kItems = kSortTile / kKeyThreads;
         uint32_t bk[kItems];
         // the first bucket counted in LDS: 0 (every bucket), or, with more
         // buckets, a quarter of the window below the tile's first key's (the
@@ -838,7 +937,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters(
         // quantiles of the previous step's sorted keys, if they are in order
         if (threadIdx.x == 0) s_unsorted = 0;
         for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads)
-            s[b] = prev[(uint64_t)(b + 1) * n / nb];
+            s[b] = prev[muldiv_floor(b + 1, n, nb)];
         __syncthreads();
         bool bad = false;
         for (uint32_t b = threadIdx.x; b + 2 < nb; b += kSplitThreads) bad |= s[b] > s[b + 1];
@@ -857,7 +956,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters(
     uint32_t p2 = 1;
     while (p2 < m) p2 <<= 1;
     for (uint32_t j = threadIdx.x; j < p2; j += kSplitThreads)
-        s[j] = j < m ? keys[(uint64_t)j * n / m] : ~0ull;
+        s[j] = j < m ? keys[muldiv_floor(j, n, m)] : ~0ull;
     __syncthreads();
     lds_bitonic_sort<kSplitThreads>(s, p2);
     for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads) spl[b] = s[(b + 1) * kSampleOver];
@@ -889,7 +988,7 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_count(
         const uint64_t m = (uint64_t)nb * kQuantiles;
         for (uint64_t t = (uint64_t)blockIdx.x * kSortThreads + threadIdx.x; t < m;
              t += (uint64_t)gridDim.x * kSortThreads)
-            quant[t] = prev[t * n / m];
+            quant[t] = prev[muldiv_floor(t, n, m)];
     }
     __syncthreads();
 #pragma unroll
@@ -1108,7 +1207,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters_wide(
     __syncthreads();
     if (prev_ok) {
         for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads)
-            s[b] = prev[(uint64_t)(b + 1) * n / nb];
+            s[b] = prev[muldiv_floor(b + 1, n, nb)];
         __syncthreads();
         bool bad = false;
         if (!fused)
@@ -1128,7 +1227,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_splitters_wide(
     uint32_t p2 = 1;
     while (p2 < m) p2 <<= 1;
     for (uint32_t j = threadIdx.x; j < p2; j += kSplitThreads)
-        s[j] = j < m ? keys[(uint64_t)j * n / m] : ~0ull;
+        s[j] = j < m ? keys[muldiv_floor(j, n, m)] : ~0ull;
     __syncthreads();
     lds_bitonic_sort<kSplitThreads>(s, p2);
     for (uint32_t b = threadIdx.x; b + 1 < nb; b += kSplitThreads)
@@ -1169,7 +1268,7 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_count_wide(
         const uint64_t m = (uint64_t)nb * kQuantiles;
         for (uint64_t t = (uint64_t)blockIdx.x * kSortThreads + tid; t < m;
              t += (uint64_t)gridDim.x * kSortThreads)
-            quant[t] = prev[t * n / m];
+            quant[t] = prev[muldiv_floor(t, n, m)];
     }
     __syncthreads();
     uint32_t b[kSortItems];
@@ -1913,27 +2012,59 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
         for (uint32_t c = tid; c < nchunks; c += kBlock) ext_partial[c] = 0;
     __syncthreads();
 
-#pragma unroll 2
+    // A node tile lies in one segment (tiles start at every segment's first
+    // body): the segment's fields are read once, for the segment named by
+    // the tile's first key, and each body's own key is checked against it.
+    // Every thread requests its bodies' keys, neighbour keys and source
+    // indices together, then their state records together (indices clamped
+    // into range rather than guarded), so a body costs two memory round
+    // trips rather than one per dependent field.
+    const uint64_t total = tile_start[gridDim.x];  // (one block per tile)
+    uint64_t kc[kScanItems], kl[kScanItems], kr[kScanItems];
+    uint32_t sv[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = min(tbase + (uint32_t)k * kBlock + tid, tend - 1);
+        kc[k] = skeys[i];
+        kl[k] = skeys[i > 0 ? i - 1 : 0];
+        kr[k] = skeys[i + 1 < total ? i + 1 : i];
+        sv[k] = svals[i];
+    }
+    const uint32_t s_t = uniform_u32(seg_of_key(skeys[tbase], segbits));
+    const SegDesc& d = desc[min(s_t, nseg - 1)];
+    const uint64_t off = d.off, n = d.n, end = off + n;
+    const bool seg_ok = s_t < nseg && n > 0;  // (else every body is flagged)
+    const bool near = seg_ok && d.near > 0.0;
+    const bool vperm = seg_ok && gload(&d.st->vperm) != 0;
+    bool bad[kScanItems];
+    double4 lp[kScanItems];
+    if (seg_ok) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint64_t i = min(tbase + (uint32_t)k * kBlock + tid, tend - 1);
+            const uint64_t src = (uint64_t)sv[k] - off;
+            bad[k] = seg_of_key(kc[k], segbits) != s_t || src >= n || i < off || i >= end;
+            lp[k] = gload(d.spos + (bad[k] ? 0 : src));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) bad[k] = true;
+    }
+#pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t li = (uint32_t)k * kBlock + tid;
         const uint64_t i = tbase + li;
         uint32_t c = 0;
         if (i < tend) {
-            const uint64_t key = skeys[i];
-            const uint32_t s = seg_of_key(key, segbits);
-            const uint64_t src = s < nseg ? (uint64_t)svals[i] - desc[s].off : ~0ull;
             int lo = 0, leaf = 0;
-            if (s >= nseg || src >= desc[s].n || i < desc[s].off || i >= desc[s].off + desc[s].n) {
+            if (bad[k]) {
                 atomicOr(err, kErrCapacity);
-                c = 0;
             } else {
-                const SegDesc& d = desc[s];
-                const uint64_t end = d.off + d.n;
                 // two bodies share at most kMaxLcp levels (the reference
                 // panics past depth 64, octree.rs:60-62)
                 const int depth = kMaxLcp + 1;
-                int lcpL = i > d.off ? key_lcp2<DIM>(skeys[i - 1], key, deep, i - 1, i, lmax) : -1;
-                int lcpR = i + 1 < end ? key_lcp2<DIM>(key, skeys[i + 1], deep, i, i + 1, lmax) : -1;
+                int lcpL = i > off ? key_lcp2<DIM>(kl[k], kc[k], deep, i - 1, i, lmax) : -1;
+                int lcpR = i + 1 < end ? key_lcp2<DIM>(kc[k], kr[k], deep, i, i + 1, lmax) : -1;
                 if (lcpR >= depth) {
                     atomicOr(&d.st->err, kErrDepth);
                     lcpR = depth - 1;
@@ -1944,19 +2075,21 @@ __global__ __launch_bounds__(kBlock) void k_scan_gather(
                 c = (uint32_t)max(0, lcpR - lcpL);
                 lo = lcpL + 1;
                 leaf = max(lcpL, lcpR) + 1;  // the level of the body's own leaf
-                const uint64_t j = i - d.off;
                 // position and mass as one record in sorted order: the node
                 // builder's input and the walk's leaves (the walk writes the
                 // state's own arrays, and moves each body's record of previous
                 // position and tag itself: it is that record's only reader)
-                leafpos[i] = d.spos[src];
-                if (d.st->vperm) {
-                    d.dvx[j] = d.svx[src]; d.dvy[j] = d.svy[src]; d.dvz[j] = d.svz[src];
+                leafpos[i] = lp[k];
+                if (vperm) {
+                    const uint64_t j = i - off, src = (uint64_t)sv[k] - off;
+                    gstore(d.dvx + j, gload(d.svx + src));
+                    gstore(d.dvy + j, gload(d.svy + src));
+                    gstore(d.dvz + j, gload(d.svz + src));
                 }
             }
             lo_level[i] = (uint8_t)lo;
             // by sorted position, read by a walk with a near field
-            if (s < nseg && desc[s].near > 0.0) leaf_lv[i] = (uint8_t)leaf;
+            if (near) leaf_lv[i] = (uint8_t)leaf;
             // (a body below every internal level, lo = kMaxLcp + 1, starts none)
             if (lo < kTopSlots) atomicMin(&s_firstlo[lo], (uint32_t)i);
         }
