@@ -2556,7 +2556,7 @@ __global__ __launch_bounds__(kNodeThreads) __attribute__((amdgpu_waves_per_eu(6)
     if (threadIdx.x == kPathThread) {
         path_ka = skeys_g[b0];
         path_kz = skeys_g[b1 - 1];
-        path_e = d.st->extent_bits;
+        path_e = gload(&d.st->extent_bits);
     }
     const uint32_t last_li = nb - 1u;  // (a tile holds a body)
     double4 after = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -3080,7 +3080,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         const double4 p = leafpos[g];
         x = p.x; y = p.y; z = p.z;
         m = p.w;
-        pr = d.sprv[src];  // previous position and tag, previous order
+        pr = gload(d.sprv + src);  // previous position and tag, previous order
         fixed = tag_fixed(pr.w);
     }
     Kahan3 ka;
@@ -3218,7 +3218,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 az = (double)ka.sz - (double)ka.cz;
             }
         }
-        if (COUNT && d.count) d.counts[tag_id(pr.w)] = count;
+        if (COUNT && d.count) gstore(d.counts + tag_id(pr.w), (int64_t)count);
         if (INTEGRATE) {
             // verlet.rs:24-82. After the Taylor first step the velocity is
             // (x' - x) / dt of the stored positions and is formed on download.
@@ -3226,20 +3226,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             const double dt2 = dt * dt;
             double nx, ny, nz;
             if (d.st->history == 0) {
-                const double ux = d.dvx[j], uy = d.dvy[j], uz = d.dvz[j];
+                const double ux = gload(d.dvx + j), uy = gload(d.dvy + j), uz = gload(d.dvz + j);
                 nx = x + ux * dt + 0.5 * ax * dt2;
                 ny = y + uy * dt + 0.5 * ay * dt2;
                 nz = z + uz * dt + 0.5 * az * dt2;
-                d.dvx[j] = ux + ax * dt;
-                d.dvy[j] = uy + ay * dt;
-                d.dvz[j] = uz + az * dt;
+                gstore(d.dvx + j, ux + ax * dt);
+                gstore(d.dvy + j, uy + ay * dt);
+                gstore(d.dvz + j, uz + az * dt);
             } else {
                 nx = 2.0 * x - pr.x + ax * dt2;
                 ny = 2.0 * y - pr.y + ay * dt2;
                 nz = 2.0 * z - pr.z + az * dt2;
             }
-            d.dpos[j] = make_double4(nx, ny, nz, m);
-            d.dprv[j] = make_double4(x, y, z, pr.w);
+            // (global stores: a flat store would hold up the LDS waits of the
+            // extent merge below until HBM acknowledged it)
+            gstore(d.dpos + j, make_double4(nx, ny, nz, m));
+            gstore(d.dprv + j, make_double4(x, y, z, pr.w));
             emax = f64_bits(fmax(fmax(fabs(nx), fabs(ny)), fabs(nz)));
             nonfinite = !isfinite(nx) || !isfinite(ny) || !isfinite(nz);
             chunk = d.chunk0 + (uint32_t)(j / kExtentChunk);
@@ -3248,8 +3250,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             double* o = acc_out + 3 * (d.off + tag_id(pr.w));
             o[0] = ax; o[1] = ay; o[2] = az;
             // the permuted state keeps its positions, masses and history
-            d.dpos[j] = make_double4(x, y, z, m);
-            d.dprv[j] = pr;
+            gstore(d.dpos + j, make_double4(x, y, z, m));
+            gstore(d.dprv + j, pr);
         }
     }
     if (INTEGRATE) {
