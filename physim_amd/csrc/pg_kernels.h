@@ -1681,23 +1681,30 @@ __device__ __forceinline__ bool bucket_sort_slots(uint64_t* __restrict__ keys,
     uint64_t* s_q = reinterpret_cast<uint64_t*>(s_bin + kCoarse);  // the bucket's quantiles
     uint32_t* s_wide = reinterpret_cast<uint32_t*>(s_q + kCoarse);   // count, then wide slots
     const uint32_t t = threadIdx.x;
-    if (t == 0) s_wide[0] = 0;
     const int lane = (int)(t & 63u), wave = (int)(t >> 6);
+    // every load goes out before anything waits on one (indices clamped into
+    // the bucket, cnt >= 2: a load inside a divergent branch is waited for at
+    // the branch's end)
+    const uint64_t qv = quant ? quant[min(t, kCoarse - 1u)] : 0ull;
+    uint64_t key[4];
+    uint32_t val[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint64_t i = base + min((uint32_t)e * kBucketThreads + t, cnt - 1u);
+        key[e] = keys[i];
+        val[e] = vals[i];
+    }
+    if (t == 0) s_wide[0] = 0;
     s_pos[2 * t] = 0;
     s_pos[2 * t + 1] = 0;
     if (t < kCoarse) {
         s_bin[t] = 0;
-        if (quant) s_q[t] = quant[t];
+        if (quant) s_q[t] = qv;
     }
-    uint64_t key[4];
-    uint32_t val[4];
     uint64_t lo = ~0ull, hi = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const uint32_t i = (uint32_t)e * kBucketThreads + t;
-        key[e] = i < cnt ? keys[base + i] : 0ull;
-        val[e] = i < cnt ? vals[base + i] : 0u;
-        if (i < cnt) {
+        if ((uint32_t)e * kBucketThreads + t < cnt) {
             lo = key[e] < lo ? key[e] : lo;
             hi = key[e] > hi ? key[e] : hi;
         }
