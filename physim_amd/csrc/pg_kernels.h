@@ -2993,17 +2993,27 @@ __device__ __forceinline__ bool accept_cell(double hw, double s, double theta, b
 // Pair acceleration on body a from (b, mb), mixed precision: the separation
 // b - a is taken in FP64 and rounded once; |F|/m_a = mb / (s + e) along the
 // unsoftened direction (astro/src/lib.rs:84-113, with m_a cancelled). Where
-// `use` is false (or the separation rounds to zero) the term is zero, by a
-// select rather than a branch: no exec-mask handling in the walk's loop.
+// `use` is false (or the separation rounds to zero) the term is zero; where
+// `take` is false the lane takes nothing from this source and its sums are
+// left as they were (not even a zero is added: a compensated add of zero
+// moves the sum's bits). Both by selects rather than branches: no exec-mask
+// handling in the walk's loop.
 __device__ __forceinline__ void pair_mixed(double ax, double ay, double az, double bx, double by,
                                               double bz, double mb, float ef, Kahan3& acc,
-                                              bool use) {
+                                              bool take, bool use) {
     const float fx = (float)(bx - ax), fy = (float)(by - ay), fz = (float)(bz - az);
     const float s = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
     const float w = use && s > 0.0f
                         ? (float)mb * __builtin_amdgcn_rsqf(s) * __builtin_amdgcn_rcpf(s + ef)
                         : 0.0f;
-    acc.add(fx * w, fy * w, fz * w);
+    Kahan3 n = acc;
+    n.add(fx * w, fy * w, fz * w);
+    acc.sx = take ? n.sx : acc.sx;
+    acc.sy = take ? n.sy : acc.sy;
+    acc.sz = take ? n.sz : acc.sz;
+    acc.cx = take ? n.cx : acc.cx;
+    acc.cy = take ? n.cy : acc.cy;
+    acc.cz = take ? n.cz : acc.cz;
 }
 
 // Pair force on a from b exactly as astro/src/lib.rs:84-113 computes it (FP64,
@@ -3129,10 +3139,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         // next item is internal node k when k starts at body b (its span's
         // first body), else the leaf of body b. After node k the next item is
         // (b, k + 1) when some lane opened it, else (the body after its
-        // range, its skip). Both possible next records are loaded while this
-        // one is tested (indices clamped into the node array; a record past
-        // kend is never used). Each step raises b or k (guarded against a
-        // corrupt record), so the loop ends.
+        // range, its skip). The next record is loaded once the lanes have
+        // decided (index clamped into the node array; a record past kend is
+        // never used). Each step raises b or k (guarded against a corrupt
+        // record), so the loop ends.
         const uint32_t clamp = (uint32_t)(cap - 1);
         NodeHot h = at32(hot, min(k, clamp));
         uint2 sp = at32(node_span, min(k, clamp));
@@ -3142,10 +3152,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             double bx, by, bz, bm;
             bool add = false;
             if (k < kend && sp.x == b) {
-                const NodeHot h_open = at32(hot, min(k + 1u, clamp));
-                const uint2 sp_open = at32(node_span, min(k + 1u, clamp));
-                const NodeHot h_skip = at32(hot, min(h.skip, clamp));
-                const uint2 sp_skip = at32(node_span, min(h.skip, clamp));
                 bx = h.mx; by = h.my; bz = h.mz; bm = h.mass;
                 // the opening test for every lane, its result kept by the
                 // lanes still walking (no branch on the common path)
@@ -3164,16 +3170,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 const bool open = act && !take;
                 skip_until = take ? sp.y : skip_until;
                 add = take;
-                if (__ballot(open) || h.skip <= k || sp.y <= b) {
-                    k = uniform_u32(k + 1u);
-                    h = h_open;
-                    sp = sp_open;
-                } else {
-                    b = uniform_u32(sp.y);
-                    k = uniform_u32(h.skip);
-                    h = h_skip;
-                    sp = sp_skip;
-                }
+                // the next item's index by selects (one pair of record loads,
+                // no branch: each branch cost scalar instructions)
+                const bool go_open = __ballot(open) || h.skip <= k || sp.y <= b;
+                const uint32_t kn = go_open ? k + 1u : h.skip;
+                b = uniform_u32(go_open ? b : sp.y);
+                k = uniform_u32(kn);
+                h = at32(hot, min(k, clamp));
+                sp = at32(node_span, min(k, clamp));
             } else {
                 // the leaf of body b: its exact position and mass
                 const double4 lp = at32(leafpos, b);
@@ -3181,15 +3185,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 add = walker && b >= skip_until;
                 b = uniform_u32(b + 1u);
             }
-            if (add) {
-                if (COUNT) ++count;
-                // (the reference skips a source at the body's own position)
-                const bool other = !(bx == x && by == y && bz == z);
-                if (FP64) {
+            // (the reference skips a source at the body's own position)
+            const bool other = !(bx == x && by == y && bz == z);
+            if (FP64) {
+                if (add) {
+                    if (COUNT) ++count;
                     if (other) pair_fp64(x, y, z, m, bx, by, bz, bm, e, f64);
-                } else {
-                    pair_mixed(x, y, z, bx, by, bz, bm, ef, ka, other);
                 }
+            } else {
+                // (every lane forms the term; the lanes that add keep it)
+                if (COUNT) count += add ? 1u : 0u;
+                pair_mixed(x, y, z, bx, by, bz, bm, ef, ka, add, other);
             }
         }
         pending = pending && !mine;
