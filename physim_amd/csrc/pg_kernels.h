@@ -21,6 +21,18 @@ __device__ __forceinline__ uint32_t f32_bits(float f) { return __float_as_uint(f
 // into the subnormal range), without relying on the compiler to see that.
 __device__ __forceinline__ double half(double x) { return x * 0.5; }
 
+// A uniform double copied into a vector register. The walk copies an item's
+// source (a node's centre of mass and mass, read through the scalar unit)
+// before it loads the next node record: the record then lands in the
+// registers it is carried in, and the pair term runs while the load is in
+// flight, instead of the compiler copying the new record over and waiting
+// for it first. (A move, not a load: nothing is in flight in it.)
+__device__ __forceinline__ double vcopy(double v) {
+    double r;
+    asm("v_mov_b64 %0, %1" : "=v"(r) : "s"(v));
+    return r;
+}
+
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
@@ -3152,15 +3164,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             double bx, by, bz, bm;
             bool add = false;
             if (k < kend && sp.x == b) {
-                bx = h.mx; by = h.my; bz = h.mz; bm = h.mass;
+                bx = vcopy(h.mx); by = vcopy(h.my); bz = vcopy(h.mz); bm = vcopy(h.mass);
                 // the opening test for every lane, its result kept by the
                 // lanes still walking (no branch on the common path)
                 const bool act = walker && b >= skip_until;
                 const double d0x = x - h.cx, d0y = y - h.cy, d0z = z - h.cz;
-                const double hw = ldexp(E, -h.level());
+                // (the level is taken from the record where it is used: left
+                // to itself the compiler extracts it right after the next
+                // record's load, which makes that load wait at once)
+                int lvw;
+                asm("s_and_b32 %0, %1, 0xffff" : "=s"(lvw) : "s"(h.lvl_oct) : "scc");
+                const double hw = ldexp(E, -lvw);
                 bool take = accept_cell(hw, d0x * d0x + d0y * d0y + d0z * d0z, theta, act);
                 if (nearf) {
-                    const int lv = h.level();
+                    const int lv = lvw;
                     const double reach = hw + margin;
                     const bool close = fabs(d0x) < reach && fabs(d0y) < reach &&
                                        (DIM == 2 || fabs(d0z) < reach);
