@@ -1626,10 +1626,11 @@ __device__ __forceinline__ void bucket_sort_regs(uint64_t* __restrict__ keys,
 // the slot within the bin's. Each slot so holds one
 // contiguous key range, the slots in key order, a key or two each. One LDS
 // atomic per key counts its slot and returns its rank there; the scanned
-// counts place every key; each thread then insertion-sorts the keys of its
-// four slots (a few, all in place but within their slot), and a slot of more
-// than kSlotRun keys is sorted by a wave (one key per lane). Equal keys end in
-// no fixed order, as in the bitonic sort. Returns false, with the bucket
+// counts place every key; in a slot of up to kSlotRun keys each key then
+// counts the slot's keys below it (independent LDS reads, where a per-thread
+// insertion sort made a chain of dependent ones) and moves to that rank, and
+// a slot of more than kSlotRun keys is sorted by a wave (one key per lane).
+// Equal keys end in no fixed order, as in the bitonic sort. Returns false, with the bucket
 // untouched, when a slot holds more than 64 keys (clustered keys: the bitonic
 // sort takes the bucket).
 constexpr int kSlotBits = 11;
@@ -1799,7 +1800,7 @@ __device__ __forceinline__ bool bucket_sort_slots(uint64_t* __restrict__ keys,
         }
     uint32_t all;
     const uint32_t s0 = block_exclusive_scan<kBucketThreads>(c0 + c1 + c2 + c3, s_scan, &all);
-    const uint32_t s1 = s0 + c0, s2 = s1 + c1, s3 = s2 + c2, s4 = s3 + c3;
+    const uint32_t s1 = s0 + c0, s2 = s1 + c1, s3 = s2 + c2;
     // (the scan's barriers ordered the wide-slot count)
     if (s_wide[0] > kWideSlots) return false;
     s_pos[2 * t] = s0 | (s1 << 16);
@@ -1816,21 +1817,38 @@ __device__ __forceinline__ bool bucket_sort_slots(uint64_t* __restrict__ keys,
         }
     }
     __syncthreads();
-    const uint32_t ss[5] = {s0, s1, s2, s3, s4};
+    // a slot of 2 .. kSlotRun keys: each key's place is its rank among the
+    // slot's keys, equal keys keeping their arrival order (a stable sort);
+    // the ranks are read first (independent loads), then every key moves
+    uint32_t dst[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (ss[k + 1] - ss[k] > kSlotRun) continue;  // a wave's
-        for (uint32_t a = ss[k] + 1u; a < ss[k + 1]; ++a) {
-            const uint64_t x = s_k[a];
-            const uint32_t xv = s_v[a];
-            uint32_t b = a;
-            while (b > ss[k] && s_k[b - 1] > x) {
-                s_k[b] = s_k[b - 1];
-                s_v[b] = s_v[b - 1];
-                --b;
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t i = (uint32_t)e * kBucketThreads + t;
+        dst[e] = ~0u;
+        if (i < cnt) {
+            const uint32_t sl = slot[e];
+            const uint32_t a = (s_pos[sl >> 1] >> ((sl & 1u) * 16u)) & 0xFFFFu;
+            const uint32_t z = sl + 1u < kSlots ? (s_pos[(sl + 1u) >> 1] >> (((sl + 1u) & 1u) * 16u)) & 0xFFFFu : cnt;
+            const uint32_t c = z - a, p = a + rank[e];
+            if (c > 1u && c <= kSlotRun) {
+                uint32_t r = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < kSlotRun; ++j) {
+                    if (j < c) {
+                        const uint64_t y = s_k[a + j];
+                        r += (y < key[e] || (y == key[e] && a + j < p)) ? 1u : 0u;
+                    }
+                }
+                dst[e] = a + r;
             }
-            s_k[b] = x;
-            s_v[b] = xv;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if (dst[e] != ~0u) {
+            s_k[dst[e]] = key[e];
+            s_v[dst[e]] = val[e];
         }
     }
     const uint32_t nwide = s_wide[0];
