@@ -1092,13 +1092,21 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_scatter(
         c0 += x0;
         c1 += x1;
     }
+    // the tile's ranges in its buckets, reserved before the scans so that the
+    // atomics' round trip overlaps them (each result is used only under its
+    // own condition: no wait where the branches join)
+    uint32_t f0, f1;
+    if (c0) f0 = atomicAdd(&bfill[b0], c0);
+    if (c1) f1 = atomicAdd(&bfill[b1], c1);
     uint32_t dummy;
     const uint32_t l0 = block_exclusive_scan<kSortThreads>(c0 + c1, s_scan, &dummy);
     const uint32_t g0 = block_exclusive_scan<kSortThreads>(n0 + n1, s_scan, &dummy);
     s_lstart[b0] = (uint16_t)l0;
     s_lstart[b1] = (uint16_t)(l0 + c0);
-    s_dst[b0] = c0 ? g0 + atomicAdd(&bfill[b0], c0) : 0u;
-    s_dst[b1] = c1 ? g0 + n0 + atomicAdd(&bfill[b1], c1) : 0u;
+    s_dst[b0] = 0u;
+    s_dst[b1] = 0u;
+    if (c0) s_dst[b0] = g0 + f0;
+    if (c1) s_dst[b1] = g0 + n0 + f1;
     if (blockIdx.x == 0) {
         if (b0 < nb) bbase[b0] = g0;
         if (b1 < nb) bbase[b1] = g0 + n0;
