@@ -71,7 +71,7 @@ __device__ __forceinline__ void gstore(T* p, const T& v) {
 }
 
 // floor(a * b / c) for a * b < 2^53 and c > 0 by one FP64 division (a 64-bit
-// integer division is a software loop of about 150 instructions). Both
+// integer division is a software sequence of about 150 instructions). Both
 // operands are exact in FP64; a quotient that is not an integer lies at
 // least 1/c from the integers around it, and the division's rounding moves
 // it by at most q 2^-53 = a b / c 2^-53 < 1/c, so its floor is exact. The
@@ -1638,9 +1638,9 @@ __device__ __forceinline__ void bucket_sort_regs(uint64_t* __restrict__ keys,
 // counts the slot's keys below it (independent LDS reads, where a per-thread
 // insertion sort made a chain of dependent ones) and moves to that rank, and
 // a slot of more than kSlotRun keys is sorted by a wave (one key per lane).
-// Equal keys end in no fixed order, as in the bitonic sort. Returns false, with the bucket
-// untouched, when a slot holds more than 64 keys (clustered keys: the bitonic
-// sort takes the bucket).
+// Equal keys end in no fixed order, as in the bitonic sort. Returns false,
+// with the bucket untouched, when a slot holds more than 64 keys (clustered
+// keys: the bitonic sort takes the bucket).
 constexpr int kSlotBits = 11;
 constexpr uint32_t kSlots = 1u << kSlotBits;
 constexpr uint32_t kSlotRun = 8;
